@@ -1,0 +1,226 @@
+/*
+ * cfd_solver.h -- C-ABI of the device-resident SIMPLE engine (MI355X, gfx950).
+ *
+ * The reference's face loops are C++ member functions of NavierStokesSolver
+ * and HeatConductionSolver (src/navier.h:87-117, src/HeatConductionSolver.h:
+ * 173-201, src/BasicType.h:139). This ABI keeps their names: each cfd_ns_call
+ * method runs the same computation as the member function of that name, over
+ * device-resident structure-of-arrays mirrors of the reference's Face/Cell/Bnd
+ * arrays and fields. Host arrays cross the boundary only through
+ * cfd_ns_field / cfd_ns_matrix (set-up, tests, output).
+ *
+ * Set-up mirrors main() (src/main.cpp:8-40): the raw mesh (what ReadGridFile
+ * produces, src/Preprocessor.cpp:13-146) goes through face construction
+ * (CreateFaces/FindFace, src/Preprocessor.cpp:181-299), the optional conjugate
+ * heat-transfer split (buildSolver/findCoupledBoundary, src/Preprocessor.cpp:
+ * 308-515), geometry (CellFaceInfo, src/BasicSolver.cpp:317-512), allocation
+ * (CheckAndAllocate, src/navier.cpp:493-583) and InitFlowField
+ * (src/navier.cpp:113-188).
+ *
+ * All functions return 0 on success and a negative code on error;
+ * cfd_last_error() gives the message. Nothing here calls exit().
+ */
+#ifndef SYSU_CFD_HEAT_AMD_CFD_SOLVER_H
+#define SYSU_CFD_HEAT_AMD_CFD_SOLVER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- mesh -- */
+
+/* Mesh as ReadGridFile leaves it (src/Preprocessor.cpp:13-146): every cell is
+ * an 8-vertex hexahedron (a tetrahedron a,b,c,d is stored as a,b,c,c,d,d,d,d,
+ * src/Preprocessor.cpp:99-108), every boundary face has 4 vertices (a triangle
+ * a,b,c is a,b,c,c, src/Preprocessor.cpp:72-78). Indices are 0-based. */
+typedef struct {
+    int nvrt, ncel, nbnd;
+    const double *vert;    /* [nvrt][3] */
+    const int *cell_vert;  /* [ncel][8] */
+    const int *cell_rid;   /* [ncel] region id (first gmsh tag) */
+    const int *bnd_vert;   /* [nbnd][4] */
+    const int *bnd_rid;    /* [nbnd] region id (first gmsh tag) */
+} cfd_raw_mesh;
+
+/* Synthetic box generator (no reference counterpart: it replaces the gmsh
+ * files the reference reads). nx*ny*nz hexahedra over [0,lx]x[0,ly]x[0,lz];
+ * tets!=0 splits each into the 6 Kuhn tetrahedra sharing the (0,0,0)-(1,1,1)
+ * diagonal. Interior vertices move by U(-jitter*h, +jitter*h) per axis from a
+ * splitmix64 stream seeded with `seed`. Boundary faces are listed side by side
+ * (x-, x+, y-, y+, z-, z+) with region ids side_rid[0..5]. Cells whose hex
+ * centre lies inside block[6] = {x0,x1,y0,y1,z0,z1} get solid_rid, the others
+ * fluid_rid; block may be NULL. Arrays are caller-allocated with the sizes
+ * from cfd_mesh_box_counts. */
+int cfd_mesh_box_counts(int nx, int ny, int nz, int tets,
+                        int *nvrt, int *ncel, int *nbnd);
+int cfd_mesh_box(int nx, int ny, int nz, double lx, double ly, double lz,
+                 int tets, double jitter, uint64_t seed,
+                 const int side_rid[6], int fluid_rid, int solid_rid,
+                 const double *block,
+                 double *vert, int *cell_vert, int *cell_rid,
+                 int *bnd_vert, int *bnd_rid);
+
+/* gmsh 2.2 ASCII in the subset ReadGridFile accepts (src/Preprocessor.cpp:
+ * 26-31: no $PhysicalNames; element types 2-7; 2 tags; %.17g coordinates). */
+int cfd_gmsh_write(const char *path, const cfd_raw_mesh *m);
+int cfd_gmsh_read_counts(const char *path, int *nvrt, int *ncel, int *nbnd);
+int cfd_gmsh_read(const char *path, double *vert, int *cell_vert,
+                  int *cell_rid, int *bnd_vert, int *bnd_rid);
+
+/* Connectivity + geometry of one (sub)mesh, host side. */
+typedef struct cfd_mesh_s cfd_mesh;
+cfd_mesh *cfd_mesh_build(const cfd_raw_mesh *m);
+void cfd_mesh_free(cfd_mesh *h);
+/* counts[0..4] = Nvrt, Ncel, Nfac, Nbnd, NCoupledBnd */
+int cfd_mesh_counts(const cfd_mesh *h, int counts[5]);
+/* Named arrays, reference layout (row-major [N][k]):
+ *   int:    face_vert[F][4] face_cell1 face_cell2 face_bnd cell_nface
+ *           cell_face[C][6] cell_cell[C][6] cell_vert[C][8] cell_rid
+ *           bnd_face bnd_rid cell_colour
+ *   double: vert[V][3] face_x[F][3] face_n[F][3] face_area face_lambda
+ *           face_rlencos face_xpac[F][3] face_xnac[F][3] cell_x[C][3]
+ *           cell_vol bnd_distance
+ * Unused slots (cell_face/cell_cell beyond nface) hold -1. */
+int cfd_mesh_get_int(const cfd_mesh *h, const char *name, int *out);
+int cfd_mesh_get_dbl(const cfd_mesh *h, const char *name, double *out);
+
+/* --------------------------------------------------------- parameters -- */
+
+#define CFD_MAX_REGIONS 16
+
+/* BdRegion (src/BasicType.h:69-95). type1: 1 wall, 2 inlet, 3 outlet,
+ * 4 symmetric, 5 body. type2 (wall): 0 given T, 1 given flux, 2 coupled;
+ * (body): 0 fluid, 1 solid. init[]: inlet u,v,w,p,r,t,te,ed
+ * (src/readparamfile.cpp:182-194); fluid body u,v,w,ro,t,visLam,te,ed;
+ * solid body t,ro,diffCoef (src/readparamfile.cpp:206-229). */
+typedef struct {
+    int id;
+    int type1;
+    int type2;
+    double fixed;
+    double init[10];
+} cfd_region;
+
+enum { CFD_CG = 0, CFD_BICGSTAB = 1, CFD_GMRES = 2 };
+enum { CFD_PC_NONE = 0, CFD_PC_JACOBI = 1, CFD_PC_SSOR = 2, CFD_PC_ILU = 3 };
+
+/* One linear-solve configuration: SolveLinearEqu arguments
+ * (src/tools.cpp:351-359). */
+typedef struct {
+    int method;
+    int precond;
+    double omega;
+    double tol;
+    int maxit;
+} cfd_solve_cfg;
+
+typedef struct {
+    int nregion;
+    cfd_region region[CFD_MAX_REGIONS];
+    int steady;            /* IfSteady */
+    int max_step;          /* MaxStep: steady runs MaxStep-1 outer iterations */
+    double residual_steady;/* ResidualSteady */
+    int solve_energy;      /* SolveEnergy */
+    int turb_model;        /* TurModel: 0 laminar, 1 k-epsilon */
+    int density_model;     /* DensityModel: only 0 (constant) is supported */
+    int cht;               /* Solve3DHeatConduction */
+    double gravity[3];
+    double pressure_ref;   /* PressureReference */
+    int cell_pressure_ref; /* cellPressureRef (0-based) */
+    double urf[8];         /* URF: u,v,w,p,T,scalar(k,eps,T),e,species */
+    int limiter;           /* 0 none, 1 Barth, 2 MLP, 3 WENO */
+    double prl, prte, cp;  /* laminar/turbulent Prandtl, cp */
+    cfd_solve_cfg mom;     /* momentum u,v,w */
+    cfd_solve_cfg pres;    /* pressure correction */
+    cfd_solve_cfg ke;      /* k and epsilon */
+    cfd_solve_cfg energy;  /* fluid T */
+    cfd_solve_cfg solid;   /* solid T */
+    double solid_heat_source; /* extension: volumetric heat in solid cells
+                                 (the reference hard-codes 0,
+                                 src/HeatConductionSolver.cpp:60-62) */
+    int pres_precision;    /* 0: FP64 pressure solve; 1: FP32 inner Krylov
+                              with FP64 iterative refinement */
+    int multi_rhs;         /* 1: solve u,v,w together (one launch per step) */
+    int pres_project;      /* 1: remove the constant null-space component of
+                              the pure-Neumann pressure system */
+} cfd_params;
+
+/* Reference defaults: constructor values (src/navier.cpp:354-416), the
+ * solver settings hard-coded at the call sites (momentum GMRES+SSOR(1.2)
+ * 1e-8/500, pressure CG+SSOR(1.3) 1e-6/1000, k/eps/T GMRES+SSOR(1.3) 1e-8/500,
+ * solid CG+SSOR(1.3) 1e-8/500), no regions. */
+void cfd_params_default(cfd_params *p);
+
+/* One part of the conjugate-heat-transfer split that cfd_ns_create performs
+ * (buildSolver + findCoupledBoundary, src/Preprocessor.cpp:441-515): part 0
+ * = fluid cells, 1 = solid cells, with the coupled region id chosen as the
+ * reference does (first unused id counting from 0), plus its geometry.
+ * Host only (no device needed). */
+cfd_mesh *cfd_mesh_build_part(const cfd_raw_mesh *m, const cfd_params *p, int part);
+
+/* ------------------------------------------------------------- solver -- */
+
+typedef struct cfd_ns_s cfd_ns;
+
+cfd_ns *cfd_ns_create(const cfd_raw_mesh *m, const cfd_params *p);
+void cfd_ns_free(cfd_ns *h);
+/* counts of fluid (part 0) or solid (part 1) mesh: Nvrt,Ncel,Nfac,Nbnd,NCoupledBnd */
+int cfd_ns_counts(const cfd_ns *h, int part, int counts[5]);
+
+/* Run one named method on the device, same computation as the reference's
+ * member function of that name:
+ *   "NSSolve" (honours max_step/residual_steady), "Iterate" (one outer
+ *   iteration of NSSolve's loop body), "CalculateVelocity",
+ *   "BuildVelocityMatrix", "CalRUFace2", "CalculatePressure",
+ *   "BuildPressureMatrix", "CorrectRUFace2" (dp = xsol), "SetBCVelocity",
+ *   "SetBCPressure", "SetBCDeltaP" (dp = xsol), "SetBCTemperature",
+ *   "UpdateTurKEpsilon", "UpdateEnergy", "SolidSolve", "InitFlowField",
+ *   "GradientU"/"GradientV"/"GradientW"/"GradientP" (dUdX.. from Un..,BU..),
+ *   "GradientT" (dPhidX from Tn,BTem), "SolidGradientT".
+ * Blocks until the device work is done. */
+int cfd_ns_call(cfd_ns *h, const char *method);
+
+/* Host <-> device copy of a named field. part 0 = fluid, 1 = solid.
+ * dir 0: device -> host (get), 1: host -> device (set). count = doubles.
+ * Fluid names: Rn Un Vn Wn Pn Tn TE ED VisLam VisTur Apr RUFace
+ *   dPdX dUdX dVdX dWdX dPhidX ([Ncel][3]) BRo BU BV BW BPre BTem BTE BED
+ *   BndQ xsol bu bv bw bp bs Residual(10)
+ * Solid names: Tn Rn diffCoef dPhidX BTem BndQ xsol bs */
+int cfd_ns_field(cfd_ns *h, int part, const char *name, double *host,
+                 size_t count, int dir);
+
+/* Assembled matrix in LASPack row form: per row the diagonal entry first,
+ * then the off-diagonal entries in the order Q_SetEntry received them
+ * (upper triangle only for the symmetric pressure matrix).
+ * which: "As" (last non-symmetric build), "Ap" (pressure), "SolidAs".
+ * Call with row_len only (col/val NULL) to get lengths; total = sum. */
+int cfd_ns_matrix(cfd_ns *h, const char *which, int *row_len, int *col,
+                  double *val);
+
+/* Last linear solve: iterations and true FP64 relative residual. */
+int cfd_ns_last_solve(const cfd_ns *h, const char *eq, int *iters,
+                      double *relres);
+
+/* Per-outer-iteration totals since creation (or the last reset):
+ * out[0] outer iterations, out[1] device seconds, out[2..6] Krylov iterations
+ * of momentum(u+v+w), pressure, k+eps, fluid T, solid T. */
+int cfd_ns_stats(const cfd_ns *h, double out[8]);
+int cfd_ns_reset_stats(cfd_ns *h);
+
+/* Live HIP-event timing of the pressure CG's SpMV kernel (the dominant
+ * kernel of the step), recorded on the solver's own stream around each
+ * launch. enable resets the counters. out = {launches that did work,
+ * total ms, algorithmic bytes per launch, stored off-diagonals}. */
+int cfd_ns_profile(cfd_ns *h, int enable);
+int cfd_ns_profile_read(const cfd_ns *h, double out[4]);
+
+const char *cfd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SYSU_CFD_HEAT_AMD_CFD_SOLVER_H */

@@ -1,0 +1,160 @@
+/*
+ * laspack.h -- LASPack-compatible C-ABI, MI355X (gfx950) backend.
+ *
+ * Drop-in for the external LASPack library that the reference links as
+ * lib/liblaspack.a (makefile:12,19-23) and includes as "laspack/laspack.h"
+ * (src/BasicType.h:11-13, src/tools.h:182-184, src/navier.h:6-8).
+ * Every symbol the reference binds is declared here with LASPack's call shape:
+ *
+ *   V_Constr / V_Destr / V_SetAllCmp   src/navier.cpp:568-573, 484-489;
+ *                                      src/navier_pressure.cpp:21;
+ *                                      src/HeatConductionSolver.cpp:262-263
+ *   Q_Constr / Q_Destr                 src/navier_velocity.cpp:19,73;
+ *                                      src/navier_pressure.cpp:14-18,74;
+ *                                      src/navier_scalar.cpp:95,113,126,151,262,391;
+ *                                      src/HeatConductionSolver.cpp:52,105
+ *   Q_SetLen / Q_SetEntry              src/navier_velocity.cpp:340-345;
+ *                                      src/navier_pressure.cpp:186-192;
+ *                                      src/navier_scalar.cpp:135-136,634-639;
+ *                                      src/HeatConductionSolver.cpp:210-215
+ *   CGIter / GMRESIter                 src/navier_pressure.cpp:29,31;
+ *                                      src/navier_velocity.cpp:26,41,53;
+ *                                      src/navier_scalar.cpp:101,143,374;
+ *                                      src/HeatConductionSolver.cpp:90
+ *   SSORPrecond                        same call sites (preconditioner argument)
+ *   SetRTCAccuracy / GetLastNoIter /
+ *   GetLastAccuracy                    src/tools.cpp:355-358 (SolveLinearEqu)
+ *   BiCGSTABIter / JacobiPrecond /
+ *   ILUPrecond                         named at src/tools.cpp:347-349
+ *
+ * Semantics kept from LASPack:
+ *   - Vector components are 1-based, host-addressable through Cmp[1..Dim]
+ *     (callers read and write them directly, e.g. src/navier_velocity.cpp:24-34).
+ *   - QMatrix rows and columns are 1-based; Symmetry=True stores the upper
+ *     triangle only (src/navier_pressure.cpp:15,158-163); a row may be
+ *     re-Q_SetLen'd to shrink it (src/navier_scalar.cpp:135-136).
+ *   - Solvers return x; the last iteration count and accuracy are process
+ *     globals read through GetLastNoIter / GetLastAccuracy.
+ *
+ * Differences (the GPU backend is not bit-identical to LASPack's iterates):
+ *   - GetLastAccuracy reports the TRUE relative residual ||b - A x||_2/||b||_2
+ *     recomputed in FP64 on the device after the solve.
+ *   - SSORPrecond and ILUPrecond are applied in a deterministic greedy
+ *     multicolour order (first-fit over the matrix graph in natural row
+ *     order), which is what makes the triangular sweeps parallel; LASPack's
+ *     natural-order sweeps give a different preconditioner of the same family.
+ *   - GMRESIter is restarted GMRES(10) with right preconditioning.
+ *   - Errors never call exit(): they set LASResult() and leave x unchanged.
+ *   - The interface is not re-entrant (as LASPack): one stream, one thread.
+ */
+#ifndef SYSU_CFD_HEAT_AMD_LASPACK_H
+#define SYSU_CFD_HEAT_AMD_LASPACK_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum { False = 0, True = 1 } Boolean;
+typedef enum { Normal = 0, Tempor = 1 } InstanceType;
+typedef enum { Rowws = 0, Clmws = 1 } ElOrderType;
+
+typedef enum {
+    LASOK = 0,
+    LASMemAllocErr,
+    LASLValErr,
+    LASDimErr,
+    LASRangeErr,
+    LASSymStorErr,
+    LASMatrCombErr,
+    LASMulInvErr,
+    LASElNotSortedErr,
+    LASZeroInDiagErr,
+    LASZeroPivotErr,
+    LASILUStructErr,
+    LASBreakdownErr,
+    LASDeviceErr,
+    LASUserErr
+} LASErrIdType;
+
+/* Vector: LASPack field names, so code that touches Cmp compiles unchanged. */
+typedef struct {
+    char *Name;
+    size_t Dim;
+    InstanceType Instance;
+    int LockLevel;
+    double Multipl;
+    Boolean OwnData;
+    double *Cmp;          /* host, 1-based: Cmp[1..Dim] */
+    void *Dev;            /* backend-private device mirror */
+} Vector;
+
+/* QMatrix: only touched through Q_* calls by the reference; fields are
+ * backend-private except Name/Dim/Symmetry. */
+typedef struct {
+    char *Name;
+    size_t Dim;
+    Boolean Symmetry;
+    ElOrderType ElOrder;
+    InstanceType Instance;
+    int LockLevel;
+    Boolean OwnData;
+    size_t *Len;          /* entries per row (1-based rows) */
+    size_t *Cap;          /* allocated entries per row */
+    size_t **Col;         /* 1-based column of each entry */
+    double **Val;
+    int Dirty;            /* host rows changed since the last upload */
+    void *Dev;            /* backend-private device matrix */
+} QMatrix;
+
+typedef Vector *(*PrecondProcType)(QMatrix *, Vector *, Vector *, double);
+typedef Vector *(*IterProcType)(QMatrix *, Vector *, Vector *, int,
+                                PrecondProcType, double);
+
+/* vector.h */
+void V_Constr(Vector *V, char *Name, size_t Dim, InstanceType Instance,
+              Boolean OwnData);
+void V_Destr(Vector *V);
+void V_SetAllCmp(Vector *V, double Val);
+size_t V_GetDim(Vector *V);
+
+/* qmatrix.h */
+void Q_Constr(QMatrix *Q, char *Name, size_t Dim, Boolean Symmetry,
+              ElOrderType ElOrder, InstanceType Instance, Boolean OwnData);
+void Q_Destr(QMatrix *Q);
+void Q_SetLen(QMatrix *Q, size_t RoC, size_t Len);
+void Q_SetEntry(QMatrix *Q, size_t RoC, size_t Entry, size_t Pos, double Val);
+size_t Q_GetDim(QMatrix *Q);
+
+/* itersolv.h: x is returned; MaxIter counts iterations as LASPack does */
+Vector *CGIter(QMatrix *A, Vector *x, Vector *b, int MaxIter,
+               PrecondProcType PrecondProc, double OmegaPrecond);
+Vector *BiCGSTABIter(QMatrix *A, Vector *x, Vector *b, int MaxIter,
+                     PrecondProcType PrecondProc, double OmegaPrecond);
+Vector *GMRESIter(QMatrix *A, Vector *x, Vector *b, int MaxIter,
+                  PrecondProcType PrecondProc, double OmegaPrecond);
+void SetGMRESRestart(int MaxSteps);
+
+/* precond.h: y = M^{-1} c, callable directly and usable as PrecondProc */
+Vector *JacobiPrecond(QMatrix *A, Vector *y, Vector *c, double Omega);
+Vector *SSORPrecond(QMatrix *A, Vector *y, Vector *c, double Omega);
+Vector *ILUPrecond(QMatrix *A, Vector *y, Vector *c, double Omega);
+
+/* rtc.h */
+void SetRTCAccuracy(double Eps);
+int GetLastNoIter(void);
+double GetLastAccuracy(void);
+
+/* errhandl.h: LASResult() is the first error since the last clear;
+ * LASBreak() is True while an error is pending. LASClearResult() (no
+ * LASPack counterpart) resets the error state. */
+LASErrIdType LASResult(void);
+Boolean LASBreak(void);
+void LASClearResult(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SYSU_CFD_HEAT_AMD_LASPACK_H */

@@ -1,0 +1,104 @@
+// ORACLE -- TEST INFRASTRUCTURE ONLY.
+// CPU restatement of the reference's SIMPLE assembly + LASPack solve path
+// (Meiyim/SYSU_CFD_HEAT). Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it, and only as the checker / CPU baseline. The
+// product (sysu_cfd_heat_amd) never links or calls it.
+//
+// Each function cites the reference file:line it restates. Operation order
+// follows the reference (compiled with -ffp-contract=off) so that integer
+// results are identical and FP64 results are bit-reproducible.
+//
+// Parity pins (see tests/): closed forms (unit hexahedra, uniform grids,
+// linear fields), invariants (closed-cell area sums, symmetric/zero-row-sum
+// pressure matrix, mass conservation after correction, proper colouring),
+// brute force (set-based face matching, dense LAPACK solves).
+#pragma once
+#include <array>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "cfd_solver.h"
+
+namespace oref {
+
+// FaceData / CellData / BoundaryData (src/BasicType.h:22-67)
+struct Face {
+    int bnd = -10;
+    int v[4] = {0, 0, 0, 0};
+    int c1 = -10, c2 = -10;
+    double x[3] = {0, 0, 0}, n[3] = {0, 0, 0}, area = 0;
+    double lambda = 0, rlencos = 0;
+    double xpac[3] = {0, 0, 0}, xnac[3] = {0, 0, 0};
+};
+struct Cell {
+    int nface = 0, rid = 0;
+    int face[6] = {-1, -1, -1, -1, -1, -1};
+    int cell[6] = {-1, -1, -1, -1, -1, -1};
+    int v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double vol = 0, x[3] = {0, 0, 0};
+};
+struct Bnd {
+    int face = -1;
+    int v[4] = {-1, -1, -1, -1};
+    int rid = -1;
+    double distance = 0, yplus = 0, uplus = 0, shear[3] = {0, 0, 0}, h = 0,
+           q = 0;
+};
+struct Region {
+    int type1 = -1, type2 = -1;
+    double fixed = 2147483647.0;  // CYCASHUGE_I (src/BasicType.h:92)
+    double init[10] = {0};
+};
+
+struct Mesh {
+    int nvrt = 0, ncel = 0, nfac = 0, nbnd = 0, ncb = 0;
+    std::vector<std::array<double, 3>> vert;
+    std::vector<Cell> cell;
+    std::vector<Face> face;
+    std::vector<Bnd> bnd;
+};
+
+// mesh
+void load_raw(const cfd_raw_mesh &m, Mesh &out);
+int create_faces(Mesh &m);                 // src/Preprocessor.cpp:181-299
+int cell_face_info(Mesh &m);               // src/BasicSolver.cpp:317-512
+int split_region(const Mesh &full, const std::vector<int> &new2old,
+                 int coupled_id, Mesh &out);  // src/Preprocessor.cpp:308-438
+std::vector<int> greedy_colour(int n, const std::vector<std::vector<int>> &adj);
+
+// LASPack restatement (rows with 0-based columns; entry 0 is whatever the
+// caller set first, the reference always sets the diagonal there)
+struct QMat {
+    int dim = 0;
+    bool sym = false;
+    std::vector<std::vector<std::pair<int, double>>> rows;
+    void constr(int n, bool s) {
+        dim = n;
+        sym = s;
+        rows.assign(n, {});
+    }
+    void set_len(int r, int len) { rows[r].resize(len, {r, 0.0}); }
+    void set_entry(int r, int e, int c, double v) { rows[r][e] = {c, v}; }
+};
+
+struct SolveCfg {
+    int method = CFD_CG, precond = CFD_PC_NONE;
+    double omega = 1.0, tol = 1e-8;
+    int maxit = 500;
+    int mc = 0;  // 1: multicolour order for SSOR / ILU (the GPU's order)
+    const std::vector<int> *colour = nullptr;  // mc colouring; null: from A
+};
+struct SolveResult {
+    int iters = 0;
+    double relres = 0;   // true ||b-Ax||/||b|| at exit
+    double rtcres = 0;   // the residual the stopping test saw
+};
+SolveResult las_solve(const QMat &A, std::vector<double> &x,
+                      const std::vector<double> &b, const SolveCfg &cfg);
+void las_mul(const QMat &A, const double *x, double *y);
+void las_precond(const QMat &A, int precond, double omega, int mc,
+                 const double *c, double *y);
+
+}  // namespace oref

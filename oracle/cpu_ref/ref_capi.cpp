@@ -1,0 +1,315 @@
+// ORACLE -- TEST INFRASTRUCTURE ONLY (see ref.h).
+// C entry points for tests / bench cpu_baseline (ctypes, oracle/oracle.py).
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "ref_navier.h"
+
+using namespace oref;
+
+static std::string g_err;
+
+static int mesh_get_int(const Mesh &M, const char *name, int *o) {
+    std::string s(name);
+    if (s == "face_vert") {
+        for (int i = 0; i < M.nfac; ++i) for (int k = 0; k < 4; ++k) o[4 * i + k] = M.face[i].v[k];
+    } else if (s == "face_cell1") {
+        for (int i = 0; i < M.nfac; ++i) o[i] = M.face[i].c1;
+    } else if (s == "face_cell2") {
+        for (int i = 0; i < M.nfac; ++i) o[i] = M.face[i].c2;
+    } else if (s == "face_bnd") {
+        for (int i = 0; i < M.nfac; ++i) o[i] = M.face[i].bnd;
+    } else if (s == "cell_nface") {
+        for (int i = 0; i < M.ncel; ++i) o[i] = M.cell[i].nface;
+    } else if (s == "cell_face" || s == "cell_cell") {
+        bool f = s == "cell_face";
+        for (int i = 0; i < M.ncel; ++i)
+            for (int k = 0; k < 6; ++k)
+                o[6 * i + k] = k < M.cell[i].nface ? (f ? M.cell[i].face[k] : M.cell[i].cell[k]) : -1;
+    } else if (s == "cell_vert") {
+        for (int i = 0; i < M.ncel; ++i) for (int k = 0; k < 8; ++k) o[8 * i + k] = M.cell[i].v[k];
+    } else if (s == "cell_rid") {
+        for (int i = 0; i < M.ncel; ++i) o[i] = M.cell[i].rid;
+    } else if (s == "bnd_face") {
+        for (int i = 0; i < M.nbnd; ++i) o[i] = M.bnd[i].face;
+    } else if (s == "bnd_rid") {
+        for (int i = 0; i < M.nbnd; ++i) o[i] = M.bnd[i].rid;
+    } else {
+        g_err = "unknown int array " + s;
+        return -1;
+    }
+    return 0;
+}
+
+static int mesh_get_dbl(const Mesh &M, const char *name, double *o) {
+    std::string s(name);
+    auto f3 = [&](auto get) {
+        for (int i = 0; i < M.nfac; ++i) for (int k = 0; k < 3; ++k) o[3 * i + k] = get(M.face[i])[k];
+    };
+    if (s == "vert") {
+        for (int i = 0; i < M.nvrt; ++i) for (int k = 0; k < 3; ++k) o[3 * i + k] = M.vert[i][k];
+    } else if (s == "face_x") {
+        f3([](const Face &F) { return F.x; });
+    } else if (s == "face_n") {
+        f3([](const Face &F) { return F.n; });
+    } else if (s == "face_xpac") {
+        f3([](const Face &F) { return F.xpac; });
+    } else if (s == "face_xnac") {
+        f3([](const Face &F) { return F.xnac; });
+    } else if (s == "face_area") {
+        for (int i = 0; i < M.nfac; ++i) o[i] = M.face[i].area;
+    } else if (s == "face_lambda") {
+        for (int i = 0; i < M.nfac; ++i) o[i] = M.face[i].lambda;
+    } else if (s == "face_rlencos") {
+        for (int i = 0; i < M.nfac; ++i) o[i] = M.face[i].rlencos;
+    } else if (s == "cell_x") {
+        for (int i = 0; i < M.ncel; ++i) for (int k = 0; k < 3; ++k) o[3 * i + k] = M.cell[i].x[k];
+    } else if (s == "cell_vol") {
+        for (int i = 0; i < M.ncel; ++i) o[i] = M.cell[i].vol;
+    } else if (s == "bnd_distance") {
+        for (int i = 0; i < M.nbnd; ++i) o[i] = M.bnd[i].distance;
+    } else {
+        g_err = "unknown double array " + s;
+        return -1;
+    }
+    return 0;
+}
+
+static void counts_of(const Mesh &M, int c[5]) {
+    c[0] = M.nvrt; c[1] = M.ncel; c[2] = M.nfac; c[3] = M.nbnd; c[4] = M.ncb;
+}
+
+struct OracleNS {
+    RefNS ns;
+};
+
+extern "C" {
+
+const char *oracle_last_error(void) { return g_err.c_str(); }
+
+// Face construction + geometry of one mesh (no split).
+void *oracle_mesh_build(const cfd_raw_mesh *raw) {
+    Mesh *m = new Mesh;
+    load_raw(*raw, *m);
+    if (create_faces(*m)) { g_err = "face construction failed"; delete m; return nullptr; }
+    m->ncb = m->nbnd;
+    int rc = cell_face_info(*m);
+    if (rc) { g_err = "geometry check failed " + std::to_string(rc); delete m; return nullptr; }
+    return m;
+}
+void oracle_mesh_free(void *h) { delete (Mesh *)h; }
+int oracle_mesh_counts(void *h, int c[5]) { counts_of(*(Mesh *)h, c); return 0; }
+int oracle_mesh_get_int(void *h, const char *n, int *o) { return mesh_get_int(*(Mesh *)h, n, o); }
+int oracle_mesh_get_dbl(void *h, const char *n, double *o) { return mesh_get_dbl(*(Mesh *)h, n, o); }
+
+void *oracle_ns_create(const cfd_raw_mesh *raw, const cfd_params *p, int mc) {
+    OracleNS *o = new OracleNS;
+    o->ns.mc = mc;
+    if (o->ns.setup(*raw, *p)) { g_err = o->ns.error; delete o; return nullptr; }
+    return o;
+}
+void oracle_ns_free(void *h) { delete (OracleNS *)h; }
+
+int oracle_ns_counts(void *h, int part, int c[5]) {
+    RefNS &ns = ((OracleNS *)h)->ns;
+    if (part == 1) {
+        if (!ns.ht) return -1;
+        counts_of(ns.ht->M, c);
+    } else {
+        counts_of(ns.M, c);
+    }
+    return 0;
+}
+int oracle_ns_mesh_int(void *h, int part, const char *n, int *o) {
+    RefNS &ns = ((OracleNS *)h)->ns;
+    if (part == 1 && !ns.ht) return -1;
+    return mesh_get_int(part == 1 ? ns.ht->M : ns.M, n, o);
+}
+int oracle_ns_mesh_dbl(void *h, int part, const char *n, double *o) {
+    RefNS &ns = ((OracleNS *)h)->ns;
+    if (part == 1 && !ns.ht) return -1;
+    return mesh_get_dbl(part == 1 ? ns.ht->M : ns.M, n, o);
+}
+
+int oracle_ns_call(void *h, const char *method) {
+    RefNS &ns = ((OracleNS *)h)->ns;
+    std::string m(method);
+    int rc = 0;
+    if (m == "NSSolve") { int it; rc = ns.NSSolve(it); }
+    else if (m == "Iterate") { double r; rc = ns.Iterate(r); }
+    else if (m == "CalculateVelocity") rc = ns.CalculateVelocity();
+    else if (m == "BuildVelocityMatrix") rc = ns.BuildVelocityMatrix();
+    else if (m == "CalRUFace2") rc = ns.CalRUFace2();
+    else if (m == "CalculatePressure") rc = ns.CalculatePressure();
+    else if (m == "BuildPressureMatrix") rc = ns.BuildPressureMatrix();
+    else if (m == "CorrectRUFace2") rc = ns.CorrectRUFace2(ns.xsol.data());
+    else if (m == "SetBCVelocity") rc = ns.SetBCVelocity(ns.BRo.data(), ns.BU.data(), ns.BV.data(), ns.BW.data());
+    else if (m == "SetBCPressure") rc = ns.SetBCPressure(ns.BPre.data());
+    else if (m == "SetBCDeltaP") rc = ns.SetBCDeltaP(ns.BPre.data(), ns.xsol.data());
+    else if (m == "SetBCTemperature") rc = ns.SetBCTemperature(ns.BTem.data());
+    else if (m == "UpdateTurKEpsilon") rc = ns.UpdateTurKEpsilon();
+    else if (m == "UpdateEnergy") rc = ns.UpdateEnergy();
+    else if (m == "SolidSolve") rc = ns.ht ? ns.ht->UpdateEnergy() : -1;
+    else if (m == "InitFlowField") rc = ns.InitFlowField();
+    else if (m == "GradientU") rc = gradient(ns.M, ns.P.limiter, ns.Un.data(), ns.BU.data(), ns.dUdX.data());
+    else if (m == "GradientV") rc = gradient(ns.M, ns.P.limiter, ns.Vn.data(), ns.BV.data(), ns.dVdX.data());
+    else if (m == "GradientW") rc = gradient(ns.M, ns.P.limiter, ns.Wn.data(), ns.BW.data(), ns.dWdX.data());
+    else if (m == "GradientP") rc = gradient(ns.M, ns.P.limiter, ns.Pn.data(), ns.BPre.data(), ns.dPdX.data());
+    else if (m == "GradientT") rc = gradient(ns.M, ns.P.limiter, ns.Tn.data(), ns.BTem.data(), ns.dPhidX.data());
+    else if (m == "SolidGradientT") rc = ns.ht ? gradient(ns.ht->M, ns.P.limiter, ns.ht->Tn.data(), ns.ht->BTem.data(), ns.ht->dPhidX.data()) : -1;
+    else { g_err = "unknown method " + m; return -2; }
+    if (rc) g_err = ns.error.empty() ? ("method failed: " + m) : ns.error;
+    return rc;
+}
+
+static std::vector<double> *field_of(RefNS &ns, int part, const std::string &s,
+                                     double **raw, size_t *rawn) {
+    *raw = nullptr;
+    if (part == 1) {
+        if (!ns.ht) return nullptr;
+        RefHT &h = *ns.ht;
+        if (s == "Tn") return &h.Tn;
+        if (s == "Rn") return &h.Rn;
+        if (s == "diffCoef") return &h.diffCoef;
+        if (s == "dPhidX") return &h.dPhidX;
+        if (s == "BTem") return &h.BTem;
+        if (s == "xsol") return &h.xsol;
+        if (s == "bs") return &h.bs;
+        return nullptr;
+    }
+    if (s == "Residual") { *raw = ns.Residual; *rawn = 10; return nullptr; }
+    std::vector<double> *t[] = {&ns.Rn, &ns.Un, &ns.Vn, &ns.Wn, &ns.Pn, &ns.Tn, &ns.TE, &ns.ED,
+                                &ns.VisLam, &ns.VisTur, &ns.Apr, &ns.RUFace, &ns.dPdX, &ns.dUdX,
+                                &ns.dVdX, &ns.dWdX, &ns.dPhidX, &ns.BRo, &ns.BU, &ns.BV, &ns.BW,
+                                &ns.BPre, &ns.BTem, &ns.BTE, &ns.BED, &ns.xsol, &ns.bu, &ns.bv,
+                                &ns.bw, &ns.bp, &ns.bs};
+    const char *nm[] = {"Rn", "Un", "Vn", "Wn", "Pn", "Tn", "TE", "ED", "VisLam", "VisTur", "Apr",
+                        "RUFace", "dPdX", "dUdX", "dVdX", "dWdX", "dPhidX", "BRo", "BU", "BV",
+                        "BW", "BPre", "BTem", "BTE", "BED", "xsol", "bu", "bv", "bw", "bp", "bs"};
+    for (size_t k = 0; k < sizeof(nm) / sizeof(nm[0]); ++k)
+        if (s == nm[k]) return t[k];
+    return nullptr;
+}
+
+int oracle_ns_field(void *h, int part, const char *name, double *host, size_t count, int dir) {
+    RefNS &ns = ((OracleNS *)h)->ns;
+    std::string s(name);
+    if (s == "BndQ") {
+        Mesh &M = (part == 1 && ns.ht) ? ns.ht->M : ns.M;
+        if (count != (size_t)M.nbnd) { g_err = "size mismatch BndQ"; return -1; }
+        for (int i = 0; i < M.nbnd; ++i) {
+            if (dir == 0) host[i] = M.bnd[i].q;
+            else M.bnd[i].q = host[i];
+        }
+        return 0;
+    }
+    double *raw;
+    size_t rawn = 0;
+    std::vector<double> *v = field_of(ns, part, s, &raw, &rawn);
+    double *p = raw;
+    size_t n = rawn;
+    if (v) { p = v->data(); n = v->size(); }
+    if (!p) { g_err = "unknown field " + s; return -1; }
+    if (count != n) { g_err = "size mismatch for " + s; return -1; }
+    if (dir == 0) std::memcpy(host, p, sizeof(double) * n);
+    else std::memcpy(p, host, sizeof(double) * n);
+    return 0;
+}
+
+int oracle_ns_matrix(void *h, const char *which, int *row_len, int *col, double *val) {
+    RefNS &ns = ((OracleNS *)h)->ns;
+    std::string w(which);
+    QMat *A = w == "As" ? &ns.As : w == "Ap" ? &ns.Ap : (w == "SolidAs" && ns.ht) ? &ns.ht->As : nullptr;
+    if (!A) { g_err = "unknown matrix"; return -1; }
+    size_t k = 0;
+    for (int i = 0; i < A->dim; ++i) {
+        row_len[i] = (int)A->rows[i].size();
+        if (col)
+            for (auto &e : A->rows[i]) { col[k] = e.first; val[k] = e.second; ++k; }
+    }
+    return 0;
+}
+
+int oracle_ns_last_solve(void *h, const char *eq, int *iters, double *relres) {
+    RefNS &ns = ((OracleNS *)h)->ns;
+    std::string e(eq);
+    SolveResult r;
+    if (e == "u") r = ns.last_mom[0];
+    else if (e == "v") r = ns.last_mom[1];
+    else if (e == "w") r = ns.last_mom[2];
+    else if (e == "p") r = ns.last_pres;
+    else if (e == "k") r = ns.last_k;
+    else if (e == "eps") r = ns.last_e;
+    else if (e == "T") r = ns.last_t;
+    else if (e == "solid" && ns.ht) r = ns.ht->last;
+    else return -1;
+    *iters = r.iters;
+    *relres = r.relres;
+    return 0;
+}
+
+// Timed outer iterations (bench cpu_baseline): returns seconds, resmax.
+double oracle_ns_time_iterations(void *h, int n, double *resmax) {
+    RefNS &ns = ((OracleNS *)h)->ns;
+    auto t0 = std::chrono::steady_clock::now();
+    double r = 0;
+    for (int i = 0; i < n; ++i)
+        if (ns.Iterate(r)) return -1.0;
+    auto t1 = std::chrono::steady_clock::now();
+    if (resmax) *resmax = r;
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// LASPack-restatement solve of a row-form matrix (0-based columns).
+static void build_q(QMat &A, int n, int sym, const int *row_len, const int *col, const double *val) {
+    A.constr(n, sym != 0);
+    size_t k = 0;
+    for (int i = 0; i < n; ++i) {
+        A.set_len(i, row_len[i]);
+        for (int e = 0; e < row_len[i]; ++e, ++k) A.set_entry(i, e, col[k], val[k]);
+    }
+}
+
+int oracle_las_solve(int n, int sym, const int *row_len, const int *col, const double *val,
+                     const double *b, double *x, int method, int precond, double omega,
+                     double tol, int maxit, int mc, int *iters, double *relres) {
+    QMat A;
+    build_q(A, n, sym, row_len, col, val);
+    std::vector<double> xv(x, x + n), bv(b, b + n);
+    SolveCfg c;
+    c.method = method; c.precond = precond; c.omega = omega; c.tol = tol; c.maxit = maxit; c.mc = mc;
+    SolveResult r = las_solve(A, xv, bv, c);
+    std::memcpy(x, xv.data(), sizeof(double) * n);
+    *iters = r.iters;
+    *relres = r.relres;
+    return 0;
+}
+
+int oracle_las_mul(int n, int sym, const int *row_len, const int *col, const double *val,
+                   const double *x, double *y) {
+    QMat A;
+    build_q(A, n, sym, row_len, col, val);
+    las_mul(A, x, y);
+    return 0;
+}
+
+int oracle_las_precond(int n, int sym, const int *row_len, const int *col, const double *val,
+                       int precond, double omega, int mc, const double *c, double *y) {
+    QMat A;
+    build_q(A, n, sym, row_len, col, val);
+    las_precond(A, precond, omega, mc, c, y);
+    return 0;
+}
+
+// Greedy first-fit colouring of a graph in CSR adjacency form.
+int oracle_colour(int n, const int *ptr, const int *adj, int *colour) {
+    std::vector<std::vector<int>> a(n);
+    for (int i = 0; i < n; ++i) a[i].assign(adj + ptr[i], adj + ptr[i + 1]);
+    std::vector<int> c = greedy_colour(n, a);
+    std::memcpy(colour, c.data(), sizeof(int) * n);
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,397 @@
+// ORACLE -- TEST INFRASTRUCTURE ONLY (see ref.h).
+//
+// Restatement of the LASPack solve boundary the reference calls through
+// SolveLinearEqu (src/tools.cpp:351-359). LASPack itself (T. Skalicky,
+// TU Dresden; the makefile builds it from ./src/laspack/*.c, makefile:12,19-23)
+// is not vendored and no version is pinned, so its published algorithms are
+// restated from Barrett et al., "Templates for the Solution of Linear
+// Systems" (SIAM 1994), which LASPack follows:
+//   CG (Fig. 2.5), BiCGSTAB (Fig. 2.10), restarted GMRES(m) (Fig. 2.6; here
+//   with right preconditioning and classical Gram-Schmidt applied twice),
+//   Jacobi, SSOR  M = (D + wL) D^{-1} (D + wU) / (w(2-w)),  ILU(0).
+// Stopping test (LASPack RTC): ||r||_2 < eps * ||b||_2 on the recursively
+// updated, unpreconditioned residual (GMRES: its residual estimate).
+// The reported accuracy is the TRUE ||b - A x||_2 / ||b||_2 at exit.
+// mc=1 applies SSOR / ILU(0) in greedy multicolour order (the GPU order).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "ref.h"
+
+namespace oref {
+
+void las_mul(const QMat &A, const double *x, double *y) {
+    int n = A.dim;
+    if (!A.sym) {
+        for (int i = 0; i < n; ++i) {
+            double s = 0.;
+            for (auto &e : A.rows[i]) s += e.second * x[e.first];
+            y[i] = s;
+        }
+        return;
+    }
+    for (int i = 0; i < n; ++i) y[i] = 0.;
+    for (int i = 0; i < n; ++i)
+        for (auto &e : A.rows[i]) {
+            if (e.first == i) {
+                y[i] += e.second * x[i];
+            } else {
+                y[i] += e.second * x[e.first];
+                y[e.first] += e.second * x[i];
+            }
+        }
+}
+
+// Expanded (full) row structure: diagonal separate, off-diagonals.
+struct Full {
+    int n = 0;
+    std::vector<double> diag;
+    std::vector<std::vector<std::pair<int, double>>> off;
+};
+static Full expand(const QMat &A) {
+    Full F;
+    F.n = A.dim;
+    F.diag.assign(F.n, 0.);
+    F.off.assign(F.n, {});
+    for (int i = 0; i < F.n; ++i)
+        for (auto &e : A.rows[i]) {
+            if (e.first == i) {
+                F.diag[i] += e.second;
+            } else {
+                F.off[i].push_back(e);
+                if (A.sym) F.off[e.first].push_back({i, e.second});
+            }
+        }
+    return F;
+}
+
+static std::vector<int> colour_of(const Full &F) {
+    std::vector<std::vector<int>> adj(F.n);
+    for (int i = 0; i < F.n; ++i)
+        for (auto &e : F.off[i]) {
+            adj[i].push_back(e.first);
+            adj[e.first].push_back(i);
+        }
+    return greedy_colour(F.n, adj);
+}
+
+// Ordering used by the triangular sweeps: natural, or (colour, index).
+static std::vector<int> sweep_pos(const Full &F, int mc,
+                                  const std::vector<int> *given) {
+    std::vector<int> pos(F.n);
+    if (!mc) {
+        std::iota(pos.begin(), pos.end(), 0);
+        return pos;
+    }
+    std::vector<int> col = given ? *given : colour_of(F);
+    std::vector<int> order(F.n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return col[a] < col[b]; });
+    for (int k = 0; k < F.n; ++k) pos[order[k]] = k;
+    return pos;
+}
+
+struct Precond {
+    int kind = CFD_PC_NONE;
+    double omega = 1.;
+    Full F;
+    std::vector<int> pos, order;
+    // ILU(0): per row, factor values aligned with F.off[i]; udiag
+    std::vector<std::vector<double>> lu;
+    std::vector<double> udiag;
+
+    void setup(const QMat &A, int k, double w, int mc,
+               const std::vector<int> *given = nullptr) {
+        kind = k;
+        omega = w;
+        F = expand(A);
+        if (kind == CFD_PC_SSOR || kind == CFD_PC_ILU) {
+            pos = sweep_pos(F, mc, given);
+            order.assign(F.n, 0);
+            for (int i = 0; i < F.n; ++i) order[pos[i]] = i;
+        }
+        if (kind == CFD_PC_ILU) factor_ilu();
+    }
+    // ILU(0) on the pattern, rows in sweep order (IKJ variant): for row i,
+    // for each earlier neighbour k in ascending sweep position:
+    //   l_ik = a_ik / u_kk;  a_ij -= l_ik * u_kj for later j in the pattern.
+    void factor_ilu() {
+        int n = F.n;
+        lu.assign(n, {});
+        udiag = F.diag;
+        for (int i = 0; i < n; ++i) {
+            lu[i].resize(F.off[i].size());
+            for (size_t t = 0; t < F.off[i].size(); ++t) lu[i][t] = F.off[i][t].second;
+        }
+        for (int s = 0; s < n; ++s) {
+            int i = order[s];
+            auto &oi = F.off[i];
+            std::vector<int> lows;
+            for (size_t t = 0; t < oi.size(); ++t)
+                if (pos[oi[t].first] < s) lows.push_back((int)t);
+            std::sort(lows.begin(), lows.end(), [&](int a, int b) {
+                return pos[oi[a].first] < pos[oi[b].first];
+            });
+            for (int t : lows) {
+                int k = oi[t].first;
+                lu[i][t] = lu[i][t] / udiag[k];
+                double lik = lu[i][t];
+                // a_ij -= l_ik u_kj for j in pattern(i) with pos[j] > pos[k]
+                auto &ok = F.off[k];
+                for (size_t q = 0; q < ok.size(); ++q) {
+                    int j = ok[q].first;
+                    if (pos[j] <= pos[k]) continue;
+                    if (j == i) {
+                        udiag[i] -= lik * lu[k][q];
+                        continue;
+                    }
+                    for (size_t r = 0; r < oi.size(); ++r)
+                        if (oi[r].first == j) {
+                            lu[i][r] -= lik * lu[k][q];
+                            break;
+                        }
+                }
+            }
+        }
+    }
+    void apply(const double *c, double *y) const {
+        int n = F.n;
+        if (kind == CFD_PC_NONE) {
+            std::memcpy(y, c, sizeof(double) * n);
+        } else if (kind == CFD_PC_JACOBI) {
+            for (int i = 0; i < n; ++i) y[i] = c[i] / F.diag[i];
+        } else if (kind == CFD_PC_SSOR) {
+            std::vector<double> t(n);
+            for (int s = 0; s < n; ++s) {
+                int i = order[s];
+                double acc = c[i];
+                for (auto &e : F.off[i])
+                    if (pos[e.first] < s) acc -= omega * e.second * t[e.first];
+                t[i] = acc / F.diag[i];
+            }
+            for (int i = 0; i < n; ++i) t[i] = F.diag[i] * t[i];
+            for (int s = n - 1; s >= 0; --s) {
+                int i = order[s];
+                double acc = t[i];
+                for (auto &e : F.off[i])
+                    if (pos[e.first] > s) acc -= omega * e.second * y[e.first];
+                y[i] = acc / F.diag[i];
+            }
+            double f = omega * (2. - omega);
+            for (int i = 0; i < n; ++i) y[i] = f * y[i];
+        } else {  // ILU
+            std::vector<double> t(n);
+            for (int s = 0; s < n; ++s) {
+                int i = order[s];
+                double acc = c[i];
+                for (size_t q = 0; q < F.off[i].size(); ++q)
+                    if (pos[F.off[i][q].first] < s) acc -= lu[i][q] * t[F.off[i][q].first];
+                t[i] = acc;
+            }
+            for (int s = n - 1; s >= 0; --s) {
+                int i = order[s];
+                double acc = t[i];
+                for (size_t q = 0; q < F.off[i].size(); ++q)
+                    if (pos[F.off[i][q].first] > s) acc -= lu[i][q] * y[F.off[i][q].first];
+                y[i] = acc / udiag[i];
+            }
+        }
+    }
+};
+
+void las_precond(const QMat &A, int precond, double omega, int mc,
+                 const double *c, double *y) {
+    Precond P;
+    P.setup(A, precond, omega, mc);
+    P.apply(c, y);
+}
+
+static double dot(const std::vector<double> &a, const std::vector<double> &b) {
+    double s = 0.;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+static double nrm(const std::vector<double> &a) { return std::sqrt(dot(a, a)); }
+
+static double true_relres(const QMat &A, const std::vector<double> &x,
+                          const std::vector<double> &b) {
+    std::vector<double> r(A.dim);
+    las_mul(A, x.data(), r.data());
+    for (int i = 0; i < A.dim; ++i) r[i] = b[i] - r[i];
+    double bn = nrm(b);
+    return bn > 0 ? nrm(r) / bn : nrm(r);
+}
+
+static bool rtc(double rn, double bn, double eps) {
+    return bn > 0 ? rn < eps * bn : rn < eps;
+}
+
+static int cg(const QMat &A, const Precond &M, std::vector<double> &x,
+              const std::vector<double> &b, double eps, int maxit, double &rt) {
+    int n = A.dim;
+    std::vector<double> r(n), z(n), p(n), q(n);
+    las_mul(A, x.data(), q.data());
+    for (int i = 0; i < n; ++i) r[i] = b[i] - q[i];
+    double bn = nrm(b), rho = 0, rho_old = 0;
+    int it = 0;
+    while (!rtc(rt = nrm(r), bn, eps) && it < maxit) {
+        ++it;
+        M.apply(r.data(), z.data());
+        rho = dot(r, z);
+        if (it == 1) {
+            p = z;
+        } else {
+            double beta = rho / rho_old;
+            for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+        }
+        las_mul(A, p.data(), q.data());
+        double alpha = rho / dot(p, q);
+        for (int i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * q[i];
+        }
+        rho_old = rho;
+    }
+    return it;
+}
+
+static int bicgstab(const QMat &A, const Precond &M, std::vector<double> &x,
+                    const std::vector<double> &b, double eps, int maxit,
+                    double &rt) {
+    int n = A.dim;
+    std::vector<double> r(n), rt0(n), p(n), ph(n), v(n), s(n), sh(n), t(n);
+    las_mul(A, x.data(), v.data());
+    for (int i = 0; i < n; ++i) r[i] = b[i] - v[i];
+    rt0 = r;
+    double bn = nrm(b), rho = 0, rho_old = 1, alpha = 0, omega = 1;
+    int it = 0;
+    while (!rtc(rt = nrm(r), bn, eps) && it < maxit) {
+        ++it;
+        rho = dot(rt0, r);
+        if (rho == 0.) break;
+        if (it == 1) {
+            p = r;
+        } else {
+            double beta = (rho / rho_old) * (alpha / omega);
+            for (int i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+        }
+        M.apply(p.data(), ph.data());
+        las_mul(A, ph.data(), v.data());
+        alpha = rho / dot(rt0, v);
+        for (int i = 0; i < n; ++i) s[i] = r[i] - alpha * v[i];
+        if (rtc(nrm(s), bn, eps)) {
+            for (int i = 0; i < n; ++i) x[i] += alpha * ph[i];
+            r = s;
+            rt = nrm(r);
+            break;
+        }
+        M.apply(s.data(), sh.data());
+        las_mul(A, sh.data(), t.data());
+        omega = dot(t, s) / dot(t, t);
+        for (int i = 0; i < n; ++i) {
+            x[i] = x[i] + alpha * ph[i] + omega * sh[i];
+            r[i] = s[i] - omega * t[i];
+        }
+        rho_old = rho;
+        if (omega == 0.) break;
+    }
+    return it;
+}
+
+static int gmres(const QMat &A, const Precond &M, std::vector<double> &x,
+                 const std::vector<double> &b, double eps, int maxit, int m,
+                 double &rt) {
+    int n = A.dim;
+    std::vector<std::vector<double>> V(m + 1, std::vector<double>(n));
+    std::vector<double> w(n), z(n), u(n);
+    std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), h(m + 1), h2(m + 1);
+    double bn = nrm(b);
+    int it = 0;
+    las_mul(A, x.data(), w.data());
+    for (int i = 0; i < n; ++i) V[0][i] = b[i] - w[i];
+    double beta = nrm(V[0]);
+    rt = beta;
+    while (!rtc(beta, bn, eps) && it < maxit) {
+        for (int i = 0; i < n; ++i) V[0][i] /= beta;
+        std::fill(g.begin(), g.end(), 0.);
+        g[0] = beta;
+        int j = 0, k = 0;
+        for (j = 0; j < m; ++j) {
+            ++it;
+            M.apply(V[j].data(), z.data());
+            las_mul(A, z.data(), w.data());
+            // classical Gram-Schmidt, twice
+            for (int i = 0; i <= j; ++i) h[i] = dot(w, V[i]);
+            for (int i = 0; i <= j; ++i)
+                for (int q = 0; q < n; ++q) w[q] -= h[i] * V[i][q];
+            for (int i = 0; i <= j; ++i) h2[i] = dot(w, V[i]);
+            for (int i = 0; i <= j; ++i)
+                for (int q = 0; q < n; ++q) w[q] -= h2[i] * V[i][q];
+            for (int i = 0; i <= j; ++i) h[i] += h2[i];
+            double hn = nrm(w);
+            for (int i = 0; i <= j; ++i) H[i * m + j] = h[i];
+            H[(j + 1) * m + j] = hn;
+            if (hn != 0.)
+                for (int q = 0; q < n; ++q) V[j + 1][q] = w[q] / hn;
+            // Givens
+            for (int i = 0; i < j; ++i) {
+                double a = H[i * m + j], c = H[(i + 1) * m + j];
+                H[i * m + j] = cs[i] * a + sn[i] * c;
+                H[(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
+            }
+            double a = H[j * m + j], c = H[(j + 1) * m + j];
+            if (c == 0.) {
+                cs[j] = 1.;
+                sn[j] = 0.;
+            } else {
+                double t = std::hypot(a, c);
+                cs[j] = a / t;
+                sn[j] = c / t;
+            }
+            H[j * m + j] = cs[j] * a + sn[j] * c;
+            H[(j + 1) * m + j] = 0.;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            rt = std::fabs(g[j + 1]);
+            k = j + 1;
+            if (rtc(rt, bn, eps) || it >= maxit || hn == 0.) break;
+        }
+        // y = H^{-1} g (upper triangular, k x k)
+        std::vector<double> y(k);
+        for (int i = k - 1; i >= 0; --i) {
+            double s = g[i];
+            for (int q = i + 1; q < k; ++q) s -= H[i * m + q] * y[q];
+            y[i] = s / H[i * m + i];
+        }
+        std::fill(u.begin(), u.end(), 0.);
+        for (int i = 0; i < k; ++i)
+            for (int q = 0; q < n; ++q) u[q] += y[i] * V[i][q];
+        M.apply(u.data(), z.data());
+        for (int q = 0; q < n; ++q) x[q] += z[q];
+        las_mul(A, x.data(), w.data());
+        for (int q = 0; q < n; ++q) V[0][q] = b[q] - w[q];
+        beta = nrm(V[0]);
+        rt = beta;
+    }
+    return it;
+}
+
+SolveResult las_solve(const QMat &A, std::vector<double> &x,
+                      const std::vector<double> &b, const SolveCfg &cfg) {
+    Precond M;
+    M.setup(A, cfg.precond, cfg.omega, cfg.mc, cfg.colour);
+    SolveResult R;
+    double rt = 0;
+    if (cfg.method == CFD_CG) R.iters = cg(A, M, x, b, cfg.tol, cfg.maxit, rt);
+    else if (cfg.method == CFD_BICGSTAB)
+        R.iters = bicgstab(A, M, x, b, cfg.tol, cfg.maxit, rt);
+    else R.iters = gmres(A, M, x, b, cfg.tol, cfg.maxit, 10, rt);
+    R.rtcres = rt;
+    R.relres = true_relres(A, x, b);
+    return R;
+}
+
+}  // namespace oref

@@ -1,0 +1,1058 @@
+// ORACLE -- TEST INFRASTRUCTURE ONLY (see ref.h).
+// Restatement of NavierStokesSolver / HeatConductionSolver's per-iteration
+// path (steady, DensityModel 0). Names follow the reference's methods.
+#include "ref_navier.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace oref {
+
+static inline double cmax(double x, double y) { return x > y ? x : y; }  // CYCASMAX
+static inline double cmin(double x, double y) { return x < y ? x : y; }  // CYCASMIN
+static inline double dot3(const double *a, const double *b) {  // vec_dot
+    double s = 0.;
+    for (int i = 0; i < 3; ++i) s += a[i] * b[i];
+    return s;
+}
+static inline void sub3(double *a, const double *b, const double *c) {  // vec_minus
+    for (int i = 0; i < 3; ++i) a[i] = b[i] - c[i];
+}
+static inline double len3(const double *a) { return std::sqrt(dot3(a, a)); }
+
+namespace kev {  // TurKEpsilonVar (src/navier.h:129-140)
+const double kappa = 0.419, Cmu = 0.09, Ceps1 = 1.44, Ceps2 = 1.92,
+             Sigma_k = 1.0, Sigma_e = 1.3;
+}
+static const double SMALL = 1.e-16;
+
+// ------------------------------------------------------------ Gradient --
+// CycasSolver::Gradient (src/BasicSolver.cpp:29-77) + limiters (:88-288)
+int gradient(const Mesh &M, int limiter, const double *phi, const double *bphi,
+             double *g) {
+    int n = M.ncel;
+    for (int i = 0; i < 3 * n; ++i) g[i] = 0.;
+    for (int i = 0; i < M.nfac; ++i) {
+        const Face &F = M.face[i];
+        double lambda = F.lambda;
+        int c1 = F.c1, c2 = F.c2;
+        if (c2 >= 0) {
+            double pf = lambda * phi[c1] + (1. - lambda) * phi[c2];
+            for (int k = 0; k < 3; ++k) {
+                g[3 * c1 + k] += pf * F.n[k];
+                g[3 * c2 + k] -= pf * F.n[k];
+            }
+        } else {
+            double pf = bphi[F.bnd];
+            for (int k = 0; k < 3; ++k) g[3 * c1 + k] += pf * F.n[k];
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) g[3 * i + k] /= M.cell[i].vol;
+    if (limiter == 0) return 0;
+    auto barth = [](double us, double umin, double umax) {
+        if (us > 1.0e-10) return cmin(1., umax / us);
+        if (us < -1.0e-10) return cmin(1., umin / us);
+        return 1.;
+    };
+    std::vector<double> fic(n);
+    if (limiter == 1) {  // Limiter_Barth (:182-224)
+        for (int i = 0; i < n; ++i) {
+            const Cell &C = M.cell[i];
+            double fi = 10., umax = phi[i], umin = phi[i];
+            for (int j = 0; j < C.nface; ++j) {
+                int in = C.cell[j];
+                if (in < 0) continue;
+                umax = cmax(umax, phi[in]);
+                umin = cmin(umin, phi[in]);
+            }
+            umax -= phi[i];
+            umin -= phi[i];
+            for (int j = 0; j < 8; ++j) {
+                const double *xv = M.vert[C.v[j]].data();
+                double dx = xv[0] - C.x[0], dy = xv[1] - C.x[1], dz = xv[2] - C.x[2];
+                double us = g[3 * i] * dx + g[3 * i + 1] * dy + g[3 * i + 2] * dz;
+                fi = cmin(fi, barth(us, umin, umax));
+            }
+            fic[i] = fi;
+        }
+    } else if (limiter == 2) {  // Limiter_MLP (:118-180)
+        std::vector<double> vmax(M.nvrt, -1.e8), vmin(M.nvrt, 1.e8);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < 8; ++j) {
+                int iv = M.cell[i].v[j];
+                vmax[iv] = cmax(vmax[iv], phi[i]);
+                vmin[iv] = cmin(vmin[iv], phi[i]);
+            }
+        for (int i = 0; i < n; ++i) {
+            const Cell &C = M.cell[i];
+            double fi = 10.;
+            for (int j = 0; j < 8; ++j) {
+                int iv = C.v[j];
+                double umax = vmax[iv] - phi[i], umin = vmin[iv] - phi[i];
+                const double *xv = M.vert[iv].data();
+                double dx = xv[0] - C.x[0], dy = xv[1] - C.x[1], dz = xv[2] - C.x[2];
+                double us = g[3 * i] * dx + g[3 * i + 1] * dy + g[3 * i + 2] * dz;
+                fi = cmin(fi, barth(us, umin, umax));
+            }
+            fic[i] = fi;
+        }
+    } else if (limiter == 3) {  // Limiter_WENO (:226-288)
+        std::vector<double> t(3 * n, 0.);
+        for (int i = 0; i < n; ++i) {
+            const Cell &C = M.cell[i];
+            double ss = 0.;
+            for (int j = 0; j < C.nface; ++j) {
+                int ic = C.cell[j];
+                if (ic < 0) continue;
+                const double *G = g + 3 * ic;
+                double wei = 1. / (G[0] * G[0] + G[1] * G[1] + G[2] * G[2] + 1.e-16);
+                ss += wei;
+                for (int k = 0; k < 3; ++k) t[3 * i + k] += wei * G[k];
+            }
+            const double *G = g + 3 * i;
+            double wei = 1. / (G[0] * G[0] + G[1] * G[1] + G[2] * G[2] + 1.e-16);
+            ss += wei;
+            for (int k = 0; k < 3; ++k) t[3 * i + k] += wei * G[k];
+            for (int k = 0; k < 3; ++k) t[3 * i + k] /= ss;
+        }
+        for (int i = 0; i < 3 * n; ++i) g[i] = t[i];
+        return 0;
+    } else {
+        return -1;
+    }
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) g[3 * i + k] = fic[i] * g[3 * i + k];
+    return 0;
+}
+
+// -------------------------------------------------------------- set-up --
+int RefNS::setup(const cfd_raw_mesh &raw, const cfd_params &p) {
+    P = p;
+    for (int r = 0; r < p.nregion; ++r) {
+        Region R;
+        R.type1 = p.region[r].type1;
+        R.type2 = p.region[r].type2;
+        R.fixed = p.region[r].fixed;
+        for (int k = 0; k < 10; ++k) R.init[k] = p.region[r].init[k];
+        reg[p.region[r].id] = R;
+    }
+    for (int k = 0; k < 8; ++k) URF[k] = p.urf[k];
+    Mesh full;
+    load_raw(raw, full);
+    if (create_faces(full)) return err("face construction failed");
+    if (!p.cht) {
+        M = full;
+    } else {
+        // buildSolver (src/Preprocessor.cpp:441-515)
+        int cid = -1;
+        for (int i = 0; i != (int)reg.size() + 1; ++i)
+            if (reg.find(i) == reg.end()) {
+                Region R;
+                R.type1 = 1;
+                R.type2 = 2;
+                reg[i] = R;
+                cid = i;
+                break;
+            }
+        std::vector<int> fl, so;
+        for (int i = 0; i < full.ncel; ++i) {
+            auto it = reg.find(full.cell[i].rid);
+            if (it == reg.end() || it->second.type1 != 5) return err("bad body rid");
+            if (it->second.type2 == 0) fl.push_back(i);
+            else so.push_back(i);
+        }
+        split_region(full, fl, cid, M);
+        ht.reset(new RefHT);
+        split_region(full, so, cid, ht->M);
+        ht->reg = reg;
+        ht->ns = this;
+    }
+    if (cell_face_info(M)) return err("geometry check failed");
+    if (ht && cell_face_info(ht->M)) return err("solid geometry check failed");
+    // CheckAndAllocate (src/navier.cpp:493-583)
+    for (int i = 0; i < M.nfac; ++i) {
+        int c2 = M.face[i].c2;
+        if (!(c2 == -10 || c2 >= 0)) return err("error in face right hand side");
+    }
+    for (int i = 0; i < M.nfac; ++i)
+        if (M.face[i].c2 < 0 && M.face[i].bnd < 0) return err("unlisted boundary face");
+    int n = M.ncel, nb = M.nbnd;
+    for (auto *v : {&Rn, &Un, &Vn, &Wn, &Pn, &Tn, &TE, &ED, &VisLam, &VisTur, &Apr,
+                    &xsol, &bu, &bv, &bw, &bp, &bs})
+        v->assign(n, 0.);
+    for (auto *v : {&dPdX, &dUdX, &dVdX, &dWdX, &dPhidX}) v->assign(3 * n, 0.);
+    for (auto *v : {&BRo, &BU, &BV, &BW, &BPre, &BTem, &BTE, &BED}) v->assign(nb, 0.);
+    RUFace.assign(M.nfac, 0.);
+    std::vector<std::vector<int>> adj(n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < M.cell[i].nface; ++j)
+            if (M.cell[i].cell[j] >= 0) adj[i].push_back(M.cell[i].cell[j]);
+    colour = greedy_colour(n, adj);
+    if (ht) {
+        RefHT &h = *ht;
+        int m = h.M.ncel;
+        for (auto *v : {&h.Tn, &h.Rn, &h.diffCoef, &h.xsol, &h.bs}) v->assign(m, 0.);
+        h.dPhidX.assign(3 * m, 0.);
+        h.BTem.assign(h.M.nbnd, 0.);
+        std::vector<std::vector<int>> a2(m);
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < h.M.cell[i].nface; ++j)
+                if (h.M.cell[i].cell[j] >= 0) a2[i].push_back(h.M.cell[i].cell[j]);
+        h.colour = greedy_colour(m, a2);
+    }
+    return InitFlowField();
+}
+
+// InitFlowField (src/navier.cpp:113-188)
+int RefNS::InitFlowField() {
+    for (int i = 0; i < M.ncel; ++i) {
+        auto it = reg.find(M.cell[i].rid);
+        if (it == reg.end() || it->second.type1 != 5 || it->second.type2 != 0)
+            return err("cell region is not a fluid body");
+        const double *iv = it->second.init;
+        Un[i] = iv[0];
+        Vn[i] = iv[1];
+        Wn[i] = iv[2];
+        Rn[i] = iv[3];
+        Tn[i] = iv[4];
+        Pn[i] = 0.0;
+        VisLam[i] = iv[5];
+        VisTur[i] = 0.;
+        if (P.turb_model == 1) {
+            TE[i] = iv[6];
+            ED[i] = iv[7];
+            VisTur[i] = Rn[i] * kev::Cmu * TE[i] * TE[i] / (ED[i] + SMALL);
+        }
+    }
+    for (int i = 0; i < M.nbnd; ++i) {
+        Region &R = reg[M.bnd[i].rid];
+        if (R.type1 == 1) {
+            if (R.type2 == 0) BTem[i] = R.fixed;
+            else if (R.type2 == 1) M.bnd[i].q = R.fixed;
+            else if (R.type2 == 2) M.bnd[i].q = 0.0;
+        } else if (R.type1 == 2) {
+            BTem[i] = R.init[5];
+        } else if (R.type1 == 4) {
+            M.bnd[i].q = 0.0;
+        }
+    }
+    for (int i = 0; i < M.nfac; ++i) RUFace[i] = 0.;
+    if (ht) return ht->InitFlowField();
+    return 0;
+}
+
+// ----------------------------------------------------------------- BCs --
+// SetBCVelocity (src/navier_bc.cpp:8-108)
+int RefNS::SetBCVelocity(double *br, double *bu_, double *bv_, double *bw_) {
+    for (int i = 0; i < M.nbnd; ++i) {
+        int rid = M.bnd[i].rid, iface = M.bnd[i].face, ic = M.face[iface].c1;
+        Region &R = reg[rid];
+        const Face &F = M.face[iface];
+        switch (R.type1) {
+        case 1:
+            br[i] = Rn[ic]; bu_[i] = 0.; bv_[i] = 0.; bw_[i] = 0.;
+            break;
+        case 2:
+            br[i] = R.init[4]; bu_[i] = R.init[0]; bv_[i] = R.init[1]; bw_[i] = R.init[2];
+            break;
+        case 3:
+            br[i] = Rn[ic]; bu_[i] = Un[ic]; bv_[i] = Vn[ic]; bw_[i] = Wn[ic];
+            break;
+        case 4: {
+            double s1 = F.n[0] / F.area, s2 = F.n[1] / F.area, s3 = F.n[2] / F.area;
+            double un = Un[ic] * s1 + Vn[ic] * s2 + Wn[ic] * s3;
+            bu_[i] = Un[ic] - un * s1;
+            bv_[i] = Vn[ic] - un * s2;
+            bw_[i] = Wn[ic] - un * s3;
+            br[i] = Rn[ic];
+            break;
+        }
+        default:
+            return err("no such boundary type in velocity bc");
+        }
+    }
+    double min_ = 0., mout = 0., rate, aout = 0.;
+    for (int i = 0; i < M.nbnd; ++i) {
+        int t = reg[M.bnd[i].rid].type1;
+        const Face &F = M.face[M.bnd[i].face];
+        if (t == 2)
+            min_ += br[i] * (bu_[i] * F.n[0] + bv_[i] * F.n[1] + bw_[i] * F.n[2]);
+        else if (t == 3) {
+            mout += br[i] * (bu_[i] * F.n[0] + bv_[i] * F.n[1] + bw_[i] * F.n[2]);
+            aout += br[i] * M.face[i].area;  // Face[i], as the reference
+        }
+    }
+    if (std::fabs(mout) > SMALL) {
+        rate = -min_ / mout;
+        for (int i = 0; i < M.nbnd; ++i)
+            if (reg[M.bnd[i].rid].type1 == 3) {
+                bu_[i] *= rate; bv_[i] *= rate; bw_[i] *= rate;
+            }
+    } else {
+        rate = (-min_ - mout) / aout;
+        for (int i = 0; i < M.nbnd; ++i)
+            if (reg[M.bnd[i].rid].type1 == 3) {
+                const Face &F = M.face[M.bnd[i].face];
+                bu_[i] += rate * F.n[0] / F.area;
+                bv_[i] += rate * F.n[1] / F.area;
+                bw_[i] += rate * F.n[2] / F.area;
+            }
+    }
+    return 0;
+}
+
+// SetBCPressure / SetBCDeltaP (src/navier_bc.cpp:110-140, 142-162)
+int RefNS::SetBCPressure(double *b) {
+    for (int i = 0; i < M.nbnd; ++i) {
+        int t = reg[M.bnd[i].rid].type1, ic = M.face[M.bnd[i].face].c1;
+        if (t < 1 || t > 4) return err("no such boundary type");
+        b[i] = Pn[ic];
+    }
+    return 0;
+}
+int RefNS::SetBCDeltaP(double *b, const double *dp) {
+    for (int i = 0; i < M.nbnd; ++i) {
+        int t = reg[M.bnd[i].rid].type1, ic = M.face[M.bnd[i].face].c1;
+        if (t < 1 || t > 4) return err("no such boundary type");
+        b[i] = dp[ic];
+    }
+    return 0;
+}
+
+// SetBCTemperature (src/navier_bc.cpp:164-203)
+int RefNS::SetBCTemperature(double *bt) {
+    for (int i = 0; i < M.nbnd; ++i) {
+        Region &R = reg[M.bnd[i].rid];
+        int ic = M.face[M.bnd[i].face].c1;
+        switch (R.type1) {
+        case 1:
+            if (R.type2 == 1) bt[i] = Tn[ic];
+            else if (R.type2 != 0 && R.type2 != 2) return err("bad wall type2");
+            break;
+        case 2:
+            break;
+        case 3:
+        case 4:
+            bt[i] = Tn[ic];
+            break;
+        default:
+            return err("no such boundary type");
+        }
+    }
+    return 0;
+}
+
+// SetBCKEpsilon (src/navier_bc.cpp:209-296)
+int RefNS::SetBCKEpsilon(double *TES, double *EDS, double *ApTE, double *ApED,
+                         double *Prod) {
+    (void)EDS;
+    (void)ApED;
+    for (int i = 0; i < M.nbnd; ++i) {
+        int bt = reg[M.bnd[i].rid].type1;
+        int iface = M.bnd[i].face, ic = M.face[iface].c1;
+        if (bt != 1) continue;
+        const Face &F = M.face[iface];
+        BTE[i] = TE[ic];
+        BED[i] = ED[ic];
+        double vol = M.cell[ic].vol;
+        double s1 = F.n[0] / (F.area + 1.e-16), s2 = F.n[1] / (F.area + 1.e-16),
+               s3 = F.n[2] / (F.area + 1.e-16);
+        double vnor = Un[ic] * s1 + Vn[ic] * s2 + Wn[ic] * s3;
+        double vt1 = Un[ic] - vnor * s1, vt2 = Vn[ic] - vnor * s2, vt3 = Wn[ic] - vnor * s3;
+        double vel = std::sqrt(vt1 * vt1 + vt2 * vt2 + vt3 * vt3);
+        double Cmu25 = std::pow(kev::Cmu, 0.25);
+        double d = M.bnd[i].distance;
+        double utau = Cmu25 * std::sqrt(TE[ic]);
+        double yplus = d * utau * Rn[ic] / (VisLam[ic] + SMALL);
+        double tauw = VisLam[ic] * vel / d;
+        if (yplus < 11.) utau = std::sqrt(tauw / Rn[ic]);
+        else utau = Cmu25 * std::sqrt(TE[ic]);
+        Prod[ic] = tauw * utau / (kev::kappa * d);
+        double eps = std::pow(kev::Cmu, 0.75) * std::pow(TE[ic], 1.5) / (kev::kappa * d);
+        TES[ic] = Prod[ic] * vol;
+        ApTE[ic] -= Rn[ic] * ED[ic] / TE[ic] * vol;
+        ApTE[ic] += Rn[ic] * eps / TE[ic] * vol;
+        ED[ic] = eps;
+        BED[i] = eps;
+    }
+    for (int i = 0; i < M.nbnd; ++i)
+        if (M.bnd[i].rid == 2) {
+            const double *iv = reg[M.bnd[i].rid].init;
+            BTE[i] = iv[6];
+            BED[i] = iv[7];
+        }
+    for (int pass = 3; pass <= 4; ++pass)
+        for (int i = 0; i < M.nbnd; ++i)
+            if (M.bnd[i].rid == pass) {
+                int ic = M.face[M.bnd[i].face].c1;
+                BTE[i] = TE[ic];
+                BED[i] = ED[ic];
+            }
+    return 0;
+}
+
+// ---------------------------------------------------------- assembly --
+// BuildVelocityMatrix (src/navier_velocity.cpp:77-357)
+int RefNS::BuildVelocityMatrix() {
+    if (SetBCVelocity(BRo.data(), BU.data(), BV.data(), BW.data())) return -1;
+    if (SetBCPressure(BPre.data())) return -1;
+    gradient(M, P.limiter, Un.data(), BU.data(), dUdX.data());
+    gradient(M, P.limiter, Vn.data(), BV.data(), dVdX.data());
+    gradient(M, P.limiter, Wn.data(), BW.data(), dWdX.data());
+    gradient(M, P.limiter, Pn.data(), BPre.data(), dPdX.data());
+    As.constr(M.ncel, false);
+    for (int i = 0; i < M.ncel; ++i) {
+        const Cell &C = M.cell[i];
+        double app = 0., apn[6] = {0, 0, 0, 0, 0, 0}, su = 0., sv = 0., sw = 0.;
+        int ani[6], nj = 0;
+        double fde[3], fdi[3], fcs[3] = {0., 0., 0.};
+        for (int j = 0; j < C.nface; ++j) {
+            int iface = C.face[j];
+            const Face &F = M.face[iface];
+            int ip = F.c1, in = F.c2;
+            if (in < 0) {
+                int bnd = F.bnd, rid = M.bnd[bnd].rid;
+                int t1 = reg[rid].type1;
+                double s1 = F.n[0], s2 = F.n[1], s3 = F.n[2], RU = RUFace[iface];
+                double Visc = VisLam[i] + VisTur[i];
+                const double *gu = &dUdX[3 * i], *gv = &dVdX[3 * i], *gw = &dWdX[3 * i];
+                double dxc[3];
+                sub3(dxc, F.x, C.x);
+                double VAL;
+                switch (t1) {
+                case 1:
+                    VAL = Visc * F.rlencos;
+                    app += VAL;
+                    fde[0] = VAL * Un[i]; fde[1] = VAL * Vn[i]; fde[2] = VAL * Wn[i];
+                    fdi[0] = VAL * Un[i]; fdi[1] = VAL * Vn[i]; fdi[2] = VAL * Wn[i];
+                    break;
+                case 2: case 3: case 4:
+                    VAL = Visc * F.rlencos;
+                    app += VAL;
+                    fde[0] = (gu[0] + gu[0]) * s1 + (gu[1] + gv[0]) * s2 + (gu[2] + gw[0]) * s3;
+                    fde[1] = (gv[0] + gu[1]) * s1 + (gv[1] + gv[1]) * s2 + (gv[2] + gw[1]) * s3;
+                    fde[2] = (gw[0] + gu[2]) * s1 + (gw[1] + gv[2]) * s2 + (gw[2] + gw[2]) * s3;
+                    fde[0] *= Visc; fde[1] *= Visc; fde[2] *= Visc;
+                    fdi[0] = VAL * (gu[0] * dxc[0] + gu[1] * dxc[1] + gu[2] * dxc[2] - BU[bnd]);
+                    fdi[1] = VAL * (gv[0] * dxc[0] + gv[1] * dxc[1] + gv[2] * dxc[2] - BV[bnd]);
+                    fdi[2] = VAL * (gw[0] * dxc[0] + gw[1] * dxc[1] + gw[2] * dxc[2] - BW[bnd]);
+                    break;
+                default:
+                    return err("no such rid");
+                }
+                if (t1 == 2 || t1 == 3) {
+                    double f = cmin(RU, 0.0);
+                    app -= f;
+                    fcs[0] = f * BU[bnd]; fcs[1] = f * BV[bnd]; fcs[2] = f * BW[bnd];
+                } else {
+                    fcs[0] = 0.; fcs[1] = 0.; fcs[2] = 0.;
+                }
+            } else {
+                double s1, s2, s3, RU, lambda;
+                if (i != ip) {
+                    in = ip;
+                    s1 = -F.n[0]; s2 = -F.n[1]; s3 = -F.n[2];
+                    RU = -RUFace[iface];
+                    lambda = 1. - F.lambda;
+                } else {
+                    s1 = F.n[0]; s2 = F.n[1]; s3 = F.n[2];
+                    RU = RUFace[iface];
+                    lambda = F.lambda;
+                }
+                double l2 = 1. - lambda;
+                double Visc = lambda * (VisLam[i] + VisTur[i]) + l2 * (VisLam[in] + VisTur[in]);
+                double g[9];
+                for (int k = 0; k < 3; ++k) {
+                    g[k] = lambda * dUdX[3 * i + k] + l2 * dUdX[3 * in + k];
+                    g[3 + k] = lambda * dVdX[3 * i + k] + l2 * dVdX[3 * in + k];
+                    g[6 + k] = lambda * dWdX[3 * i + k] + l2 * dWdX[3 * in + k];
+                }
+                if (RU < 0.) {
+                    apn[nj] += RU;
+                    app -= RU;
+                }
+                double VAL = Visc * F.rlencos;
+                app += VAL;
+                apn[nj] += -VAL;
+                ani[nj] = in;
+                nj++;
+                int up = RU > 0. ? i : in;
+                double dx[3];
+                sub3(dx, F.x, M.cell[up].x);
+                double ufi = Un[up], vfi = Vn[up], wfi = Wn[up];
+                double ufh = Un[up] + dot3(&dUdX[3 * up], dx);
+                double vfh = Vn[up] + dot3(&dVdX[3 * up], dx);
+                double wfh = Wn[up] + dot3(&dWdX[3 * up], dx);
+                fcs[0] = RU * (ufh - ufi); fcs[1] = RU * (vfh - vfi); fcs[2] = RU * (wfh - wfi);
+                fde[0] = (g[0] + g[0]) * s1 + (g[1] + g[3]) * s2 + (g[2] + g[6]) * s3;
+                fde[1] = (g[3] + g[1]) * s1 + (g[4] + g[4]) * s2 + (g[5] + g[7]) * s3;
+                fde[2] = (g[6] + g[2]) * s1 + (g[7] + g[5]) * s2 + (g[8] + g[8]) * s3;
+                fde[0] *= Visc; fde[1] *= Visc; fde[2] *= Visc;
+                double dxc[3];
+                sub3(dxc, M.cell[in].x, C.x);
+                fdi[0] = VAL * (g[0] * dxc[0] + g[1] * dxc[1] + g[2] * dxc[2]);
+                fdi[1] = VAL * (g[3] * dxc[0] + g[4] * dxc[1] + g[5] * dxc[2]);
+                fdi[2] = VAL * (g[6] * dxc[0] + g[7] * dxc[1] + g[8] * dxc[2]);
+            }
+            su += fde[0] - fdi[0] - fcs[0];
+            sv += fde[1] - fdi[1] - fcs[1];
+            sw += fde[2] - fdi[2] - fcs[2];
+        }
+        app /= URF[0];
+        Apr[i] = 1. / (app);
+        As.set_len(i, nj + 1);
+        As.set_entry(i, 0, i, app);
+        for (int j = 0; j < nj; ++j) As.set_entry(i, j + 1, ani[j], apn[j]);
+        double vol = C.vol;
+        bu[i] = su + (1. - URF[0]) * app * Un[i] + (-dPdX[3 * i + 0] + Rn[i] * P.gravity[0]) * vol;
+        bv[i] = sv + (1. - URF[0]) * app * Vn[i] + (-dPdX[3 * i + 1] + Rn[i] * P.gravity[1]) * vol;
+        bw[i] = sw + (1. - URF[0]) * app * Wn[i] + (-dPdX[3 * i + 2] + Rn[i] * P.gravity[2]) * vol;
+    }
+    return 0;
+}
+
+SolveCfg RefNS::cfg_of(const cfd_solve_cfg &c) const {
+    SolveCfg s;
+    s.method = c.method;
+    s.precond = c.precond;
+    s.omega = c.omega;
+    s.tol = c.tol;
+    s.maxit = c.maxit;
+    s.mc = mc;
+    s.colour = &colour;
+    return s;
+}
+
+// CalculateVelocity (src/navier_velocity.cpp:12-75)
+int RefNS::CalculateVelocity() {
+    for (int i = 0; i < 10; ++i) Residual[i] = 0.;
+    if (BuildVelocityMatrix()) return -1;
+    SolveCfg c = cfg_of(P.mom);
+    double *fld[3] = {Un.data(), Vn.data(), Wn.data()};
+    std::vector<double> *rhs[3] = {&bu, &bv, &bw};
+    for (int k = 0; k < 3; ++k) {
+        for (int i = 0; i < M.ncel; ++i) xsol[i] = fld[k][i];
+        last_mom[k] = las_solve(As, xsol, *rhs[k], c);
+        if (last_mom[k].iters >= c.maxit && last_mom[k].relres > c.tol)
+            return err("velocity cannot converge");
+        for (int i = 0; i < M.ncel; ++i) {
+            Residual[k] += std::fabs(fld[k][i] - xsol[i]) * M.cell[i].vol;
+            fld[k][i] = xsol[i];
+        }
+    }
+    return 0;
+}
+
+// CalRUFace2 (src/navier_velocity.cpp:389-442)
+int RefNS::CalRUFace2() {
+    for (int i = 0; i < M.nfac; ++i) {
+        const Face &F = M.face[i];
+        int c1 = F.c1, c2 = F.c2;
+        if (c2 < 0) {
+            int b = F.bnd;
+            double ruf = BU[b] * BRo[b], rvf = BV[b] * BRo[b], rwf = BW[b] * BRo[b];
+            RUFace[i] = ruf * F.n[0] + rvf * F.n[1] + rwf * F.n[2];
+        } else {
+            double l = F.lambda, l2 = 1. - l;
+            double rf = l * Rn[c1] + l2 * Rn[c2];
+            double ruf = l * Rn[c1] * Un[c1] + l2 * Rn[c2] * Un[c2];
+            double rvf = l * Rn[c1] * Vn[c1] + l2 * Rn[c2] * Vn[c2];
+            double rwf = l * Rn[c1] * Wn[c1] + l2 * Rn[c2] * Wn[c2];
+            RUFace[i] = ruf * F.n[0] + rvf * F.n[1] + rwf * F.n[2];
+            double dx[3];
+            sub3(dx, F.xpac, M.cell[c1].x);
+            double P1 = Pn[c1] + dot3(&dPdX[3 * c1], dx);
+            sub3(dx, F.xnac, M.cell[c2].x);
+            double P2 = Pn[c2] + dot3(&dPdX[3 * c2], dx);
+            sub3(dx, F.xnac, F.xpac);
+            double d12 = len3(dx);
+            double aprf = l * Apr[c1] + l2 * Apr[c2];
+            double vol = l * M.cell[c1].vol + l2 * M.cell[c2].vol;
+            double dpx = l * dPdX[3 * c1] + l2 * dPdX[3 * c2];
+            double dpy = l * dPdX[3 * c1 + 1] + l2 * dPdX[3 * c2 + 1];
+            double dpz = l * dPdX[3 * c1 + 2] + l2 * dPdX[3 * c2 + 2];
+            double dpn = dpx * dx[0] + dpy * dx[1] + dpz * dx[2];
+            RUFace[i] -= rf * F.area * aprf / d12 * vol * ((P2 - P1) - dpn);
+        }
+    }
+    return 0;
+}
+
+// BuildPressureMatrix (src/navier_pressure.cpp:78-198), DensityModel 0
+int RefNS::BuildPressureMatrix() {
+    if (SetBCVelocity(BRo.data(), BU.data(), BV.data(), BW.data())) return -1;
+    CalRUFace2();
+    Ap.constr(M.ncel, true);
+    for (int i = 0; i < M.ncel; ++i) {
+        const Cell &C = M.cell[i];
+        double valcen = 0., bpv = 0., Acn[7];
+        int cn[7], nj = 0;
+        for (int j = 0; j < C.nface; ++j) {
+            int iface = C.face[j];
+            const Face &F = M.face[iface];
+            if (i == F.c1) bpv += RUFace[iface];
+            else if (i == F.c2) bpv -= RUFace[iface];
+            else return err("face/cell mismatch");
+            int in = C.cell[j];
+            if (in < 0) continue;
+            double lambda = (i == F.c1) ? F.lambda : 1. - F.lambda;
+            double l2 = 1. - lambda;
+            double roapf = Rn[i] * Apr[i] * lambda + Rn[in] * Apr[in] * l2;
+            double vol = C.vol * lambda + M.cell[in].vol * l2;
+            double tmp = roapf * vol * F.rlencos;
+            valcen += -tmp;
+            if (in > i) {
+                nj++;
+                Acn[nj] = tmp;
+                cn[nj] = in;
+            }
+        }
+        Ap.set_len(i, nj + 1);
+        Ap.set_entry(i, 0, i, valcen);
+        for (int j = 1; j <= nj; ++j) Ap.set_entry(i, j, cn[j], Acn[j]);
+        bp[i] = bpv;
+    }
+    return 0;
+}
+
+// CorrectRUFace2 (src/navier_pressure.cpp:200-261), DensityModel 0
+int RefNS::CorrectRUFace2(const double *dp) {
+    for (int i = 0; i < M.nfac; ++i) {
+        const Face &F = M.face[i];
+        int c1 = F.c1, c2 = F.c2;
+        if (c2 < 0) {
+            int b = F.bnd;
+            double ruf = BU[b] * BRo[b], rvf = BV[b] * BRo[b], rwf = BW[b] * BRo[b];
+            RUFace[i] = ruf * F.n[0] + rvf * F.n[1] + rwf * F.n[2];
+            continue;
+        }
+        double l = F.lambda, l2 = 1. - l, dx1[3], dx2[3];
+        sub3(dx1, F.xpac, M.cell[c1].x);
+        sub3(dx2, F.xnac, M.cell[c2].x);
+        double roapf = Rn[c1] * Apr[c1] * l + Rn[c2] * Apr[c2] * l2;
+        double vol = M.cell[c1].vol * l + M.cell[c2].vol * l2;
+        double coef = roapf * vol * F.rlencos;
+        double dp1 = dp[c1] + URF[3] * dot3(&dPdX[3 * c1], dx1);
+        double dp2 = dp[c2] + URF[3] * dot3(&dPdX[3 * c2], dx2);
+        RUFace[i] += -coef * (dp2 - dp1);
+    }
+    return 0;
+}
+
+// CalculatePressure (src/navier_pressure.cpp:9-76), DensityModel 0
+int RefNS::CalculatePressure() {
+    for (int i = 0; i < M.ncel; ++i) xsol[i] = 0.;
+    if (BuildPressureMatrix()) return -1;
+    SolveCfg c = cfg_of(P.pres);
+    last_pres = las_solve(Ap, xsol, bp, c);
+    if (last_pres.relres > 1.) return err("pressure iteration cannot converge");
+    SetBCDeltaP(BPre.data(), xsol.data());
+    gradient(M, P.limiter, xsol.data(), BPre.data(), dPdX.data());
+    int pr = P.cell_pressure_ref;
+    for (int i = 0; i < M.ncel; ++i) {
+        Residual[3] += std::fabs(xsol[i] - xsol[pr]) * M.cell[i].vol;
+        Pn[i] = Pn[i] + URF[3] * (xsol[i] - xsol[pr]);
+        double coef = M.cell[i].vol * Apr[i];
+        Un[i] = Un[i] - coef * dPdX[3 * i];
+        Vn[i] = Vn[i] - coef * dPdX[3 * i + 1];
+        Wn[i] = Wn[i] - coef * dPdX[3 * i + 2];
+    }
+    if (SetBCVelocity(BRo.data(), BU.data(), BV.data(), BW.data())) return -1;
+    return CorrectRUFace2(xsol.data());
+}
+
+// specialTreatmentForWallTempreture (src/navier_scalar.cpp:176-248)
+int RefNS::wall_temperature(int icell, int jface, const double *D,
+                            const double *BPhi, double &app, double &fde,
+                            double &fdi, double &fcs) {
+    const Face &F = M.face[jface];
+    int bnd = F.bnd, rid = M.bnd[bnd].rid;
+    Region &R = reg[rid];
+    double s1 = F.n[0], s2 = F.n[1], s3 = F.n[2], RU = RUFace[jface];
+    double Visc = D[icell];
+    const double *gp = &dPhidX[3 * icell];
+    double dxc[3];
+    sub3(dxc, F.x, M.cell[icell].x);
+    if (R.type1 == 1 && R.type2 == 1) {
+        fde = 0. - M.bnd[bnd].q * F.area;
+        fdi = 0.0;
+    } else if (R.type1 == 4) {
+        fde = fdi = 0.0;
+    } else {  // given T, coupled, inlet, outlet: Dirichlet through BPhi
+        double VAL = Visc * F.rlencos;
+        app += VAL;
+        fde = Visc * (gp[0] * s1 + gp[1] * s2 + gp[2] * s3);
+        fdi = VAL * (gp[0] * dxc[0] + gp[1] * dxc[1] + gp[2] * dxc[2] - BPhi[bnd]);
+    }
+    switch (R.type1) {
+    case 1:
+    case 4:
+        fcs = 0.;
+        break;
+    case 2:
+    case 3: {
+        if (R.type1 == 2 && RU > 0) return err("reverse flow get out of inlet");
+        double f = cmin(RU, 0.0);
+        app -= f;
+        fcs = f * BPhi[bnd];
+        break;
+    }
+    default:
+        return err("no this type of boundary");
+    }
+    return 0;
+}
+
+// NavierStokesSolver::BuildScalarMatrix (src/navier_scalar.cpp:466-647)
+int RefNS::BuildScalarMatrix(const double *Phi, const double *BPhi,
+                             const double *D, const double *src,
+                             const double *App, bool wallT) {
+    gradient(M, P.limiter, Phi, BPhi, dPhidX.data());
+    As.constr(M.ncel, false);
+    for (int i = 0; i < M.ncel; ++i) {
+        const Cell &C = M.cell[i];
+        double app = App[i], apn[6] = {0, 0, 0, 0, 0, 0}, sphi = src[i];
+        int ani[6], nj = 0;
+        double fde = 0., fdi = 0., fcs = 0.;
+        for (int j = 0; j < C.nface; ++j) {
+            int iface = C.face[j];
+            const Face &F = M.face[iface];
+            int ip = F.c1, in = F.c2;
+            if (in < 0 && wallT) {
+                if (wall_temperature(i, iface, D, BPhi, app, fde, fdi, fcs)) return -1;
+            } else if (in < 0) {
+                int bnd = F.bnd, rid = M.bnd[bnd].rid, t1 = reg[rid].type1;
+                double s1 = F.n[0], s2 = F.n[1], s3 = F.n[2], RU = RUFace[iface];
+                double Visc = D[i];
+                const double *gp = &dPhidX[3 * i];
+                double dxc[3];
+                sub3(dxc, F.x, C.x);
+                if (t1 < 1 || t1 > 4) return err("no such rid");
+                double VAL = Visc * F.rlencos;
+                app += VAL;
+                fde = Visc * (gp[0] * s1 + gp[1] * s2 + gp[2] * s3);
+                fdi = VAL * (gp[0] * dxc[0] + gp[1] * dxc[1] + gp[2] * dxc[2] - BPhi[bnd]);
+                if (t1 == 2 || t1 == 3) {
+                    if (t1 == 2 && RU > 0) return err("reverse flow get out of inlet");
+                    double f = cmin(RU, 0.0);
+                    app -= f;
+                    fcs = f * BPhi[bnd];
+                } else {
+                    fcs = 0.;
+                }
+            } else {
+                double s1, s2, s3, RU, lambda;
+                if (i != ip) {
+                    in = ip;
+                    s1 = -F.n[0]; s2 = -F.n[1]; s3 = -F.n[2];
+                    RU = -RUFace[iface];
+                    lambda = 1. - F.lambda;
+                } else {
+                    s1 = F.n[0]; s2 = F.n[1]; s3 = F.n[2];
+                    RU = RUFace[iface];
+                    lambda = F.lambda;
+                }
+                double l2 = 1. - lambda;
+                double Visc = lambda * D[i] + l2 * D[in];
+                double gx = lambda * dPhidX[3 * i] + l2 * dPhidX[3 * in];
+                double gy = lambda * dPhidX[3 * i + 1] + l2 * dPhidX[3 * in + 1];
+                double gz = lambda * dPhidX[3 * i + 2] + l2 * dPhidX[3 * in + 2];
+                if (RU < 0.) {
+                    apn[nj] += RU;
+                    app -= RU;
+                }
+                double VAL = Visc * F.rlencos;
+                app += VAL;
+                apn[nj] += -VAL;
+                ani[nj] = in;
+                nj++;
+                int up = RU > 0. ? i : in;
+                double dx[3];
+                sub3(dx, F.x, M.cell[up].x);
+                double pfi = Phi[up];
+                double pfh = Phi[up] + dot3(&dPhidX[3 * up], dx);
+                fcs = RU * (pfh - pfi);
+                fde = Visc * (gx * s1 + gy * s2 + gz * s3);
+                double dxc[3];
+                sub3(dxc, M.cell[in].x, C.x);
+                fdi = VAL * (gx * dxc[0] + gy * dxc[1] + gz * dxc[2]);
+            }
+            sphi += fde - fdi - fcs;
+        }
+        app /= URF[5];
+        As.set_len(i, nj + 1);
+        As.set_entry(i, 0, i, app);
+        for (int j = 0; j < nj; ++j) As.set_entry(i, j + 1, ani[j], apn[j]);
+        bs[i] = sphi + (1. - URF[5]) * app * Phi[i];
+    }
+    return 0;
+}
+
+// UpdateTurKEpsilon (src/navier_scalar.cpp:14-172), steady
+int RefNS::UpdateTurKEpsilon() {
+    int n = M.ncel;
+    std::vector<double> Prod(n), TES(n, 0.), EDS(n, 0.), VisTE(n), VisED(n),
+        ApTE(n, 0.), ApED(n, 0.);
+    for (int i = 0; i < n; ++i) {
+        double vol = M.cell[i].vol;
+        VisTE[i] = VisLam[i] + VisTur[i] / kev::Sigma_k;
+        VisED[i] = VisLam[i] + VisTur[i] / kev::Sigma_e;
+        const double *du = &dUdX[3 * i], *dv = &dVdX[3 * i], *dw = &dWdX[3 * i];
+        double s1 = (du[0] + du[0]) * du[0] + (du[1] + dv[0]) * du[1] + (du[2] + dw[0]) * du[2];
+        double s2 = (dv[0] + du[1]) * dv[0] + (dv[1] + dv[1]) * dv[1] + (dv[2] + dw[1]) * dv[2];
+        double s3 = (dw[0] + du[2]) * dw[0] + (dw[1] + dv[2]) * dw[1] + (dw[2] + dw[2]) * dw[2];
+        Prod[i] = VisTur[i] * (s1 + s2 + s3);
+        double Dis = Rn[i] * ED[i];
+        TES[i] = Prod[i] * vol;
+        ApTE[i] = Dis / TE[i] * vol;
+    }
+    SetBCKEpsilon(TES.data(), EDS.data(), ApTE.data(), ApED.data(), Prod.data());
+    SolveCfg c = cfg_of(P.ke);
+    if (BuildScalarMatrix(TE.data(), BTE.data(), VisTE.data(), TES.data(),
+                          ApTE.data(), false))
+        return -1;
+    xsol = TE;
+    last_k = las_solve(As, xsol, bs, c);
+    if (last_k.iters >= c.maxit && last_k.relres > c.tol) return err("k cannot converge");
+    for (int i = 0; i < n; ++i) TE[i] = cmax(1.e-9, xsol[i]);
+    for (int i = 0; i < n; ++i) {
+        double fact = ED[i] / (TE[i] + SMALL) * M.cell[i].vol;
+        EDS[i] = kev::Ceps1 * fact * Prod[i];
+        ApED[i] += kev::Ceps2 * fact * Rn[i];
+    }
+    if (BuildScalarMatrix(ED.data(), BED.data(), VisED.data(), EDS.data(),
+                          ApED.data(), false))
+        return -1;
+    for (int i = 0; i < M.nbnd; ++i)
+        if (reg[M.bnd[i].rid].type1 == 1) {
+            int ic = M.face[M.bnd[i].face].c1;
+            As.set_len(ic, 1);
+            As.set_entry(ic, 0, ic, 1.);
+            bs[ic] = ED[ic];
+        }
+    xsol = ED;
+    last_e = las_solve(As, xsol, bs, c);
+    if (last_e.iters >= c.maxit && last_e.relres > c.tol) return err("eps cannot converge");
+    for (int i = 0; i < n; ++i) ED[i] = cmax(1.e-12, xsol[i]);
+    for (int i = 0; i < n; ++i)
+        VisTur[i] = ED[i] > 1.e-12 ? kev::Cmu * Rn[i] * TE[i] * TE[i] / (ED[i] + SMALL) : 0.;
+    return 0;
+}
+
+// UpdateEnergy (src/navier_scalar.cpp:257-397), steady, DensityModel 0
+int RefNS::UpdateEnergy() {
+    int n = M.ncel;
+    std::vector<double> ES(n, 0.), ApE(n, 0.);
+    kcond.assign(n, 0.);
+    for (int i = 0; i < n; ++i) {
+        kcond[i] = P.cp * (VisLam[i] / P.prl + VisTur[i] / P.prte) / P.cp;
+        kcond[i] *= 100;
+    }
+    if (SetBCTemperature(BTem.data())) return -1;
+    if (ht) ht->Solid2Fluid(BTem.data(), M.ncb, M.nbnd);
+    if (BuildScalarMatrix(Tn.data(), BTem.data(), kcond.data(), ES.data(),
+                          ApE.data(), true))
+        return -1;
+    xsol = Tn;
+    SolveCfg c = cfg_of(P.energy);
+    last_t = las_solve(As, xsol, bs, c);
+    if (last_t.iters >= c.maxit && last_t.relres > c.tol) return err("energy cannot converge");
+    Tn = xsol;
+    if (ht) ht->Fluid2Solid(*this);
+    return 0;
+}
+
+// NSSolve loop body (src/navier.cpp:33-78), steady
+int RefNS::Iterate(double &resmax) {
+    if (CalculateVelocity()) return -1;
+    if (CalculatePressure()) return -1;
+    if (P.turb_model == 1 && UpdateTurKEpsilon()) return -1;
+    if (P.solve_energy) {
+        if (UpdateEnergy()) return -1;
+        if (ht && ht->UpdateEnergy()) return -1;
+    }
+    double s = 0.;
+    for (int i = 0; i < 10; ++i) s = cmax(s, Residual[i]);  // vec_max
+    resmax = s;
+    return 0;
+}
+
+int RefNS::NSSolve(int &iters) {
+    iters = 0;
+    for (int it = 1; it < P.max_step; ++it) {
+        double r;
+        if (Iterate(r)) return -1;
+        iters = it;
+        if (r < P.residual_steady) break;
+    }
+    return 0;
+}
+
+// ----------------------------------------------------- solid module --
+// HeatConductionSolver::InitFlowField (src/HeatConductionSolver.cpp:269-304)
+int RefHT::InitFlowField() {
+    for (int i = 0; i < M.ncel; ++i) {
+        auto it = reg.find(M.cell[i].rid);
+        if (it == reg.end() || it->second.type1 != 5 || it->second.type2 != 1)
+            return -1;
+        Tn[i] = it->second.init[0];
+        Rn[i] = it->second.init[1];
+        diffCoef[i] = it->second.init[2];
+    }
+    for (int i = 0; i < M.nbnd; ++i) {
+        Region &R = reg[M.bnd[i].rid];
+        if (R.type1 == 1) {
+            if (R.type2 == 0) BTem[i] = R.fixed;
+            else if (R.type2 == 1) M.bnd[i].q = R.fixed;
+            else if (R.type2 == 2) M.bnd[i].q = 0.0;
+            else return -1;
+        } else if (R.type1 == 4) {
+            M.bnd[i].q = 0.0;
+        }
+    }
+    if (SetBCTemperature()) return -1;
+    gradient(M, ns->P.limiter, Tn.data(), BTem.data(), dPhidX.data());
+    return 0;
+}
+
+// HeatConductionSolver::SetBCTemperature (src/HeatConductionSolver.cpp:13-44)
+int RefHT::SetBCTemperature() {
+    for (int i = 0; i < M.nbnd; ++i) {
+        Region &R = reg[M.bnd[i].rid];
+        int iface = M.bnd[i].face, ic = M.face[iface].c1;
+        if (R.type1 == 1) {
+            if (R.type2 == 1)
+                BTem[i] = Tn[ic] - M.bnd[i].q / (diffCoef[ic] * M.face[iface].rlencos);
+            else if (R.type2 == 2)
+                BTem[i] = Tn[ic];
+            else if (R.type2 != 0)
+                return -1;
+        } else if (R.type1 == 4) {
+            BTem[i] = Tn[ic];
+        } else {
+            return -1;
+        }
+    }
+    return 0;
+}
+
+// HeatConductionSolver::BuildScalarMatrix (src/HeatConductionSolver.cpp:110-223)
+int RefHT::BuildScalarMatrix(const double *src) {
+    gradient(M, ns->P.limiter, Tn.data(), BTem.data(), dPhidX.data());
+    As.constr(M.ncel, false);
+    for (int i = 0; i < M.ncel; ++i) {
+        const Cell &C = M.cell[i];
+        double app = 0., apn[6] = {0, 0, 0, 0, 0, 0}, sphi = src[i];
+        int ani[6], nj = 0;
+        for (int j = 0; j < C.nface; ++j) {
+            int iface = C.face[j];
+            const Face &F = M.face[iface];
+            int ip = F.c1, in = F.c2;
+            double fde, fdi;
+            if (in < 0) {
+                int bnd = F.bnd;
+                Region &R = reg[M.bnd[bnd].rid];
+                double s1 = F.n[0], s2 = F.n[1], s3 = F.n[2], Visc = diffCoef[i];
+                const double *gp = &dPhidX[3 * i];
+                double dxc[3];
+                sub3(dxc, F.x, C.x);
+                if (R.type1 == 1 && R.type2 == 0) {
+                    double VAL = Visc * F.rlencos;
+                    app += VAL;
+                    fde = Visc * (gp[0] * s1 + gp[1] * s2 + gp[2] * s3);
+                    fdi = VAL * (gp[0] * dxc[0] + gp[1] * dxc[1] + gp[2] * dxc[2] - BTem[bnd]);
+                } else if (R.type1 == 1 || R.type1 == 4) {
+                    fde = 0. - M.bnd[bnd].q * F.area;
+                    fdi = 0.;
+                } else {
+                    return -1;
+                }
+            } else {
+                double s1, s2, s3, lambda;
+                if (i != ip) {
+                    in = ip;
+                    s1 = -F.n[0]; s2 = -F.n[1]; s3 = -F.n[2];
+                    lambda = 1. - F.lambda;
+                } else {
+                    s1 = F.n[0]; s2 = F.n[1]; s3 = F.n[2];
+                    lambda = F.lambda;
+                }
+                double l2 = 1. - lambda;
+                double Visc = lambda * diffCoef[i] + l2 * diffCoef[in];
+                double gx = lambda * dPhidX[3 * i] + l2 * dPhidX[3 * in];
+                double gy = lambda * dPhidX[3 * i + 1] + l2 * dPhidX[3 * in + 1];
+                double gz = lambda * dPhidX[3 * i + 2] + l2 * dPhidX[3 * in + 2];
+                double VAL = Visc * F.rlencos;
+                app += VAL;
+                apn[nj] += -VAL;
+                ani[nj] = in;
+                nj++;
+                fde = Visc * (gx * s1 + gy * s2 + gz * s3);
+                double dxc[3];
+                sub3(dxc, M.cell[in].x, C.x);
+                fdi = VAL * (gx * dxc[0] + gy * dxc[1] + gz * dxc[2]);
+            }
+            sphi += fde - fdi;
+        }
+        app /= URF;
+        As.set_len(i, nj + 1);
+        As.set_entry(i, 0, i, app);
+        for (int j = 0; j < nj; ++j) As.set_entry(i, j + 1, ani[j], apn[j]);
+        bs[i] = sphi + (1. - URF) * app * Tn[i];
+    }
+    return 0;
+}
+
+// HeatConductionSolver::UpdateEnergy (src/HeatConductionSolver.cpp:47-106)
+int RefHT::UpdateEnergy() {
+    int n = M.ncel;
+    std::vector<double> ES(n, 0.);
+    for (int i = 0; i < n; ++i) ES[i] = ns->P.solid_heat_source * M.cell[i].vol;
+    if (SetBCTemperature()) return -1;
+    if (BuildScalarMatrix(ES.data())) return -1;
+    xsol = Tn;
+    SolveCfg c;
+    c.method = ns->P.solid.method;
+    c.precond = ns->P.solid.precond;
+    c.omega = ns->P.solid.omega;
+    c.tol = ns->P.solid.tol;
+    c.maxit = ns->P.solid.maxit;
+    c.mc = ns->mc;
+    c.colour = &colour;
+    last = las_solve(As, xsol, bs, c);
+    if (last.iters >= c.maxit && last.relres > c.tol) return -1;
+    Tn = xsol;
+    return 0;
+}
+
+// coupledBoundCommunicationSolid2Fluid (src/HeatConductionSolver.cpp:400-427)
+void RefHT::Solid2Fluid(double *bt, int ncb, int nb) {
+    (void)nb;
+    int ifl = ncb;
+    for (int i = M.ncb; i != M.nbnd; ++i) {
+        int iface = M.bnd[i].face, ic = M.face[iface].c1;
+        double dxc[3];
+        sub3(dxc, M.face[iface].x, M.cell[ic].x);
+        double dtdr = dPhidX[3 * ic] * dxc[0] + dPhidX[3 * ic + 1] * dxc[1] +
+                      dPhidX[3 * ic + 2] * dxc[2];
+        bt[ifl++] = Tn[ic] + dtdr;
+    }
+}
+
+// coupledBoundCommunicationFluid2Solid (src/HeatConductionSolver.cpp:373-397)
+void RefHT::Fluid2Solid(RefNS &f) {
+    int ifl = f.M.ncb;
+    for (int i = M.ncb; i != M.nbnd; ++i) {
+        int ff = f.M.bnd[ifl].face, fc = f.M.face[ff].c1;
+        const Face &F = f.M.face[ff];
+        const double *g = &f.dPhidX[3 * fc];
+        double dtdn = (g[0] * F.n[0] + g[1] * F.n[1] + g[2] * F.n[2]) / F.area;
+        f.M.bnd[ifl].q = 0. - dtdn * f.kcond[fc];
+        M.bnd[i].q = 0. - f.M.bnd[ifl].q;
+        ifl++;
+    }
+}
+
+}  // namespace oref

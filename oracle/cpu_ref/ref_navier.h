@@ -1,0 +1,81 @@
+// ORACLE -- TEST INFRASTRUCTURE ONLY (see ref.h).
+#pragma once
+#include <memory>
+#include <string>
+
+#include "ref.h"
+
+namespace oref {
+
+int gradient(const Mesh &M, int limiter, const double *phi, const double *bphi,
+             double *g);
+
+struct RefNS;
+
+// HeatConductionSolver (src/HeatConductionSolver.h:5-64), steady
+struct RefHT {
+    Mesh M;
+    std::map<int, Region> reg;
+    RefNS *ns = nullptr;
+    double URF = 1.0;  // src/Preprocessor.cpp:506
+    std::vector<double> Tn, Rn, diffCoef, dPhidX, BTem, xsol, bs;
+    std::vector<int> colour;
+    QMat As;
+    SolveResult last;
+    int InitFlowField();
+    int SetBCTemperature();
+    int BuildScalarMatrix(const double *src);
+    int UpdateEnergy();
+    void Solid2Fluid(double *bt, int ncb, int nb);
+    void Fluid2Solid(RefNS &f);
+};
+
+// NavierStokesSolver (src/navier.h:13-127), steady, DensityModel 0
+struct RefNS {
+    cfd_params P;
+    std::map<int, Region> reg;
+    Mesh M;
+    std::unique_ptr<RefHT> ht;
+    double URF[8];
+    int mc = 0;  // 1: multicolour SSOR/ILU order (the GPU order)
+    std::vector<int> colour;
+    std::vector<double> Rn, Un, Vn, Wn, Pn, Tn, TE, ED, VisLam, VisTur, Apr;
+    std::vector<double> RUFace, dPdX, dUdX, dVdX, dWdX, dPhidX;
+    std::vector<double> BRo, BU, BV, BW, BPre, BTem, BTE, BED, kcond;
+    std::vector<double> xsol, bu, bv, bw, bp, bs;
+    double Residual[10] = {0};
+    QMat As, Ap;
+    SolveResult last_mom[3], last_pres, last_k, last_e, last_t;
+    std::string error;
+
+    int err(const char *m) {
+        error = m;
+        return -1;
+    }
+    int setup(const cfd_raw_mesh &raw, const cfd_params &p);
+    int InitFlowField();
+    int SetBCVelocity(double *br, double *bu_, double *bv_, double *bw_);
+    int SetBCPressure(double *b);
+    int SetBCDeltaP(double *b, const double *dp);
+    int SetBCTemperature(double *bt);
+    int SetBCKEpsilon(double *TES, double *EDS, double *ApTE, double *ApED,
+                      double *Prod);
+    int BuildVelocityMatrix();
+    int CalculateVelocity();
+    int CalRUFace2();
+    int BuildPressureMatrix();
+    int CorrectRUFace2(const double *dp);
+    int CalculatePressure();
+    int wall_temperature(int icell, int jface, const double *D,
+                         const double *BPhi, double &app, double &fde,
+                         double &fdi, double &fcs);
+    int BuildScalarMatrix(const double *Phi, const double *BPhi, const double *D,
+                          const double *src, const double *App, bool wallT);
+    int UpdateTurKEpsilon();
+    int UpdateEnergy();
+    int Iterate(double &resmax);
+    int NSSolve(int &iters);
+    SolveCfg cfg_of(const cfd_solve_cfg &c) const;
+};
+
+}  // namespace oref

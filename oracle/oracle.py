@@ -1,0 +1,182 @@
+"""ORACLE -- TEST INFRASTRUCTURE ONLY.
+
+ctypes loader for oracle/liboracle.so, the CPU restatement of the
+reference's SIMPLE + LASPack path (oracle/cpu_ref). Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import this
+module, and only to check or time the product; the product never calls it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+import sysu_cfd_heat_amd as P
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-j8", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        dp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p
+        L.oracle_last_error.restype = C.c_char_p
+        L.oracle_mesh_build.restype = vp
+        L.oracle_mesh_build.argtypes = [C.POINTER(P.RawMeshC)]
+        L.oracle_mesh_free.argtypes = [vp]
+        L.oracle_mesh_counts.argtypes = [vp, ip]
+        L.oracle_mesh_get_int.argtypes = [vp, C.c_char_p, ip]
+        L.oracle_mesh_get_dbl.argtypes = [vp, C.c_char_p, dp]
+        L.oracle_ns_create.restype = vp
+        L.oracle_ns_create.argtypes = [C.POINTER(P.RawMeshC), C.POINTER(P.ParamsC), C.c_int]
+        L.oracle_ns_free.argtypes = [vp]
+        L.oracle_ns_counts.argtypes = [vp, C.c_int, ip]
+        L.oracle_ns_mesh_int.argtypes = [vp, C.c_int, C.c_char_p, ip]
+        L.oracle_ns_mesh_dbl.argtypes = [vp, C.c_int, C.c_char_p, dp]
+        L.oracle_ns_call.argtypes = [vp, C.c_char_p]
+        L.oracle_ns_field.argtypes = [vp, C.c_int, C.c_char_p, dp, C.c_size_t, C.c_int]
+        L.oracle_ns_matrix.argtypes = [vp, C.c_char_p, ip, ip, dp]
+        L.oracle_ns_last_solve.argtypes = [vp, C.c_char_p, ip, dp]
+        L.oracle_ns_time_iterations.restype = C.c_double
+        L.oracle_ns_time_iterations.argtypes = [vp, C.c_int, dp]
+        L.oracle_las_solve.argtypes = ([C.c_int, C.c_int, ip, ip, dp, dp, dp] + [C.c_int] * 2 +
+                                       [C.c_double, C.c_double] + [C.c_int] * 2 + [ip, dp])
+        L.oracle_las_mul.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp]
+        L.oracle_las_precond.argtypes = ([C.c_int, C.c_int, ip, ip, dp, C.c_int, C.c_double,
+                                          C.c_int, dp, dp])
+        L.oracle_colour.argtypes = [C.c_int, ip, ip, ip]
+        _lib = L
+    return _lib
+
+
+def _err():
+    return lib().oracle_last_error().decode(errors="replace")
+
+
+class Mesh:
+    """Restated CreateFaces + CellFaceInfo of one mesh."""
+
+    def __init__(self, raw: P.RawMesh):
+        L = lib()
+        m = raw.c()
+        self.h = L.oracle_mesh_build(C.byref(m))
+        if not self.h:
+            raise RuntimeError(_err())
+        c = (C.c_int * 5)()
+        L.oracle_mesh_counts(self.h, c)
+        self.counts = tuple(c)
+
+    def get(self, name):
+        n = P._array_len(name, self.counts)
+        if name in P.INT_ARRAYS:
+            k = P.INT_ARRAYS[name]
+            out = np.zeros(n * k, np.int32)
+            rc = lib().oracle_mesh_get_int(self.h, name.encode(), P._ip(out))
+        else:
+            k = P.DBL_ARRAYS[name]
+            out = np.zeros(n * k)
+            rc = lib().oracle_mesh_get_dbl(self.h, name.encode(), P._dp(out))
+        if rc:
+            raise RuntimeError(_err())
+        return out.reshape(n, k) if k > 1 else out
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.oracle_mesh_free(self.h)
+
+
+class NSSolver(P.NSSolver):
+    """Restated NavierStokesSolver with the product's Python interface.
+    mc=1 applies SSOR / ILU(0) in the product's multicolour order."""
+
+    _prefix = "oracle_ns_"
+
+    def __init__(self, raw, params, mc: int = 1):
+        self._mc = mc
+        super().__init__(raw, params)
+
+    def _load(self):
+        return lib()
+
+    def _error(self):
+        return _err()
+
+    def _create(self, m, params):
+        return self._L.oracle_ns_create(C.byref(m), C.byref(params), self._mc)
+
+    def mesh_get(self, name, part=0):
+        n = P._array_len(name, self.counts(part))
+        if name in P.INT_ARRAYS:
+            k = P.INT_ARRAYS[name]
+            out = np.zeros(n * k, np.int32)
+            rc = lib().oracle_ns_mesh_int(self.h, part, name.encode(), P._ip(out))
+        else:
+            k = P.DBL_ARRAYS[name]
+            out = np.zeros(n * k)
+            rc = lib().oracle_ns_mesh_dbl(self.h, part, name.encode(), P._dp(out))
+        if rc:
+            raise RuntimeError(_err())
+        return out.reshape(n, k) if k > 1 else out
+
+    def time_iterations(self, n: int):
+        r = C.c_double()
+        t = lib().oracle_ns_time_iterations(self.h, n, C.byref(r))
+        if t < 0:
+            raise RuntimeError(_err())
+        return t, r.value
+
+
+def rows_from_dense(A: np.ndarray, sym: bool = False):
+    """LASPack row form (diagonal first) of a dense matrix."""
+    n = A.shape[0]
+    rl, col, val = [], [], []
+    for i in range(n):
+        cs = [i] + [j for j in range(n) if j != i and A[i, j] != 0 and (not sym or j > i)]
+        rl.append(len(cs))
+        col += cs
+        val += [A[i, j] for j in cs]
+    return (np.array(rl, np.int32), np.array(col, np.int32), np.array(val, np.float64))
+
+
+def las_solve(rows, b, x0, method, precond, omega=1.0, tol=1e-10, maxit=1000, mc=0,
+              sym=False):
+    rl, col, val = rows
+    n = len(rl)
+    x = np.array(x0, np.float64).copy()
+    b = np.ascontiguousarray(b, np.float64)
+    it, rr = C.c_int(), C.c_double()
+    lib().oracle_las_solve(n, int(sym), P._ip(rl), P._ip(col), P._dp(val), P._dp(b), P._dp(x),
+                           method, precond, omega, tol, maxit, mc, C.byref(it), C.byref(rr))
+    return x, it.value, rr.value
+
+
+def las_precond(rows, c, precond, omega=1.0, mc=0, sym=False):
+    rl, col, val = rows
+    n = len(rl)
+    c = np.ascontiguousarray(c, np.float64)
+    y = np.zeros(n)
+    lib().oracle_las_precond(n, int(sym), P._ip(rl), P._ip(col), P._dp(val), precond, omega,
+                             mc, P._dp(c), P._dp(y))
+    return y
+
+
+def colour(adj_lists):
+    n = len(adj_lists)
+    ptr = np.zeros(n + 1, np.int32)
+    for i, a in enumerate(adj_lists):
+        ptr[i + 1] = ptr[i] + len(a)
+    adj = np.array([j for a in adj_lists for j in a] or [0], np.int32)
+    out = np.zeros(n, np.int32)
+    lib().oracle_colour(n, P._ip(ptr), P._ip(adj), P._ip(out))
+    return out

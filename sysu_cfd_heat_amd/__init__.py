@@ -1,0 +1,377 @@
+"""MI355X-native SIMPLE assembly + Krylov engine (SYSU_CFD_HEAT hot path).
+
+The product is the C-ABI shared library ``libsysu_cfd_heat_amd.so`` built from
+``csrc/`` for gfx950: a LASPack-compatible solver ABI (``include/laspack.h``)
+and a device-resident SIMPLE engine (``include/cfd_solver.h``). This module
+is a thin ctypes loader over it; it never falls back to a CPU path: if the
+library is missing, importing the bindings raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsysu_cfd_heat_amd.so")
+
+CFD_CG, CFD_BICGSTAB, CFD_GMRES = 0, 1, 2
+CFD_PC_NONE, CFD_PC_JACOBI, CFD_PC_SSOR, CFD_PC_ILU = 0, 1, 2, 3
+MAX_REGIONS = 16
+
+
+class RawMeshC(C.Structure):
+    _fields_ = [("nvrt", C.c_int), ("ncel", C.c_int), ("nbnd", C.c_int),
+                ("vert", C.POINTER(C.c_double)), ("cell_vert", C.POINTER(C.c_int)),
+                ("cell_rid", C.POINTER(C.c_int)), ("bnd_vert", C.POINTER(C.c_int)),
+                ("bnd_rid", C.POINTER(C.c_int))]
+
+
+class RegionC(C.Structure):
+    _fields_ = [("id", C.c_int), ("type1", C.c_int), ("type2", C.c_int),
+                ("fixed", C.c_double), ("init", C.c_double * 10)]
+
+
+class SolveCfgC(C.Structure):
+    _fields_ = [("method", C.c_int), ("precond", C.c_int), ("omega", C.c_double),
+                ("tol", C.c_double), ("maxit", C.c_int)]
+
+
+class ParamsC(C.Structure):
+    _fields_ = [("nregion", C.c_int), ("region", RegionC * MAX_REGIONS),
+                ("steady", C.c_int), ("max_step", C.c_int), ("residual_steady", C.c_double),
+                ("solve_energy", C.c_int), ("turb_model", C.c_int),
+                ("density_model", C.c_int), ("cht", C.c_int),
+                ("gravity", C.c_double * 3), ("pressure_ref", C.c_double),
+                ("cell_pressure_ref", C.c_int), ("urf", C.c_double * 8),
+                ("limiter", C.c_int), ("prl", C.c_double), ("prte", C.c_double),
+                ("cp", C.c_double), ("mom", SolveCfgC), ("pres", SolveCfgC),
+                ("ke", SolveCfgC), ("energy", SolveCfgC), ("solid", SolveCfgC),
+                ("solid_heat_source", C.c_double), ("pres_precision", C.c_int),
+                ("multi_rhs", C.c_int), ("pres_project", C.c_int)]
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+            "(make -C sysu_cfd_heat_amd/csrc); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    dp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p
+    L.cfd_last_error.restype = C.c_char_p
+    L.cfd_params_default.argtypes = [C.POINTER(ParamsC)]
+    L.cfd_mesh_box_counts.argtypes = [C.c_int] * 4 + [ip] * 3
+    L.cfd_mesh_box.argtypes = ([C.c_int] * 3 + [C.c_double] * 3 + [C.c_int, C.c_double,
+                               C.c_uint64, ip, C.c_int, C.c_int, dp, dp, ip, ip, ip, ip])
+    L.cfd_gmsh_write.argtypes = [C.c_char_p, C.POINTER(RawMeshC)]
+    L.cfd_gmsh_read_counts.argtypes = [C.c_char_p, ip, ip, ip]
+    L.cfd_gmsh_read.argtypes = [C.c_char_p, dp, ip, ip, ip, ip]
+    L.cfd_mesh_build.restype = vp
+    L.cfd_mesh_build.argtypes = [C.POINTER(RawMeshC)]
+    L.cfd_mesh_build_part.restype = vp
+    L.cfd_mesh_build_part.argtypes = [C.POINTER(RawMeshC), C.POINTER(ParamsC), C.c_int]
+    L.cfd_mesh_free.argtypes = [vp]
+    L.cfd_mesh_counts.argtypes = [vp, ip]
+    L.cfd_mesh_get_int.argtypes = [vp, C.c_char_p, ip]
+    L.cfd_mesh_get_dbl.argtypes = [vp, C.c_char_p, dp]
+    L.cfd_ns_create.restype = vp
+    L.cfd_ns_create.argtypes = [C.POINTER(RawMeshC), C.POINTER(ParamsC)]
+    L.cfd_ns_free.argtypes = [vp]
+    L.cfd_ns_counts.argtypes = [vp, C.c_int, ip]
+    L.cfd_ns_call.argtypes = [vp, C.c_char_p]
+    L.cfd_ns_field.argtypes = [vp, C.c_int, C.c_char_p, dp, C.c_size_t, C.c_int]
+    L.cfd_ns_matrix.argtypes = [vp, C.c_char_p, ip, ip, dp]
+    L.cfd_ns_last_solve.argtypes = [vp, C.c_char_p, ip, dp]
+    L.cfd_ns_stats.argtypes = [vp, dp]
+    L.cfd_ns_reset_stats.argtypes = [vp]
+    L.cfd_ns_profile.argtypes = [vp, C.c_int]
+    L.cfd_ns_profile_read.argtypes = [vp, dp]
+    _lib = L
+    return L
+
+
+def _err(L) -> str:
+    return L.cfd_last_error().decode(errors="replace")
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+# ------------------------------------------------------------------ mesh --
+class RawMesh:
+    """The mesh as the reference's ReadGridFile leaves it (8-vertex cells,
+    4-vertex boundary faces, region ids); numpy-owned."""
+
+    def __init__(self, vert, cell_vert, cell_rid, bnd_vert, bnd_rid):
+        self.vert = np.ascontiguousarray(vert, dtype=np.float64).reshape(-1, 3)
+        self.cell_vert = np.ascontiguousarray(cell_vert, dtype=np.int32).reshape(-1, 8)
+        self.cell_rid = np.ascontiguousarray(cell_rid, dtype=np.int32).reshape(-1)
+        self.bnd_vert = np.ascontiguousarray(bnd_vert, dtype=np.int32).reshape(-1, 4)
+        self.bnd_rid = np.ascontiguousarray(bnd_rid, dtype=np.int32).reshape(-1)
+
+    @property
+    def counts(self):
+        return len(self.vert), len(self.cell_vert), len(self.bnd_vert)
+
+    def c(self) -> RawMeshC:
+        return RawMeshC(len(self.vert), len(self.cell_vert), len(self.bnd_vert),
+                        _dp(self.vert), _ip(self.cell_vert), _ip(self.cell_rid),
+                        _ip(self.bnd_vert), _ip(self.bnd_rid))
+
+    def write_gmsh(self, path: str) -> None:
+        L = lib()
+        m = self.c()
+        if L.cfd_gmsh_write(path.encode(), C.byref(m)):
+            raise RuntimeError(_err(L))
+
+    @staticmethod
+    def read_gmsh(path: str) -> "RawMesh":
+        L = lib()
+        nv, nc, nb = C.c_int(), C.c_int(), C.c_int()
+        if L.cfd_gmsh_read_counts(path.encode(), C.byref(nv), C.byref(nc), C.byref(nb)):
+            raise RuntimeError(_err(L))
+        v = np.zeros((nv.value, 3)); cv = np.zeros((nc.value, 8), np.int32)
+        cr = np.zeros(nc.value, np.int32); bv = np.zeros((nb.value, 4), np.int32)
+        br = np.zeros(nb.value, np.int32)
+        if L.cfd_gmsh_read(path.encode(), _dp(v), _ip(cv), _ip(cr), _ip(bv), _ip(br)):
+            raise RuntimeError(_err(L))
+        return RawMesh(v, cv, cr, bv, br)
+
+
+def box_mesh(nx: int, ny: int, nz: int, lx=1.0, ly=1.0, lz=1.0, tets=False,
+             jitter=0.0, seed=1, side_rid: Sequence[int] = (1, 1, 1, 1, 1, 1),
+             fluid_rid=5, solid_rid=6, block: Optional[Sequence[float]] = None) -> RawMesh:
+    """Structured box of hexahedra (or 6 Kuhn tetrahedra per hexahedron)."""
+    L = lib()
+    nv, nc, nb = C.c_int(), C.c_int(), C.c_int()
+    if L.cfd_mesh_box_counts(nx, ny, nz, int(tets), C.byref(nv), C.byref(nc), C.byref(nb)):
+        raise RuntimeError(_err(L))
+    v = np.zeros((nv.value, 3)); cv = np.zeros((nc.value, 8), np.int32)
+    cr = np.zeros(nc.value, np.int32); bv = np.zeros((nb.value, 4), np.int32)
+    br = np.zeros(nb.value, np.int32)
+    sr = np.ascontiguousarray(side_rid, dtype=np.int32)
+    blk = None if block is None else np.ascontiguousarray(block, dtype=np.float64)
+    rc = L.cfd_mesh_box(nx, ny, nz, lx, ly, lz, int(tets), float(jitter), int(seed), _ip(sr),
+                        fluid_rid, solid_rid, None if blk is None else _dp(blk),
+                        _dp(v), _ip(cv), _ip(cr), _ip(bv), _ip(br))
+    if rc:
+        raise RuntimeError(_err(L))
+    return RawMesh(v, cv, cr, bv, br)
+
+
+INT_ARRAYS = {"face_vert": 4, "face_cell1": 1, "face_cell2": 1, "face_bnd": 1,
+              "cell_nface": 1, "cell_face": 6, "cell_cell": 6, "cell_vert": 8,
+              "cell_rid": 1, "bnd_face": 1, "bnd_rid": 1, "cell_colour": 1}
+DBL_ARRAYS = {"vert": 3, "face_x": 3, "face_n": 3, "face_area": 1, "face_lambda": 1,
+              "face_rlencos": 1, "face_xpac": 3, "face_xnac": 3, "cell_x": 3,
+              "cell_vol": 1, "bnd_distance": 1}
+
+
+def _array_len(name: str, counts) -> int:
+    nv, nc, nf, nb = counts[:4]
+    if name.startswith("face"):
+        return nf
+    if name.startswith("cell"):
+        return nc
+    if name.startswith("bnd"):
+        return nb
+    return nv
+
+
+class Mesh:
+    """Face construction + geometry of one mesh (host side of the product)."""
+
+    def __init__(self, raw: RawMesh, params: Optional[ParamsC] = None, part: int = 0):
+        L = lib()
+        self._raw = raw
+        m = raw.c()
+        if params is None:
+            self.h = L.cfd_mesh_build(C.byref(m))
+        else:  # fluid (0) / solid (1) part of the conjugate-heat-transfer split
+            self.h = L.cfd_mesh_build_part(C.byref(m), C.byref(params), part)
+        if not self.h:
+            raise RuntimeError(_err(L))
+        c = (C.c_int * 5)()
+        L.cfd_mesh_counts(self.h, c)
+        self.counts = tuple(c)
+
+    def get(self, name: str) -> np.ndarray:
+        L = lib()
+        n = _array_len(name, self.counts)
+        if name in INT_ARRAYS:
+            k = INT_ARRAYS[name]
+            out = np.zeros(n * k, np.int32)
+            rc = L.cfd_mesh_get_int(self.h, name.encode(), _ip(out))
+        else:
+            k = DBL_ARRAYS[name]
+            out = np.zeros(n * k)
+            rc = L.cfd_mesh_get_dbl(self.h, name.encode(), _dp(out))
+        if rc:
+            raise RuntimeError(_err(L))
+        return out.reshape(n, k) if k > 1 else out
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.cfd_mesh_free(self.h)
+            self.h = None
+
+
+# ---------------------------------------------------------------- params --
+def default_params() -> ParamsC:
+    p = ParamsC()
+    lib().cfd_params_default(C.byref(p))
+    return p
+
+
+def add_region(p: ParamsC, rid: int, type1: int, type2: int = 0, fixed: float = 0.0,
+               init: Sequence[float] = ()) -> None:
+    r = p.region[p.nregion]
+    r.id, r.type1, r.type2, r.fixed = rid, type1, type2, fixed
+    for k, v in enumerate(init):
+        r.init[k] = v
+    p.nregion += 1
+
+
+def solve_cfg(method: int, precond: int, omega: float, tol: float, maxit: int) -> SolveCfgC:
+    return SolveCfgC(method, precond, omega, tol, maxit)
+
+
+# ---------------------------------------------------------------- solver --
+FIELD_LEN = {  # name -> (count kind, components)
+    "Rn": ("c", 1), "Un": ("c", 1), "Vn": ("c", 1), "Wn": ("c", 1), "Pn": ("c", 1),
+    "Tn": ("c", 1), "TE": ("c", 1), "ED": ("c", 1), "VisLam": ("c", 1), "VisTur": ("c", 1),
+    "Apr": ("c", 1), "RUFace": ("f", 1), "dPdX": ("c", 3), "dUdX": ("c", 3), "dVdX": ("c", 3),
+    "dWdX": ("c", 3), "dPhidX": ("c", 3), "BRo": ("b", 1), "BU": ("b", 1), "BV": ("b", 1),
+    "BW": ("b", 1), "BPre": ("b", 1), "BTem": ("b", 1), "BTE": ("b", 1), "BED": ("b", 1),
+    "BndQ": ("b", 1), "xsol": ("c", 1), "bu": ("c", 1), "bv": ("c", 1), "bw": ("c", 1),
+    "bp": ("c", 1), "bs": ("c", 1), "diffCoef": ("c", 1), "Residual": ("r", 10)}
+
+
+def field_size(name: str, counts) -> int:
+    kind, k = FIELD_LEN[name]
+    if kind == "r":
+        return 10
+    n = {"c": counts[1], "f": counts[2], "b": counts[3]}[kind]
+    return n * k
+
+
+class NSSolver:
+    """Device-resident SIMPLE engine (fluid part 0, solid part 1)."""
+
+    _prefix = "cfd_ns_"
+
+    def __init__(self, raw: RawMesh, params: ParamsC):
+        self._L = self._load()
+        self._raw = raw
+        m = raw.c()
+        self.h = self._create(m, params)
+        if not self.h:
+            raise RuntimeError(self._error())
+        self.has_solid = bool(params.cht)
+
+    # hooks so the oracle wrapper can reuse this class
+    def _load(self):
+        return lib()
+
+    def _error(self) -> str:
+        return _err(self._L)
+
+    def _create(self, m, params):
+        return self._L.cfd_ns_create(C.byref(m), C.byref(params))
+
+    def _fn(self, name):
+        return getattr(self._L, self._prefix + name)
+
+    def counts(self, part: int = 0):
+        c = (C.c_int * 5)()
+        if self._fn("counts")(self.h, part, c):
+            raise RuntimeError(self._error())
+        return tuple(c)
+
+    def call(self, method: str) -> None:
+        if self._fn("call")(self.h, method.encode()):
+            raise RuntimeError(f"{method}: {self._error()}")
+
+    def get(self, name: str, part: int = 0) -> np.ndarray:
+        n = field_size(name, self.counts(part))
+        out = np.zeros(n)
+        if self._fn("field")(self.h, part, name.encode(), _dp(out), n, 0):
+            raise RuntimeError(self._error())
+        k = FIELD_LEN[name][1]
+        return out.reshape(-1, 3) if k == 3 else out
+
+    def set(self, name: str, values, part: int = 0) -> None:
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if self._fn("field")(self.h, part, name.encode(), _dp(a), a.size, 1):
+            raise RuntimeError(self._error())
+
+    def matrix(self, which: str):
+        """(row_len, col, val) in LASPack row form, diagonal first."""
+        part = 1 if which == "SolidAs" else 0
+        n = self.counts(part)[1]
+        rl = np.zeros(n, np.int32)
+        if self._fn("matrix")(self.h, which.encode(), _ip(rl), None, None):
+            raise RuntimeError(self._error())
+        tot = int(rl.sum())
+        col = np.zeros(tot, np.int32)
+        val = np.zeros(tot)
+        if self._fn("matrix")(self.h, which.encode(), _ip(rl), _ip(col), _dp(val)):
+            raise RuntimeError(self._error())
+        return rl, col, val
+
+    def last_solve(self, eq: str):
+        it, rr = C.c_int(), C.c_double()
+        if self._fn("last_solve")(self.h, eq.encode(), C.byref(it), C.byref(rr)):
+            raise RuntimeError(self._error())
+        return it.value, rr.value
+
+    def stats(self) -> np.ndarray:
+        out = np.zeros(8)
+        self._fn("stats")(self.h, _dp(out))
+        return out
+
+    def profile(self, enable: bool) -> None:
+        self._L.cfd_ns_profile(self.h, int(enable))
+
+    def profile_read(self):
+        out = np.zeros(4)
+        self._L.cfd_ns_profile_read(self.h, _dp(out))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._fn("free")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dense_from_rows(row_len, col, val, n: int, sym: bool = False) -> np.ndarray:
+    """Dense matrix from LASPack row form (upper triangle mirrored if sym)."""
+    A = np.zeros((n, n))
+    k = 0
+    for i in range(n):
+        for _ in range(int(row_len[i])):
+            j = int(col[k]); A[i, j] += val[k]
+            if sym and j != i:
+                A[j, i] += val[k]
+            k += 1
+    return A

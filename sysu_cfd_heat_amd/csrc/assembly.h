@@ -1,0 +1,120 @@
+// Device face/cell loops of the SIMPLE step (kernels in assembly.hip).
+#pragma once
+#include "common.h"
+#include "krylov.h"
+
+namespace cfd {
+
+constexpr int kMaxRegions = 16;
+
+// Region table (BdRegion, src/BasicType.h:69-95) indexed by region slot.
+struct RegTab {
+    int n = 0;
+    int id[kMaxRegions];
+    int type1[kMaxRegions];
+    int type2[kMaxRegions];
+    double fixed[kMaxRegions];
+    double init[kMaxRegions][10];
+};
+
+// Device mirror of one (sub)mesh, structure of arrays. Per-cell lists are
+// slot-major ([slot][ncel]) so consecutive cells read consecutive words.
+struct DevMesh {
+    int ncel = 0, nfac = 0, nbnd = 0, ncb = 0, nvrt = 0, K = 0;
+    const int *f_c1, *f_c2, *f_bnd;
+    const double *f_lambda, *f_area, *f_rlencos;
+    const double *f_n[3], *f_x[3], *f_xp[3], *f_xn[3];
+    const double *c_vol, *c_x[3];
+    const int *c_nface;  // [C]
+    const int *c_face;   // [6][C] faces in slot order
+    const int *c_gface;  // [6][C] faces in ascending id (gradient order)
+    const int *c_bface;  // [6][C] boundary faces in ascending boundary index
+    const int *c_vert;   // [8][C]
+    const int *ell_col;  // [K][C] neighbour of the k-th interior face (pad: self)
+    const double *v_x[3];
+    const int *v2c_ptr, *v2c;  // vertex -> cells (MLP limiter)
+    const int *b_face, *b_cell, *b_reg;
+    const double *b_dist;
+    const RegTab *reg;     // device copy
+};
+
+// Scalars produced on the device by SetBCVelocity's global mass balance.
+struct BcScal {
+    double massin, massout, areaout, rate;
+    int mode;  // 1: outlet rescale, 2: additive correction
+};
+
+// All launches go to stream s. part/tk: reduction scratch (12*kRedGrid).
+struct Red {
+    double *part;
+    unsigned *tk;
+};
+
+void launch_gradient(const DevMesh &m, int nf, const double *const *phi,
+                     const double *const *bphi, double *const *out, int limiter,
+                     double *tmp, hipStream_t s);
+void launch_bc_velocity(const DevMesh &m, const double *R, const double *U, const double *V,
+                        const double *W, double *BR, double *BU, double *BV, double *BW,
+                        BcScal *sc, Red red, hipStream_t s);
+void launch_bc_pressure(const DevMesh &m, const double *P, double *BP, hipStream_t s);
+void launch_bc_temperature(const DevMesh &m, const double *T, double *BT, hipStream_t s);
+
+struct VelArgs {
+    const double *R, *U, *V, *W, *VisLam, *VisTur, *RUFace, *gU, *gV, *gW, *gP;
+    const double *BU, *BV, *BW;
+    double urf, grav[3];
+    double *diag, *val, *bu, *bv, *bw, *Apr;
+};
+void launch_build_velocity(const DevMesh &m, const VelArgs &a, int *errflag, hipStream_t s);
+
+void launch_rhie_chow(const DevMesh &m, const double *R, const double *U, const double *V,
+                      const double *W, const double *P, const double *gP, const double *Apr,
+                      const double *BR, const double *BU, const double *BV, const double *BW,
+                      double *RUFace, hipStream_t s);
+void launch_build_pressure(const DevMesh &m, const double *R, const double *Apr,
+                           const double *RUFace, double *diag, double *val, double *bp,
+                           hipStream_t s);
+void launch_correct_ruface(const DevMesh &m, const double *R, const double *Apr,
+                           const double *dp, const double *gdp, const double *BR,
+                           const double *BU, const double *BV, const double *BW, double urf_p,
+                           double *RUFace, hipStream_t s);
+// P += urf (x - x_ref); U -= vol Apr dPdX ; Residual[3] += |x - x_ref| vol
+void launch_pressure_update(const DevMesh &m, const double *x, int ref, double urf_p,
+                            const double *Apr, const double *gx, double *P, double *U,
+                            double *V, double *W, double *res_slot, Red red, hipStream_t s);
+// res_slot += sum |fld - x| vol ; fld = x (clip applied to x first if lo set)
+void launch_take_solution(const DevMesh &m, const double *x, double *fld, double *res_slot,
+                          double clip_lo, int clip, Red red, hipStream_t s);
+
+struct ScaArgs {
+    const double *Phi, *BPhi, *D, *src, *App, *RUFace, *g, *BndQ;
+    double urf;
+    int mode;  // 0: k/eps boundary rules, 1: energy wall-temperature rules, 2: solid
+    double *diag, *val, *bs;
+};
+void launch_build_scalar(const DevMesh &m, const ScaArgs &a, int *errflag, hipStream_t s);
+
+struct KeArgs {
+    const double *R, *VisLam, *gU, *gV, *gW, *U, *V, *W;
+    double *VisTur, *TE, *ED, *BTE, *BED;
+    double *Prod, *TES, *EDS, *ApTE, *ApED, *VisTE, *VisED;
+};
+void launch_ke_sources(const DevMesh &m, const KeArgs &a, hipStream_t s);
+void launch_ke_wall(const DevMesh &m, const KeArgs &a, hipStream_t s);
+void launch_ke_eps_sources(const DevMesh &m, const KeArgs &a, hipStream_t s);
+void launch_eps_wall_rows(const DevMesh &m, const double *ED, double *diag, double *val,
+                          double *bs, hipStream_t s);
+void launch_ke_visc(const DevMesh &m, const KeArgs &a, hipStream_t s);
+void launch_kcond(const DevMesh &m, const double *VisLam, const double *VisTur, double cp,
+                  double prl, double prte, double *kcond, hipStream_t s);
+// CHT exchange (src/HeatConductionSolver.cpp:373-427)
+void launch_solid2fluid(const DevMesh &sm, const double *Ts, const double *gTs,
+                        int fluid_ncb, double *BTem_f, hipStream_t s);
+void launch_fluid2solid(const DevMesh &fm, const DevMesh &sm, const double *gTf,
+                        const double *kcond, double *q_f, double *q_s, hipStream_t s);
+void launch_solid_bc_temperature(const DevMesh &m, const double *T, const double *D,
+                                 const double *q, double *BT, hipStream_t s);
+void launch_fill(double *p, size_t n, double v, hipStream_t s);
+void launch_scale_source(const DevMesh &m, double q, double *src, hipStream_t s);
+
+}  // namespace cfd

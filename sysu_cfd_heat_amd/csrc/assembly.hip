@@ -1,0 +1,983 @@
+// Face and cell loops of the SIMPLE step, one thread per cell / face /
+// boundary face. Every loop is a gather (no atomics): a cell visits its own
+// faces, so FP64 sums keep the reference's order and are bit-identical to it
+// when compiled with -ffp-contract=off (global reductions excepted).
+#include "assembly.h"
+
+namespace cfd {
+
+namespace {
+
+constexpr double kSmall = 1.e-16;  // SMALL (src/BasicType.h:15)
+constexpr double kKappa = 0.419, kCmu = 0.09, kCeps1 = 1.44, kCeps2 = 1.92, kSigK = 1.0,
+                 kSigE = 1.3;  // TurKEpsilonVar (src/navier.h:129-140)
+
+__device__ __forceinline__ double cmax(double x, double y) { return x > y ? x : y; }
+__device__ __forceinline__ double cmin(double x, double y) { return x < y ? x : y; }
+// vec_dot (src/tools.cpp:16-22): s = 0; s += a_k b_k
+__device__ __forceinline__ double vdot(double a0, double a1, double a2, double b0, double b1,
+                                       double b2) {
+    double s = 0.;
+    s += a0 * b0;
+    s += a1 * b1;
+    s += a2 * b2;
+    return s;
+}
+
+#define CELL_LOOP(i, n) \
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+
+// ----------------------------------------------------------- Gradient --
+// CycasSolver::Gradient (src/BasicSolver.cpp:29-63): Green-Gauss sum over a
+// cell's faces in ascending face id (the reference's scatter order).
+struct GradTab {
+    const double *phi[4];
+    const double *bphi[4];
+    double *out[4];
+};
+
+template <int NF>
+__global__ void k_gradient(DevMesh m, GradTab tb) {
+    const double *phi[NF], *bphi[NF];
+    double *out[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+        phi[k] = tb.phi[k];
+        bphi[k] = tb.bphi[k];
+        out[k] = tb.out[k];
+    }
+    const int C = m.ncel;
+    CELL_LOOP(i, C) {
+        double a[NF][3];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) a[k][0] = a[k][1] = a[k][2] = 0.;
+        const int nf = m.c_nface[i];
+        for (int t = 0; t < nf; ++t) {
+            const int f = m.c_gface[t * C + i];
+            const int c1 = m.f_c1[f], c2 = m.f_c2[f];
+            const double n0 = m.f_n[0][f], n1 = m.f_n[1][f], n2 = m.f_n[2][f];
+            if (c2 >= 0) {
+                const double l = m.f_lambda[f];
+#pragma unroll
+                for (int k = 0; k < NF; ++k) {
+                    double pf = l * phi[k][c1] + (1. - l) * phi[k][c2];
+                    if (i == c1) {
+                        a[k][0] += pf * n0;
+                        a[k][1] += pf * n1;
+                        a[k][2] += pf * n2;
+                    } else {
+                        a[k][0] -= pf * n0;
+                        a[k][1] -= pf * n1;
+                        a[k][2] -= pf * n2;
+                    }
+                }
+            } else {
+                const int b = m.f_bnd[f];
+#pragma unroll
+                for (int k = 0; k < NF; ++k) {
+                    double pf = bphi[k][b];
+                    a[k][0] += pf * n0;
+                    a[k][1] += pf * n1;
+                    a[k][2] += pf * n2;
+                }
+            }
+        }
+        const double vol = m.c_vol[i];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            out[k][i] = a[k][0] / vol;
+            out[k][C + i] = a[k][1] / vol;
+            out[k][2 * C + i] = a[k][2] / vol;
+        }
+    }
+}
+
+// Barth_fun (src/BasicSolver.cpp:88-98)
+__device__ __forceinline__ double barth(double us, double umin, double umax) {
+    if (us > 1.0e-10) return cmin(1., umax / us);
+    if (us < -1.0e-10) return cmin(1., umin / us);
+    return 1.;
+}
+
+// Limiter_Barth (src/BasicSolver.cpp:182-224) / Limiter_MLP (:118-180):
+// mlp=0 uses face-neighbour extrema, mlp=1 vertex extrema vmin/vmax.
+__global__ void k_lim_barth(DevMesh m, int mlp, const double *__restrict__ phi,
+                            const double *__restrict__ vmin, const double *__restrict__ vmax,
+                            double *__restrict__ g) {
+    const int C = m.ncel;
+    CELL_LOOP(i, C) {
+        const double ui = phi[i];
+        double umax = ui, umin = ui;
+        if (!mlp) {
+            for (int j = 0; j < m.c_nface[i]; ++j) {
+                int f = m.c_face[j * C + i];
+                int in = m.f_c1[f] == i ? m.f_c2[f] : m.f_c1[f];
+                if (in < 0) continue;
+                umax = cmax(umax, phi[in]);
+                umin = cmin(umin, phi[in]);
+            }
+            umax -= ui;
+            umin -= ui;
+        }
+        const double gx = g[i], gy = g[C + i], gz = g[2 * C + i];
+        const double cx = m.c_x[0][i], cy = m.c_x[1][i], cz = m.c_x[2][i];
+        double fi = 10.;
+        for (int j = 0; j < 8; ++j) {
+            int iv = m.c_vert[j * C + i];
+            double lo = umin, hi = umax;
+            if (mlp) {
+                hi = vmax[iv] - ui;
+                lo = vmin[iv] - ui;
+            }
+            double dx = m.v_x[0][iv] - cx, dy = m.v_x[1][iv] - cy, dz = m.v_x[2][iv] - cz;
+            double us = gx * dx + gy * dy + gz * dz;
+            fi = cmin(fi, barth(us, lo, hi));
+        }
+        g[i] = fi * gx;
+        g[C + i] = fi * gy;
+        g[2 * C + i] = fi * gz;
+    }
+}
+
+// vertex extrema over the cells that list the vertex (order-independent)
+__global__ void k_vertex_minmax(int nv, const int *__restrict__ ptr, const int *__restrict__ cells,
+                                const double *__restrict__ phi, double *__restrict__ vmin,
+                                double *__restrict__ vmax) {
+    CELL_LOOP(v, nv) {
+        double hi = -1.e8, lo = 1.e8;
+        for (int e = ptr[v]; e < ptr[v + 1]; ++e) {
+            double u = phi[cells[e]];
+            hi = cmax(hi, u);
+            lo = cmin(lo, u);
+        }
+        vmax[v] = hi;
+        vmin[v] = lo;
+    }
+}
+
+// Limiter_WENO (src/BasicSolver.cpp:226-288): reads g, writes t
+__global__ void k_lim_weno(DevMesh m, const double *__restrict__ g, double *__restrict__ t) {
+    const int C = m.ncel;
+    CELL_LOOP(i, C) {
+        double ss = 0., a0 = 0., a1 = 0., a2 = 0.;
+        for (int j = 0; j < m.c_nface[i]; ++j) {
+            int f = m.c_face[j * C + i];
+            int ic = m.f_c1[f] == i ? m.f_c2[f] : m.f_c1[f];
+            if (ic < 0) continue;
+            double g0 = g[ic], g1 = g[C + ic], g2 = g[2 * C + ic];
+            double wei = 1. / (g0 * g0 + g1 * g1 + g2 * g2 + 1.e-16);
+            ss += wei;
+            a0 += wei * g0;
+            a1 += wei * g1;
+            a2 += wei * g2;
+        }
+        double g0 = g[i], g1 = g[C + i], g2 = g[2 * C + i];
+        double wei = 1. / (g0 * g0 + g1 * g1 + g2 * g2 + 1.e-16);
+        ss += wei;
+        a0 += wei * g0;
+        a1 += wei * g1;
+        a2 += wei * g2;
+        t[i] = a0 / ss;
+        t[C + i] = a1 / ss;
+        t[2 * C + i] = a2 / ss;
+    }
+}
+
+__global__ void k_copy_n(size_t n, const double *__restrict__ a, double *__restrict__ b) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+__global__ void k_fill(size_t n, double v, double *__restrict__ p) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ------------------------------------------------------------- BCs --
+// SetBCVelocity (src/navier_bc.cpp:8-72): boundary values + mass balance sums
+__global__ void k_bc_velocity(DevMesh m, const double *__restrict__ R,
+                              const double *__restrict__ U, const double *__restrict__ V,
+                              const double *__restrict__ W, double *__restrict__ BR,
+                              double *__restrict__ BU, double *__restrict__ BV,
+                              double *__restrict__ BW, BcScal *sc, double *part, unsigned *tk) {
+    double acc[3] = {0., 0., 0.};
+    CELL_LOOP(b, m.nbnd) {
+        const int r = m.b_reg[b], t1 = m.reg->type1[r], f = m.b_face[b], ic = m.b_cell[b];
+        double br, bu, bv, bw;
+        const double n0 = m.f_n[0][f], n1 = m.f_n[1][f], n2 = m.f_n[2][f];
+        if (t1 == 1) {
+            br = R[ic]; bu = 0.; bv = 0.; bw = 0.;
+        } else if (t1 == 2) {
+            br = m.reg->init[r][4]; bu = m.reg->init[r][0]; bv = m.reg->init[r][1];
+            bw = m.reg->init[r][2];
+        } else if (t1 == 3) {
+            br = R[ic]; bu = U[ic]; bv = V[ic]; bw = W[ic];
+        } else {
+            const double a = m.f_area[f];
+            double s1 = n0 / a, s2 = n1 / a, s3 = n2 / a;
+            double un = U[ic] * s1 + V[ic] * s2 + W[ic] * s3;
+            bu = U[ic] - un * s1;
+            bv = V[ic] - un * s2;
+            bw = W[ic] - un * s3;
+            br = R[ic];
+        }
+        BR[b] = br; BU[b] = bu; BV[b] = bv; BW[b] = bw;
+        if (t1 == 2) acc[0] += br * (bu * n0 + bv * n1 + bw * n2);
+        else if (t1 == 3) {
+            acc[1] += br * (bu * n0 + bv * n1 + bw * n2);
+            acc[2] += br * m.f_area[b];  // Face[i] with the boundary index, as the reference
+        }
+    }
+    if (grid_sum<3>(acc, part, tk)) {
+        sc->massin = acc[0];
+        sc->massout = acc[1];
+        sc->areaout = acc[2];
+        if (fabs(acc[1]) > kSmall) {
+            sc->rate = -acc[0] / acc[1];
+            sc->mode = 1;
+        } else {
+            sc->rate = (-acc[0] - acc[1]) / acc[2];
+            sc->mode = 2;
+        }
+    }
+}
+
+// outlet correction (src/navier_bc.cpp:74-106)
+__global__ void k_bc_outlet(DevMesh m, const BcScal *sc, double *__restrict__ BU,
+                            double *__restrict__ BV, double *__restrict__ BW) {
+    const double rate = sc->rate;
+    const int mode = sc->mode;
+    CELL_LOOP(b, m.nbnd) {
+        if (m.reg->type1[m.b_reg[b]] != 3) continue;
+        if (mode == 1) {
+            BU[b] *= rate;
+            BV[b] *= rate;
+            BW[b] *= rate;
+        } else {
+            const int f = m.b_face[b];
+            const double a = m.f_area[f];
+            BU[b] += rate * m.f_n[0][f] / a;
+            BV[b] += rate * m.f_n[1][f] / a;
+            BW[b] += rate * m.f_n[2][f] / a;
+        }
+    }
+}
+
+// SetBCPressure / SetBCDeltaP (src/navier_bc.cpp:110-162): copy the cell value
+__global__ void k_bc_copy(DevMesh m, const double *__restrict__ P, double *__restrict__ BP) {
+    CELL_LOOP(b, m.nbnd) BP[b] = P[m.b_cell[b]];
+}
+
+// NavierStokesSolver::SetBCTemperature (src/navier_bc.cpp:164-203)
+__global__ void k_bc_temperature(DevMesh m, const double *__restrict__ T, double *__restrict__ BT) {
+    CELL_LOOP(b, m.nbnd) {
+        const int r = m.b_reg[b], t1 = m.reg->type1[r], t2 = m.reg->type2[r];
+        if ((t1 == 1 && t2 == 1) || t1 == 3 || t1 == 4) BT[b] = T[m.b_cell[b]];
+    }
+}
+
+// --------------------------------------------------------- assembly --
+// BuildVelocityMatrix (src/navier_velocity.cpp:95-355), steady
+__global__ void k_build_velocity(DevMesh m, VelArgs a, int *errflag) {
+    const int C = m.ncel, K = m.K;
+    CELL_LOOP(i, C) {
+        double app = 0., su = 0., sv = 0., sw = 0.;
+        int nj = 0;
+        const double cx = m.c_x[0][i], cy = m.c_x[1][i], cz = m.c_x[2][i];
+        const double ui = a.U[i], vi = a.V[i], wi = a.W[i];
+        const int nface = m.c_nface[i];
+        for (int j = 0; j < nface; ++j) {
+            const int f = m.c_face[j * C + i];
+            const int ip = m.f_c1[f];
+            int in = m.f_c2[f];
+            double fde0, fde1, fde2, fdi0, fdi1, fdi2, fcs0, fcs1, fcs2;
+            if (in < 0) {
+                const int b = m.f_bnd[f], r = m.b_reg[b], t1 = m.reg->type1[r];
+                const double s1 = m.f_n[0][f], s2 = m.f_n[1][f], s3 = m.f_n[2][f];
+                const double RU = a.RUFace[f];
+                const double Visc = a.VisLam[i] + a.VisTur[i];
+                const double gu0 = a.gU[i], gu1 = a.gU[C + i], gu2 = a.gU[2 * C + i];
+                const double gv0 = a.gV[i], gv1 = a.gV[C + i], gv2 = a.gV[2 * C + i];
+                const double gw0 = a.gW[i], gw1 = a.gW[C + i], gw2 = a.gW[2 * C + i];
+                const double d0 = m.f_x[0][f] - cx, d1 = m.f_x[1][f] - cy, d2 = m.f_x[2][f] - cz;
+                const double VAL = Visc * m.f_rlencos[f];
+                if (t1 == 1) {
+                    app += VAL;
+                    fde0 = VAL * ui; fde1 = VAL * vi; fde2 = VAL * wi;
+                    fdi0 = VAL * ui; fdi1 = VAL * vi; fdi2 = VAL * wi;
+                } else if (t1 >= 2 && t1 <= 4) {
+                    app += VAL;
+                    fde0 = (gu0 + gu0) * s1 + (gu1 + gv0) * s2 + (gu2 + gw0) * s3;
+                    fde1 = (gv0 + gu1) * s1 + (gv1 + gv1) * s2 + (gv2 + gw1) * s3;
+                    fde2 = (gw0 + gu2) * s1 + (gw1 + gv2) * s2 + (gw2 + gw2) * s3;
+                    fde0 *= Visc; fde1 *= Visc; fde2 *= Visc;
+                    fdi0 = VAL * (gu0 * d0 + gu1 * d1 + gu2 * d2 - a.BU[b]);
+                    fdi1 = VAL * (gv0 * d0 + gv1 * d1 + gv2 * d2 - a.BV[b]);
+                    fdi2 = VAL * (gw0 * d0 + gw1 * d1 + gw2 * d2 - a.BW[b]);
+                } else {
+                    *errflag = 1;
+                    fde0 = fde1 = fde2 = fdi0 = fdi1 = fdi2 = 0.;
+                }
+                if (t1 == 2 || t1 == 3) {
+                    const double fl = cmin(RU, 0.0);
+                    app -= fl;
+                    fcs0 = fl * a.BU[b]; fcs1 = fl * a.BV[b]; fcs2 = fl * a.BW[b];
+                } else {
+                    fcs0 = 0.; fcs1 = 0.; fcs2 = 0.;
+                }
+            } else {
+                double s1, s2, s3, RU, lambda;
+                if (i != ip) {
+                    in = ip;
+                    s1 = -m.f_n[0][f]; s2 = -m.f_n[1][f]; s3 = -m.f_n[2][f];
+                    RU = -a.RUFace[f];
+                    lambda = 1. - m.f_lambda[f];
+                } else {
+                    s1 = m.f_n[0][f]; s2 = m.f_n[1][f]; s3 = m.f_n[2][f];
+                    RU = a.RUFace[f];
+                    lambda = m.f_lambda[f];
+                }
+                const double l2 = 1. - lambda;
+                const double Visc = lambda * (a.VisLam[i] + a.VisTur[i]) +
+                                    l2 * (a.VisLam[in] + a.VisTur[in]);
+                const double g0 = lambda * a.gU[i] + l2 * a.gU[in];
+                const double g1 = lambda * a.gU[C + i] + l2 * a.gU[C + in];
+                const double g2 = lambda * a.gU[2 * C + i] + l2 * a.gU[2 * C + in];
+                const double g3 = lambda * a.gV[i] + l2 * a.gV[in];
+                const double g4 = lambda * a.gV[C + i] + l2 * a.gV[C + in];
+                const double g5 = lambda * a.gV[2 * C + i] + l2 * a.gV[2 * C + in];
+                const double g6 = lambda * a.gW[i] + l2 * a.gW[in];
+                const double g7 = lambda * a.gW[C + i] + l2 * a.gW[C + in];
+                const double g8 = lambda * a.gW[2 * C + i] + l2 * a.gW[2 * C + in];
+                double apn = 0.;
+                if (RU < 0.) {
+                    apn += RU;
+                    app -= RU;
+                }
+                const double VAL = Visc * m.f_rlencos[f];
+                app += VAL;
+                apn += -VAL;
+                if (nj < K) a.val[(size_t)nj * C + i] = apn;
+                nj++;
+                const int up = RU > 0. ? i : in;
+                const double dx0 = m.f_x[0][f] - m.c_x[0][up], dx1 = m.f_x[1][f] - m.c_x[1][up],
+                             dx2 = m.f_x[2][f] - m.c_x[2][up];
+                const double ufi = a.U[up], vfi = a.V[up], wfi = a.W[up];
+                const double ufh = ufi + vdot(a.gU[up], a.gU[C + up], a.gU[2 * C + up], dx0, dx1, dx2);
+                const double vfh = vfi + vdot(a.gV[up], a.gV[C + up], a.gV[2 * C + up], dx0, dx1, dx2);
+                const double wfh = wfi + vdot(a.gW[up], a.gW[C + up], a.gW[2 * C + up], dx0, dx1, dx2);
+                fcs0 = RU * (ufh - ufi); fcs1 = RU * (vfh - vfi); fcs2 = RU * (wfh - wfi);
+                fde0 = (g0 + g0) * s1 + (g1 + g3) * s2 + (g2 + g6) * s3;
+                fde1 = (g3 + g1) * s1 + (g4 + g4) * s2 + (g5 + g7) * s3;
+                fde2 = (g6 + g2) * s1 + (g7 + g5) * s2 + (g8 + g8) * s3;
+                fde0 *= Visc; fde1 *= Visc; fde2 *= Visc;
+                const double e0 = m.c_x[0][in] - cx, e1 = m.c_x[1][in] - cy, e2 = m.c_x[2][in] - cz;
+                fdi0 = VAL * (g0 * e0 + g1 * e1 + g2 * e2);
+                fdi1 = VAL * (g3 * e0 + g4 * e1 + g5 * e2);
+                fdi2 = VAL * (g6 * e0 + g7 * e1 + g8 * e2);
+            }
+            su += fde0 - fdi0 - fcs0;
+            sv += fde1 - fdi1 - fcs1;
+            sw += fde2 - fdi2 - fcs2;
+        }
+        for (int k = nj; k < K; ++k) a.val[(size_t)k * C + i] = 0.;
+        app /= a.urf;
+        a.Apr[i] = 1. / (app);
+        a.diag[i] = app;
+        const double vol = m.c_vol[i], R = a.R[i];
+        a.bu[i] = su + (1. - a.urf) * app * ui + (-a.gP[i] + R * a.grav[0]) * vol;
+        a.bv[i] = sv + (1. - a.urf) * app * vi + (-a.gP[C + i] + R * a.grav[1]) * vol;
+        a.bw[i] = sw + (1. - a.urf) * app * wi + (-a.gP[2 * C + i] + R * a.grav[2]) * vol;
+    }
+}
+
+// CalRUFace2 (src/navier_velocity.cpp:389-442)
+__global__ void k_rhie_chow(DevMesh m, const double *__restrict__ R, const double *__restrict__ U,
+                            const double *__restrict__ V, const double *__restrict__ W,
+                            const double *__restrict__ P, const double *__restrict__ gP,
+                            const double *__restrict__ Apr, const double *__restrict__ BR,
+                            const double *__restrict__ BU, const double *__restrict__ BV,
+                            const double *__restrict__ BW, double *__restrict__ RUFace) {
+    const int C = m.ncel;
+    CELL_LOOP(f, m.nfac) {
+        const int c1 = m.f_c1[f], c2 = m.f_c2[f];
+        const double n0 = m.f_n[0][f], n1 = m.f_n[1][f], n2 = m.f_n[2][f];
+        if (c2 < 0) {
+            const int b = m.f_bnd[f];
+            double ruf = BU[b] * BR[b], rvf = BV[b] * BR[b], rwf = BW[b] * BR[b];
+            RUFace[f] = ruf * n0 + rvf * n1 + rwf * n2;
+            continue;
+        }
+        const double l = m.f_lambda[f], l2 = 1. - l;
+        const double rf = l * R[c1] + l2 * R[c2];
+        const double ruf = l * R[c1] * U[c1] + l2 * R[c2] * U[c2];
+        const double rvf = l * R[c1] * V[c1] + l2 * R[c2] * V[c2];
+        const double rwf = l * R[c1] * W[c1] + l2 * R[c2] * W[c2];
+        double ru = ruf * n0 + rvf * n1 + rwf * n2;
+        const double px0 = m.f_xp[0][f], px1 = m.f_xp[1][f], px2 = m.f_xp[2][f];
+        const double nx0 = m.f_xn[0][f], nx1 = m.f_xn[1][f], nx2 = m.f_xn[2][f];
+        const double P1 = P[c1] + vdot(gP[c1], gP[C + c1], gP[2 * C + c1], px0 - m.c_x[0][c1],
+                                       px1 - m.c_x[1][c1], px2 - m.c_x[2][c1]);
+        const double P2 = P[c2] + vdot(gP[c2], gP[C + c2], gP[2 * C + c2], nx0 - m.c_x[0][c2],
+                                       nx1 - m.c_x[1][c2], nx2 - m.c_x[2][c2]);
+        const double dx0 = nx0 - px0, dx1 = nx1 - px1, dx2 = nx2 - px2;
+        const double d12 = sqrt(vdot(dx0, dx1, dx2, dx0, dx1, dx2));
+        const double aprf = l * Apr[c1] + l2 * Apr[c2];
+        const double vol = l * m.c_vol[c1] + l2 * m.c_vol[c2];
+        const double dpx = l * gP[c1] + l2 * gP[c2];
+        const double dpy = l * gP[C + c1] + l2 * gP[C + c2];
+        const double dpz = l * gP[2 * C + c1] + l2 * gP[2 * C + c2];
+        const double dpn = dpx * dx0 + dpy * dx1 + dpz * dx2;
+        ru -= rf * m.f_area[f] * aprf / d12 * vol * ((P2 - P1) - dpn);
+        RUFace[f] = ru;
+    }
+}
+
+// BuildPressureMatrix (src/navier_pressure.cpp:88-195), DensityModel 0.
+// Full symmetric pattern; off-diagonals take the value the reference stores
+// in the upper triangle (computed from the face's first cell).
+__global__ void k_build_pressure(DevMesh m, const double *__restrict__ R,
+                                 const double *__restrict__ Apr, const double *__restrict__ RUFace,
+                                 double *__restrict__ diag, double *__restrict__ val,
+                                 double *__restrict__ bp) {
+    const int C = m.ncel, K = m.K;
+    CELL_LOOP(i, C) {
+        double valcen = 0., bpv = 0.;
+        int nj = 0;
+        const int nface = m.c_nface[i];
+        for (int j = 0; j < nface; ++j) {
+            const int f = m.c_face[j * C + i];
+            const int c1 = m.f_c1[f], c2 = m.f_c2[f];
+            if (i == c1) bpv += RUFace[f];
+            else bpv -= RUFace[f];
+            if (c2 < 0) continue;
+            const int in = i == c1 ? c2 : c1;
+            const double fl = m.f_lambda[f], rl = m.f_rlencos[f];
+            const double lambda = i == c1 ? fl : 1. - fl;
+            const double l2 = 1. - lambda;
+            const double roapf = R[i] * Apr[i] * lambda + R[in] * Apr[in] * l2;
+            const double vol = m.c_vol[i] * lambda + m.c_vol[in] * l2;
+            const double tmp = roapf * vol * rl;
+            valcen += -tmp;
+            double off = tmp;
+            if (i != c1) {  // value of the face's first cell (the stored upper entry)
+                const double ro1 = R[c1] * Apr[c1] * fl + R[c2] * Apr[c2] * (1. - fl);
+                const double v1 = m.c_vol[c1] * fl + m.c_vol[c2] * (1. - fl);
+                off = ro1 * v1 * rl;
+            }
+            if (nj < K) val[(size_t)nj * C + i] = off;
+            nj++;
+        }
+        for (int k = nj; k < K; ++k) val[(size_t)k * C + i] = 0.;
+        diag[i] = valcen;
+        bp[i] = bpv;
+    }
+}
+
+// CorrectRUFace2 (src/navier_pressure.cpp:200-242), DensityModel 0
+__global__ void k_correct_ruface(DevMesh m, const double *__restrict__ R,
+                                 const double *__restrict__ Apr, const double *__restrict__ dp,
+                                 const double *__restrict__ g, const double *__restrict__ BR,
+                                 const double *__restrict__ BU, const double *__restrict__ BV,
+                                 const double *__restrict__ BW, double urf,
+                                 double *__restrict__ RUFace) {
+    const int C = m.ncel;
+    CELL_LOOP(f, m.nfac) {
+        const int c1 = m.f_c1[f], c2 = m.f_c2[f];
+        if (c2 < 0) {
+            const int b = m.f_bnd[f];
+            double ruf = BU[b] * BR[b], rvf = BV[b] * BR[b], rwf = BW[b] * BR[b];
+            RUFace[f] = ruf * m.f_n[0][f] + rvf * m.f_n[1][f] + rwf * m.f_n[2][f];
+            continue;
+        }
+        const double l = m.f_lambda[f], l2 = 1. - l;
+        const double a0 = m.f_xp[0][f] - m.c_x[0][c1], a1 = m.f_xp[1][f] - m.c_x[1][c1],
+                     a2 = m.f_xp[2][f] - m.c_x[2][c1];
+        const double b0 = m.f_xn[0][f] - m.c_x[0][c2], b1 = m.f_xn[1][f] - m.c_x[1][c2],
+                     b2 = m.f_xn[2][f] - m.c_x[2][c2];
+        const double roapf = R[c1] * Apr[c1] * l + R[c2] * Apr[c2] * l2;
+        const double vol = m.c_vol[c1] * l + m.c_vol[c2] * l2;
+        const double coef = roapf * vol * m.f_rlencos[f];
+        const double dp1 = dp[c1] + urf * vdot(g[c1], g[C + c1], g[2 * C + c1], a0, a1, a2);
+        const double dp2 = dp[c2] + urf * vdot(g[c2], g[C + c2], g[2 * C + c2], b0, b1, b2);
+        RUFace[f] += -coef * (dp2 - dp1);
+    }
+}
+
+// CalculatePressure update loop (src/navier_pressure.cpp:38-56)
+__global__ void k_pressure_update(DevMesh m, const double *__restrict__ x, int ref, double urf,
+                                  const double *__restrict__ Apr, const double *__restrict__ g,
+                                  double *__restrict__ P, double *__restrict__ U,
+                                  double *__restrict__ V, double *__restrict__ W,
+                                  double *res_slot, double *part, unsigned *tk) {
+    const int C = m.ncel;
+    const double xr = x[ref];
+    double acc[1] = {0.};
+    CELL_LOOP(i, C) {
+        const double vol = m.c_vol[i], d = x[i] - xr;
+        acc[0] += fabs(d) * vol;
+        P[i] = P[i] + urf * d;
+        const double coef = vol * Apr[i];
+        U[i] = U[i] - coef * g[i];
+        V[i] = V[i] - coef * g[C + i];
+        W[i] = W[i] - coef * g[2 * C + i];
+    }
+    if (grid_sum<1>(acc, part, tk)) *res_slot += acc[0];
+}
+
+__global__ void k_take_solution(int C, const double *__restrict__ vol, const double *__restrict__ x,
+                                double *__restrict__ fld, double *res_slot, double lo, int clip,
+                                double *part, unsigned *tk) {
+    double acc[1] = {0.};
+    CELL_LOOP(i, C) {
+        double xi = x[i];
+        if (clip) xi = cmax(lo, xi);
+        if (res_slot) acc[0] += fabs(fld[i] - xi) * vol[i];
+        fld[i] = xi;
+    }
+    if (res_slot && grid_sum<1>(acc, part, tk)) *res_slot += acc[0];
+}
+
+// BuildScalarMatrix: NavierStokesSolver's (src/navier_scalar.cpp:480-645),
+// with the wall-temperature callback (:176-248) in mode 1, and
+// HeatConductionSolver's (src/HeatConductionSolver.cpp:120-221) in mode 2.
+__global__ void k_build_scalar(DevMesh m, ScaArgs a, int *errflag) {
+    const int C = m.ncel, K = m.K;
+    CELL_LOOP(i, C) {
+        double app = a.App ? a.App[i] : 0., sphi = a.src ? a.src[i] : 0.;
+        int nj = 0;
+        const double cx = m.c_x[0][i], cy = m.c_x[1][i], cz = m.c_x[2][i];
+        const double gi0 = a.g[i], gi1 = a.g[C + i], gi2 = a.g[2 * C + i];
+        const int nface = m.c_nface[i];
+        for (int j = 0; j < nface; ++j) {
+            const int f = m.c_face[j * C + i];
+            const int ip = m.f_c1[f];
+            int in = m.f_c2[f];
+            double fde, fdi, fcs;
+            if (in < 0) {
+                const int b = m.f_bnd[f], r = m.b_reg[b];
+                const int t1 = m.reg->type1[r], t2 = m.reg->type2[r];
+                const double s1 = m.f_n[0][f], s2 = m.f_n[1][f], s3 = m.f_n[2][f];
+                const double RU = a.RUFace ? a.RUFace[f] : 0.;
+                const double Visc = a.D[i];
+                const double d0 = m.f_x[0][f] - cx, d1 = m.f_x[1][f] - cy, d2 = m.f_x[2][f] - cz;
+                bool dirichlet;
+                if (a.mode == 0) dirichlet = true;
+                else if (a.mode == 1) dirichlet = !((t1 == 1 && t2 == 1) || t1 == 4);
+                else dirichlet = (t1 == 1 && t2 == 0);
+                if (dirichlet) {
+                    const double VAL = Visc * m.f_rlencos[f];
+                    app += VAL;
+                    fde = Visc * (gi0 * s1 + gi1 * s2 + gi2 * s3);
+                    fdi = VAL * (gi0 * d0 + gi1 * d1 + gi2 * d2 - a.BPhi[b]);
+                } else if (t1 == 4 && a.mode == 1) {
+                    fde = fdi = 0.0;
+                } else {  // given flux: source -q area
+                    fde = 0. - a.BndQ[b] * m.f_area[f];
+                    fdi = 0.;
+                }
+                if (a.mode != 2 && (t1 == 2 || t1 == 3)) {
+                    if (t1 == 2 && RU > 0) *errflag = 2;  // reverse flow out of an inlet
+                    const double fl = cmin(RU, 0.0);
+                    app -= fl;
+                    fcs = fl * a.BPhi[b];
+                } else {
+                    fcs = 0.;
+                }
+            } else {
+                double s1, s2, s3, RU, lambda;
+                if (i != ip) {
+                    in = ip;
+                    s1 = -m.f_n[0][f]; s2 = -m.f_n[1][f]; s3 = -m.f_n[2][f];
+                    RU = a.RUFace ? -a.RUFace[f] : 0.;
+                    lambda = 1. - m.f_lambda[f];
+                } else {
+                    s1 = m.f_n[0][f]; s2 = m.f_n[1][f]; s3 = m.f_n[2][f];
+                    RU = a.RUFace ? a.RUFace[f] : 0.;
+                    lambda = m.f_lambda[f];
+                }
+                const double l2 = 1. - lambda;
+                const double Visc = lambda * a.D[i] + l2 * a.D[in];
+                const double gx = lambda * gi0 + l2 * a.g[in];
+                const double gy = lambda * gi1 + l2 * a.g[C + in];
+                const double gz = lambda * gi2 + l2 * a.g[2 * C + in];
+                double apn = 0.;
+                if (RU < 0.) {
+                    apn += RU;
+                    app -= RU;
+                }
+                const double VAL = Visc * m.f_rlencos[f];
+                app += VAL;
+                apn += -VAL;
+                if (nj < K) a.val[(size_t)nj * C + i] = apn;
+                nj++;
+                if (a.mode != 2) {
+                    const int up = RU > 0. ? i : in;
+                    const double pfi = a.Phi[up];
+                    const double pfh = pfi + vdot(a.g[up], a.g[C + up], a.g[2 * C + up],
+                                                  m.f_x[0][f] - m.c_x[0][up],
+                                                  m.f_x[1][f] - m.c_x[1][up],
+                                                  m.f_x[2][f] - m.c_x[2][up]);
+                    fcs = RU * (pfh - pfi);
+                } else {
+                    fcs = 0.;
+                }
+                fde = Visc * (gx * s1 + gy * s2 + gz * s3);
+                const double e0 = m.c_x[0][in] - cx, e1 = m.c_x[1][in] - cy, e2 = m.c_x[2][in] - cz;
+                fdi = VAL * (gx * e0 + gy * e1 + gz * e2);
+            }
+            sphi += fde - fdi - fcs;
+        }
+        for (int k = nj; k < K; ++k) a.val[(size_t)k * C + i] = 0.;
+        app /= a.urf;
+        a.diag[i] = app;
+        a.bs[i] = sphi + (1. - a.urf) * app * a.Phi[i];
+    }
+}
+
+// ------------------------------------------------------------- k-eps --
+// UpdateTurKEpsilon source loop (src/navier_scalar.cpp:36-67)
+__global__ void k_ke_sources(DevMesh m, KeArgs a) {
+    const int C = m.ncel;
+    CELL_LOOP(i, C) {
+        const double vol = m.c_vol[i];
+        a.VisTE[i] = a.VisLam[i] + a.VisTur[i] / kSigK;
+        a.VisED[i] = a.VisLam[i] + a.VisTur[i] / kSigE;
+        const double u0 = a.gU[i], u1 = a.gU[C + i], u2 = a.gU[2 * C + i];
+        const double v0 = a.gV[i], v1 = a.gV[C + i], v2 = a.gV[2 * C + i];
+        const double w0 = a.gW[i], w1 = a.gW[C + i], w2 = a.gW[2 * C + i];
+        const double s1 = (u0 + u0) * u0 + (u1 + v0) * u1 + (u2 + w0) * u2;
+        const double s2 = (v0 + u1) * v0 + (v1 + v1) * v1 + (v2 + w1) * v2;
+        const double s3 = (w0 + u2) * w0 + (w1 + v2) * w1 + (w2 + w2) * w2;
+        a.Prod[i] = a.VisTur[i] * (s1 + s2 + s3);
+        const double Dis = a.R[i] * a.ED[i];
+        a.TES[i] = a.Prod[i] * vol;
+        a.ApTE[i] = Dis / a.TE[i] * vol;
+        a.EDS[i] = 0.;
+        a.ApED[i] = 0.;
+    }
+}
+
+// SetBCKEpsilon wall loop (src/navier_bc.cpp:217-256), per cell over its wall
+// faces in boundary order (the reference's sequential semantics per cell)
+__global__ void k_ke_wall(DevMesh m, KeArgs a) {
+    const int C = m.ncel;
+    const double Cmu25 = pow(kCmu, 0.25), Cmu75 = pow(kCmu, 0.75);
+    CELL_LOOP(ic, C) {
+        for (int t = 0; t < 6; ++t) {
+            const int b = m.c_bface[t * C + ic];
+            if (b < 0) break;
+            if (m.reg->type1[m.b_reg[b]] != 1) continue;
+            const int f = m.b_face[b];
+            a.BTE[b] = a.TE[ic];
+            a.BED[b] = a.ED[ic];
+            const double vol = m.c_vol[ic], ar = m.f_area[f];
+            const double s1 = m.f_n[0][f] / (ar + 1.e-16), s2 = m.f_n[1][f] / (ar + 1.e-16),
+                         s3 = m.f_n[2][f] / (ar + 1.e-16);
+            const double U = a.U[ic], V = a.V[ic], W = a.W[ic];
+            const double vnor = U * s1 + V * s2 + W * s3;
+            const double vt1 = U - vnor * s1, vt2 = V - vnor * s2, vt3 = W - vnor * s3;
+            const double vel = sqrt(vt1 * vt1 + vt2 * vt2 + vt3 * vt3);
+            const double d = m.b_dist[b], TE = a.TE[ic], R = a.R[ic], mu = a.VisLam[ic];
+            double utau = Cmu25 * sqrt(TE);
+            const double yplus = d * utau * R / (mu + kSmall);
+            const double tauw = mu * vel / d;
+            if (yplus < 11.) utau = sqrt(tauw / R);
+            else utau = Cmu25 * sqrt(TE);
+            const double prod = tauw * utau / (kKappa * d);
+            a.Prod[ic] = prod;
+            const double eps = Cmu75 * pow(TE, 1.5) / (kKappa * d);
+            a.TES[ic] = prod * vol;
+            double ap = a.ApTE[ic];
+            ap -= R * a.ED[ic] / TE * vol;
+            ap += R * eps / TE * vol;
+            a.ApTE[ic] = ap;
+            a.ED[ic] = eps;
+            a.BED[b] = eps;
+        }
+    }
+}
+
+// SetBCKEpsilon inlet/outlet/symmetric loops (src/navier_bc.cpp:259-294):
+// compares the region id itself with 2, 3, 4 (as the reference does)
+__global__ void k_ke_bnd(DevMesh m, KeArgs a) {
+    CELL_LOOP(b, m.nbnd) {
+        const int r = m.b_reg[b], rid = m.reg->id[r], ic = m.b_cell[b];
+        if (rid == 2) {
+            a.BTE[b] = m.reg->init[r][6];
+            a.BED[b] = m.reg->init[r][7];
+        } else if (rid == 3 || rid == 4) {
+            a.BTE[b] = a.TE[ic];
+            a.BED[b] = a.ED[ic];
+        }
+    }
+}
+
+// epsilon sources from the new k (src/navier_scalar.cpp:116-122)
+__global__ void k_ke_eps_sources(DevMesh m, KeArgs a) {
+    CELL_LOOP(i, m.ncel) {
+        const double fact = a.ED[i] / (a.TE[i] + kSmall) * m.c_vol[i];
+        a.EDS[i] = kCeps1 * fact * a.Prod[i];
+        a.ApED[i] += kCeps2 * fact * a.R[i];
+    }
+}
+
+// wall-cell rows of the epsilon system become identity (src/navier_scalar.cpp:130-139)
+__global__ void k_eps_wall_rows(DevMesh m, const double *__restrict__ ED, double *__restrict__ diag,
+                                double *__restrict__ val, double *__restrict__ bs) {
+    const int C = m.ncel;
+    CELL_LOOP(b, m.nbnd) {
+        if (m.reg->type1[m.b_reg[b]] != 1) continue;
+        const int ic = m.b_cell[b];
+        diag[ic] = 1.;
+        for (int k = 0; k < m.K; ++k) val[(size_t)k * C + ic] = 0.;
+        bs[ic] = ED[ic];
+    }
+}
+
+// turbulent viscosity (src/navier_scalar.cpp:155-161)
+__global__ void k_ke_visc(DevMesh m, KeArgs a) {
+    CELL_LOOP(i, m.ncel) {
+        const double ed = a.ED[i];
+        a.VisTur[i] = ed > 1.e-12 ? kCmu * a.R[i] * a.TE[i] * a.TE[i] / (ed + kSmall) : 0.;
+    }
+}
+
+// kcond (src/navier_scalar.cpp:269-274)
+__global__ void k_kcond(int C, const double *__restrict__ VisLam, const double *__restrict__ VisTur,
+                        double cp, double prl, double prte, double *__restrict__ kc) {
+    CELL_LOOP(i, C) {
+        double k = cp * (VisLam[i] / prl + VisTur[i] / prte) / cp;
+        kc[i] = k * 100;
+    }
+}
+
+// coupledBoundCommunicationSolid2Fluid (src/HeatConductionSolver.cpp:400-427)
+__global__ void k_solid2fluid(DevMesh sm, const double *__restrict__ T, const double *__restrict__ g,
+                              int fluid_ncb, double *__restrict__ BTf) {
+    const int C = sm.ncel, nc = sm.nbnd - sm.ncb;
+    CELL_LOOP(k, nc) {
+        const int b = sm.ncb + k, f = sm.b_face[b], ic = sm.b_cell[b];
+        const double d0 = sm.f_x[0][f] - sm.c_x[0][ic], d1 = sm.f_x[1][f] - sm.c_x[1][ic],
+                     d2 = sm.f_x[2][f] - sm.c_x[2][ic];
+        const double dtdr = g[ic] * d0 + g[C + ic] * d1 + g[2 * C + ic] * d2;
+        BTf[fluid_ncb + k] = T[ic] + dtdr;
+    }
+}
+
+// coupledBoundCommunicationFluid2Solid (src/HeatConductionSolver.cpp:373-397)
+__global__ void k_fluid2solid(DevMesh fm, DevMesh sm, const double *__restrict__ g,
+                              const double *__restrict__ kc, double *__restrict__ qf,
+                              double *__restrict__ qs) {
+    const int C = fm.ncel, nc = sm.nbnd - sm.ncb;
+    CELL_LOOP(k, nc) {
+        const int bf = fm.ncb + k, f = fm.b_face[bf], c = fm.b_cell[bf];
+        const double dtdn = (g[c] * fm.f_n[0][f] + g[C + c] * fm.f_n[1][f] +
+                             g[2 * C + c] * fm.f_n[2][f]) / fm.f_area[f];
+        const double q = 0. - dtdn * kc[c];
+        qf[bf] = q;
+        qs[sm.ncb + k] = 0. - q;
+    }
+}
+
+// HeatConductionSolver::SetBCTemperature (src/HeatConductionSolver.cpp:13-44)
+__global__ void k_solid_bc_temperature(DevMesh m, const double *__restrict__ T,
+                                       const double *__restrict__ D, const double *__restrict__ q,
+                                       double *__restrict__ BT) {
+    CELL_LOOP(b, m.nbnd) {
+        const int r = m.b_reg[b], t1 = m.reg->type1[r], t2 = m.reg->type2[r];
+        const int ic = m.b_cell[b], f = m.b_face[b];
+        if (t1 == 1 && t2 == 1) BT[b] = T[ic] - q[b] / (D[ic] * m.f_rlencos[f]);
+        else if ((t1 == 1 && t2 == 2) || t1 == 4) BT[b] = T[ic];
+    }
+}
+
+__global__ void k_scale_source(int C, const double *__restrict__ vol, double q,
+                               double *__restrict__ src) {
+    CELL_LOOP(i, C) src[i] = q * vol[i];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers --
+void launch_fill(double *p, size_t n, double v, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for((long long)std::min<size_t>(n, 1u << 24))),
+                       dim3(kBlock), 0, s, n, v, p);
+    CFD_LAUNCH_CHECK();
+}
+
+void launch_gradient(const DevMesh &m, int nf, const double *const *phi,
+                     const double *const *bphi, double *const *out, int limiter, double *tmp,
+                     hipStream_t s) {
+    // tmp holds >= 3*ncel doubles (WENO) and 2*nvrt (MLP vertex extrema)
+    GradTab h{};
+    for (int k = 0; k < nf; ++k) {
+        h.phi[k] = phi[k];
+        h.bphi[k] = bphi[k];
+        h.out[k] = out[k];
+    }
+    const int g = grid_for(m.ncel);
+    switch (nf) {
+    case 1: hipLaunchKernelGGL(k_gradient<1>, dim3(g), dim3(kBlock), 0, s, m, h); break;
+    case 2: hipLaunchKernelGGL(k_gradient<2>, dim3(g), dim3(kBlock), 0, s, m, h); break;
+    case 3: hipLaunchKernelGGL(k_gradient<3>, dim3(g), dim3(kBlock), 0, s, m, h); break;
+    default: hipLaunchKernelGGL(k_gradient<4>, dim3(g), dim3(kBlock), 0, s, m, h); break;
+    }
+    CFD_LAUNCH_CHECK();
+    if (limiter == 0) return;
+    for (int k = 0; k < nf; ++k) {
+        if (limiter == 1) {
+            hipLaunchKernelGGL(k_lim_barth, dim3(g), dim3(kBlock), 0, s, m, 0, phi[k],
+                               (const double *)nullptr, (const double *)nullptr, out[k]);
+        } else if (limiter == 2) {
+            double *vmin = tmp, *vmax = tmp + m.nvrt;
+            hipLaunchKernelGGL(k_vertex_minmax, dim3(grid_for(m.nvrt)), dim3(kBlock), 0, s,
+                               m.nvrt, m.v2c_ptr, m.v2c, phi[k], vmin, vmax);
+            hipLaunchKernelGGL(k_lim_barth, dim3(g), dim3(kBlock), 0, s, m, 1, phi[k],
+                               (const double *)vmin, (const double *)vmax, out[k]);
+        } else if (limiter == 3) {
+            hipLaunchKernelGGL(k_lim_weno, dim3(g), dim3(kBlock), 0, s, m, (const double *)out[k], tmp);
+            hipLaunchKernelGGL(k_copy_n, dim3(grid_for(3LL * m.ncel)), dim3(kBlock), 0, s,
+                               (size_t)3 * m.ncel, (const double *)tmp, out[k]);
+        }
+        CFD_LAUNCH_CHECK();
+    }
+}
+
+void launch_bc_velocity(const DevMesh &m, const double *R, const double *U, const double *V,
+                        const double *W, double *BR, double *BU, double *BV, double *BW,
+                        BcScal *sc, Red red, hipStream_t s) {
+    if (m.nbnd == 0) return;
+    hipLaunchKernelGGL(k_bc_velocity, dim3(red_grid(m.nbnd)), dim3(kBlock), 0, s, m, R, U, V, W,
+                       BR, BU, BV, BW, sc, red.part, red.tk);
+    hipLaunchKernelGGL(k_bc_outlet, dim3(grid_for(m.nbnd)), dim3(kBlock), 0, s, m,
+                       (const BcScal *)sc, BU, BV, BW);
+    CFD_LAUNCH_CHECK();
+}
+
+void launch_bc_pressure(const DevMesh &m, const double *P, double *BP, hipStream_t s) {
+    if (m.nbnd == 0) return;
+    hipLaunchKernelGGL(k_bc_copy, dim3(grid_for(m.nbnd)), dim3(kBlock), 0, s, m, P, BP);
+    CFD_LAUNCH_CHECK();
+}
+
+void launch_bc_temperature(const DevMesh &m, const double *T, double *BT, hipStream_t s) {
+    if (m.nbnd == 0) return;
+    hipLaunchKernelGGL(k_bc_temperature, dim3(grid_for(m.nbnd)), dim3(kBlock), 0, s, m, T, BT);
+    CFD_LAUNCH_CHECK();
+}
+
+void launch_build_velocity(const DevMesh &m, const VelArgs &a, int *errflag, hipStream_t s) {
+    hipLaunchKernelGGL(k_build_velocity, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m, a, errflag);
+    CFD_LAUNCH_CHECK();
+}
+
+void launch_rhie_chow(const DevMesh &m, const double *R, const double *U, const double *V,
+                      const double *W, const double *P, const double *gP, const double *Apr,
+                      const double *BR, const double *BU, const double *BV, const double *BW,
+                      double *RUFace, hipStream_t s) {
+    hipLaunchKernelGGL(k_rhie_chow, dim3(grid_for(m.nfac)), dim3(kBlock), 0, s, m, R, U, V, W, P,
+                       gP, Apr, BR, BU, BV, BW, RUFace);
+    CFD_LAUNCH_CHECK();
+}
+
+void launch_build_pressure(const DevMesh &m, const double *R, const double *Apr,
+                           const double *RUFace, double *diag, double *val, double *bp,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_build_pressure, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m, R, Apr,
+                       RUFace, diag, val, bp);
+    CFD_LAUNCH_CHECK();
+}
+
+void launch_correct_ruface(const DevMesh &m, const double *R, const double *Apr,
+                           const double *dp, const double *gdp, const double *BR,
+                           const double *BU, const double *BV, const double *BW, double urf_p,
+                           double *RUFace, hipStream_t s) {
+    hipLaunchKernelGGL(k_correct_ruface, dim3(grid_for(m.nfac)), dim3(kBlock), 0, s, m, R, Apr,
+                       dp, gdp, BR, BU, BV, BW, urf_p, RUFace);
+    CFD_LAUNCH_CHECK();
+}
+
+void launch_pressure_update(const DevMesh &m, const double *x, int ref, double urf_p,
+                            const double *Apr, const double *gx, double *P, double *U,
+                            double *V, double *W, double *res_slot, Red red, hipStream_t s) {
+    hipLaunchKernelGGL(k_pressure_update, dim3(red_grid(m.ncel)), dim3(kBlock), 0, s, m, x, ref,
+                       urf_p, Apr, gx, P, U, V, W, res_slot, red.part, red.tk);
+    CFD_LAUNCH_CHECK();
+}
+
+void launch_take_solution(const DevMesh &m, const double *x, double *fld, double *res_slot,
+                          double clip_lo, int clip, Red red, hipStream_t s) {
+    hipLaunchKernelGGL(k_take_solution, dim3(red_grid(m.ncel)), dim3(kBlock), 0, s, m.ncel,
+                       m.c_vol, x, fld, res_slot, clip_lo, clip, red.part, red.tk);
+    CFD_LAUNCH_CHECK();
+}
+
+void launch_build_scalar(const DevMesh &m, const ScaArgs &a, int *errflag, hipStream_t s) {
+    hipLaunchKernelGGL(k_build_scalar, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m, a, errflag);
+    CFD_LAUNCH_CHECK();
+}
+
+void launch_ke_sources(const DevMesh &m, const KeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_ke_sources, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m, a);
+    CFD_LAUNCH_CHECK();
+}
+void launch_ke_wall(const DevMesh &m, const KeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_ke_wall, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m, a);
+    if (m.nbnd) hipLaunchKernelGGL(k_ke_bnd, dim3(grid_for(m.nbnd)), dim3(kBlock), 0, s, m, a);
+    CFD_LAUNCH_CHECK();
+}
+void launch_ke_eps_sources(const DevMesh &m, const KeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_ke_eps_sources, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m, a);
+    CFD_LAUNCH_CHECK();
+}
+void launch_eps_wall_rows(const DevMesh &m, const double *ED, double *diag, double *val,
+                          double *bs, hipStream_t s) {
+    if (!m.nbnd) return;
+    hipLaunchKernelGGL(k_eps_wall_rows, dim3(grid_for(m.nbnd)), dim3(kBlock), 0, s, m, ED, diag,
+                       val, bs);
+    CFD_LAUNCH_CHECK();
+}
+void launch_ke_visc(const DevMesh &m, const KeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_ke_visc, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m, a);
+    CFD_LAUNCH_CHECK();
+}
+void launch_kcond(const DevMesh &m, const double *VisLam, const double *VisTur, double cp,
+                  double prl, double prte, double *kcond, hipStream_t s) {
+    hipLaunchKernelGGL(k_kcond, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m.ncel, VisLam,
+                       VisTur, cp, prl, prte, kcond);
+    CFD_LAUNCH_CHECK();
+}
+void launch_solid2fluid(const DevMesh &sm, const double *Ts, const double *gTs, int fluid_ncb,
+                        double *BTem_f, hipStream_t s) {
+    int nc = sm.nbnd - sm.ncb;
+    if (nc <= 0) return;
+    hipLaunchKernelGGL(k_solid2fluid, dim3(grid_for(nc)), dim3(kBlock), 0, s, sm, Ts, gTs,
+                       fluid_ncb, BTem_f);
+    CFD_LAUNCH_CHECK();
+}
+void launch_fluid2solid(const DevMesh &fm, const DevMesh &sm, const double *gTf,
+                        const double *kcond, double *q_f, double *q_s, hipStream_t s) {
+    int nc = sm.nbnd - sm.ncb;
+    if (nc <= 0) return;
+    hipLaunchKernelGGL(k_fluid2solid, dim3(grid_for(nc)), dim3(kBlock), 0, s, fm, sm, gTf, kcond,
+                       q_f, q_s);
+    CFD_LAUNCH_CHECK();
+}
+void launch_solid_bc_temperature(const DevMesh &m, const double *T, const double *D,
+                                 const double *q, double *BT, hipStream_t s) {
+    if (!m.nbnd) return;
+    hipLaunchKernelGGL(k_solid_bc_temperature, dim3(grid_for(m.nbnd)), dim3(kBlock), 0, s, m, T,
+                       D, q, BT);
+    CFD_LAUNCH_CHECK();
+}
+void launch_scale_source(const DevMesh &m, double q, double *src, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale_source, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m.ncel,
+                       m.c_vol, q, src);
+    CFD_LAUNCH_CHECK();
+}
+
+}  // namespace cfd

@@ -1,0 +1,100 @@
+// Device Krylov solvers over ELL matrices (MI355X, gfx950).
+#pragma once
+#include <vector>
+
+#include "cfd_solver.h"
+#include "common.h"
+
+namespace cfd {
+
+// Sparse matrix on the device: diagonal separate, K off-diagonal slots stored
+// slot-major (val[k*n + i], col[k*n + i]) so a wave64 reads 64 consecutive
+// rows of one slot in one coalesced access. Padding slots hold col = row,
+// val = 0.
+struct Ell {
+    int n = 0, K = 0;
+    double *diag = nullptr;
+    double *val = nullptr;
+    const int *col = nullptr;
+};
+
+// Multicolour ordering of one sparsity structure: rows permuted into colour
+// blocks (greedy first-fit over the symmetrised graph in natural order,
+// stable within a colour), so each triangular sweep of SSOR / ILU(0) is one
+// contiguous launch per colour.
+struct Colouring {
+    int n = 0, K = 0, ncolour = 0;
+    const int *src_col = nullptr;  // device structure this was built for
+    std::vector<int> off;          // colour block offsets, size ncolour+1
+    std::vector<int> colour_host;
+    DBuf<int> perm, iperm, pcol;
+    // host_col: the K x n slot-major column array; given_colour: optional
+    void build(const int *host_col, int n, int K, const int *given_colour, const int *dev_col,
+               hipStream_t s);
+};
+
+struct SolveStats {
+    int iters = 0;
+    double relres = 0.;  // true ||b - A x||_2 / ||b||_2, FP64
+    double rtcres = 0.;  // residual norm the stopping test saw
+    int breakdown = 0;
+    int refinements = 0; // mixed precision: FP64 outer corrections
+};
+
+struct KState {
+    double rho, rho_old, alpha, beta, omega, pq, rr, bb, rnorm, bnorm, eps, hscale;
+    double h[12], h2[12], H[11 * 10], cs[10], sn[10], g[11], y[10];
+    int iter, maxit, done, brk, half, j, m, cycle_done;
+};
+
+class Krylov {
+  public:
+    explicit Krylov(hipStream_t s = nullptr);
+    ~Krylov();
+    hipStream_t stream() const { return st_; }
+
+    // Solve A x = b (device vectors, natural row order). cfg.tol is the
+    // relative tolerance on ||r||_2/||b||_2, cfg.maxit the iteration cap.
+    // col must be given for SSOR / ILU.
+    SolveStats solve(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
+                     Colouring *col);
+    // y = M^{-1} c with the preconditioner the solver uses (natural order)
+    void precond(const Ell &A, int pc, double omega, const double *c, double *y, Colouring *col);
+    void spmv(const Ell &A, const double *x, double *y);
+    double relres(const Ell &A, const double *x, const double *b);
+    void sync();
+
+    int check_first = 4;  // iterations before the first host check
+    int check_max = 64;   // batch cap
+
+    // Live timing of the CG SpMV kernel (k_cg_pq): HIP events around each
+    // launch on this solver's stream; only launches that did work count.
+    bool prof = false;
+    double prof_ms = 0.;
+    long long prof_launches = 0;
+
+  private:
+    hipStream_t st_;
+    DBuf<double> r_, z_, p0_, p1_, q_, rt_, v_, s_, t_, ph_, sh_, w0_, w1_, V_, part_;
+    DBuf<double> pdiag_, pval_, pb_, px_, lu_, ud_, tmp_;
+    DBuf<unsigned> ticket_;
+    DBuf<KState> state_;
+    KState *hstate_ = nullptr;
+    int nalloc_ = 0, kalloc_ = 0;
+    std::vector<hipEvent_t> evs_;
+    hipEvent_t ev(size_t k);
+    int next_batch(int batch, int &prev_it, double &prev_log, int maxit);
+    void ensure(int n, int K);
+    void read_state();
+    void setup_pc(const Ell &A, int pc, Colouring *col);
+    void apply_pc(const Ell &A, int pc, double omega, const double *c, double *y,
+                  Colouring *col);
+    SolveStats cg(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
+                  Colouring *col, int pc);
+    SolveStats bicgstab(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
+                        Colouring *col, int pc);
+    SolveStats gmres(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
+                     Colouring *col, int pc);
+};
+
+}  // namespace cfd

@@ -1,0 +1,971 @@
+// Device Krylov solvers: CG, BiCGSTAB, restarted GMRES(10) with Jacobi,
+// multicolour SSOR and multicolour ILU(0) preconditioners, FP64.
+//
+// The algorithms are those the oracle restates from LASPack / Templates
+// (src/tools.cpp:351-359 is the call boundary): same recurrences, same
+// stopping test ||r||_2 < eps ||b||_2 on the recursively updated residual,
+// iteration counting as LASPack's while-loop. Every global dot product is a
+// deterministic two-level reduction (wave64 shuffles + LDS per workgroup,
+// then the last workgroup sums the per-workgroup partials in fixed order),
+// so results are run-to-run reproducible. Scalars (alpha, beta, ...) never
+// leave the device: the last workgroup of each reducing kernel updates a
+// KState record that the following kernels read. The host reads KState only
+// every few iterations to decide when to stop; kernels launched past
+// convergence return immediately.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "krylov.h"
+
+namespace cfd {
+
+namespace {
+
+__device__ __forceinline__ bool rtc(double rn, double bn, double eps) {
+    return bn > 0. ? rn < eps * bn : rn < eps;
+}
+
+__device__ __forceinline__ double row_dot(const Ell &A, int i, const double *__restrict__ x) {
+    const int n = A.n;
+    double s = A.diag[i] * x[i];
+    for (int k = 0; k < A.K; ++k) {
+        size_t o = (size_t)k * n + i;
+        s += A.val[o] * x[A.col[o]];
+    }
+    return s;
+}
+
+#define GRID_LOOP(i, n) \
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+
+__global__ void k_spmv(Ell A, const double *__restrict__ x, double *__restrict__ y) {
+    GRID_LOOP(i, A.n) y[i] = row_dot(A, i, x);
+}
+
+// r = b - A x ; partials ||r||^2, ||b||^2, r.(r/d) (Jacobi rho)
+__global__ void k_residual(Ell A, const double *__restrict__ b, const double *__restrict__ x,
+                           double *__restrict__ r, double *__restrict__ zero1,
+                           double *__restrict__ zero2, KState *st, double *part, unsigned *tk,
+                           int mode, double eps, int maxit) {
+    double acc[3] = {0., 0., 0.};
+    GRID_LOOP(i, A.n) {
+        double ri = b[i] - row_dot(A, i, x);
+        r[i] = ri;
+        if (zero1) zero1[i] = 0.;
+        if (zero2) zero2[i] = 0.;
+        acc[0] += ri * ri;
+        acc[1] += b[i] * b[i];
+        acc[2] += ri * (ri / A.diag[i]);
+    }
+    if (grid_sum<3>(acc, part, tk)) {
+        // mode 0: plain residual (no state change); 1: CG/BiCGSTAB start;
+        // 2: GMRES (re)start
+        if (mode == 0) {
+            st->rr = acc[0];
+            st->bb = acc[1];
+            return;
+        }
+        if (mode == 1 || mode == 3) {
+            st->eps = eps;
+            st->maxit = maxit;
+            st->iter = 0;
+            st->brk = 0;
+            st->half = 0;
+            st->rr = acc[0];
+            st->bb = acc[1];
+            st->bnorm = sqrt(acc[1]);
+            st->rnorm = sqrt(acc[0]);
+            st->rho = (mode == 1) ? acc[2] : acc[0];  // CG-Jacobi rho / BiCGSTAB rt.r
+            st->rho_old = 1.;
+            st->alpha = 1.;
+            st->omega = 1.;
+            st->beta = 0.;
+            st->done = rtc(st->rnorm, st->bnorm, eps) || maxit <= 0;
+            if (mode == 3 && !st->done && st->rho == 0.) { st->done = 1; st->brk = 1; }
+        } else {  // GMRES: r goes to W; beta = ||r||
+            if (maxit >= 0) {  // first start
+                st->eps = eps;
+                st->maxit = maxit;
+                st->iter = 0;
+                st->brk = 0;
+                st->bb = acc[1];
+                st->bnorm = sqrt(acc[1]);
+            }
+            double beta = sqrt(acc[0]);
+            st->rnorm = beta;
+            st->hscale = beta;
+            for (int t = 0; t < 11; ++t) st->g[t] = 0.;
+            st->g[0] = beta;
+            st->j = 0;
+            st->cycle_done = 0;
+            st->done = rtc(beta, st->bnorm, st->eps) || st->iter >= st->maxit;
+        }
+    }
+}
+
+__global__ void k_copy(int n, const double *__restrict__ a, double *__restrict__ b) {
+    GRID_LOOP(i, n) b[i] = a[i];
+}
+
+__global__ void k_scale(int n, double f, double *__restrict__ y) { GRID_LOOP(i, n) y[i] = f * y[i]; }
+
+__global__ void k_jacobi(int n, const double *__restrict__ d, const double *__restrict__ c,
+                         double *__restrict__ y) {
+    GRID_LOOP(i, n) y[i] = c[i] / d[i];
+}
+
+// ------------------------------------------------------------------- CG --
+// p_new = z + beta p_old ; q = A p_new ; partial p.q ; alpha = rho / p.q.
+// Jacobi: z = r / d is formed on the fly for the row and its neighbours.
+template <bool JAC>
+__global__ void k_cg_pq(Ell A, const double *__restrict__ r, const double *__restrict__ z,
+                        const double *__restrict__ pold, double *__restrict__ pnew,
+                        double *__restrict__ q, KState *st, double *part, unsigned *tk) {
+    if (st->done) return;
+    const double beta = st->beta;
+    const int n = A.n;
+    double acc[1] = {0.};
+    GRID_LOOP(i, n) {
+        double zi = JAC ? r[i] / A.diag[i] : z[i];
+        double pi = zi + beta * pold[i];
+        double s = A.diag[i] * pi;
+        for (int k = 0; k < A.K; ++k) {
+            size_t o = (size_t)k * n + i;
+            int c = A.col[o];
+            double zc = JAC ? r[c] / A.diag[c] : z[c];
+            s += A.val[o] * (zc + beta * pold[c]);
+        }
+        pnew[i] = pi;
+        q[i] = s;
+        acc[0] += pi * s;
+    }
+    if (grid_sum<1>(acc, part, tk)) {
+        st->pq = acc[0];
+        st->alpha = st->rho / acc[0];
+    }
+}
+
+// x += alpha p ; r -= alpha q ; partial ||r||^2 (and Jacobi rho = r.(r/d))
+template <bool JAC>
+__global__ void k_cg_update(int n, double *__restrict__ x, double *__restrict__ r,
+                            const double *__restrict__ p, const double *__restrict__ q,
+                            const double *__restrict__ d, KState *st, double *part,
+                            unsigned *tk) {
+    if (st->done) return;
+    const double alpha = st->alpha;
+    double acc[2] = {0., 0.};
+    GRID_LOOP(i, n) {
+        x[i] += alpha * p[i];
+        double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        acc[0] += ri * ri;
+        if (JAC) acc[1] += ri * (ri / d[i]);
+    }
+    if (grid_sum<2>(acc, part, tk)) {
+        st->iter += 1;
+        st->rho_old = st->rho;
+        st->rr = acc[0];
+        st->rnorm = sqrt(acc[0]);
+        if (JAC) {
+            st->rho = acc[1];
+            st->beta = st->rho / st->rho_old;
+        }
+        st->done = rtc(st->rnorm, st->bnorm, st->eps) || st->iter >= st->maxit;
+    }
+}
+
+// rho = r.z ; beta = rho / rho_old (0 on the first iteration)
+__global__ void k_cg_rho(int n, const double *__restrict__ r, const double *__restrict__ z,
+                         KState *st, double *part, unsigned *tk) {
+    if (st->done) return;
+    double acc[1] = {0.};
+    GRID_LOOP(i, n) acc[0] += r[i] * z[i];
+    if (grid_sum<1>(acc, part, tk)) {
+        st->rho = acc[0];
+        st->beta = st->iter == 0 ? 0. : acc[0] / st->rho_old;
+    }
+}
+
+// ------------------------------------------------------------- BiCGSTAB --
+// p = r (first) or r + beta (p - omega v); Jacobi: ph = p / d
+template <bool JAC>
+__global__ void k_bi_p(int n, const double *__restrict__ r, double *__restrict__ p,
+                       const double *__restrict__ v, double *__restrict__ ph,
+                       const double *__restrict__ d, const KState *st) {
+    if (st->done) return;
+    const bool first = st->iter == 0;
+    const double beta = (st->rho / st->rho_old) * (st->alpha / st->omega), om = st->omega;
+    GRID_LOOP(i, n) {
+        double pi = first ? r[i] : r[i] + beta * (p[i] - om * v[i]);
+        p[i] = pi;
+        if (JAC) ph[i] = pi / d[i];
+    }
+}
+
+// y = A x ; partial rt.y ; mode 0: alpha = rho / rt.v ; mode 1: (t.s, t.t) -> omega
+template <int MODE>
+__global__ void k_bi_spmv(Ell A, const double *__restrict__ x, double *__restrict__ y,
+                          const double *__restrict__ other, KState *st, double *part,
+                          unsigned *tk) {
+    if (st->done || (MODE == 1 && st->half)) return;
+    double acc[2] = {0., 0.};
+    GRID_LOOP(i, A.n) {
+        double s = row_dot(A, i, x);
+        y[i] = s;
+        acc[0] += MODE == 0 ? other[i] * s : s * other[i];
+        if (MODE == 1) acc[1] += s * s;
+    }
+    if (grid_sum<2>(acc, part, tk)) {
+        if (MODE == 0) st->alpha = st->rho / acc[0];
+        else st->omega = acc[0] / acc[1];
+    }
+}
+
+// s = r - alpha v ; Jacobi sh = s / d ; partial ||s||^2 -> half-step test
+template <bool JAC>
+__global__ void k_bi_s(int n, const double *__restrict__ r, const double *__restrict__ v,
+                       double *__restrict__ s, double *__restrict__ sh,
+                       const double *__restrict__ d, KState *st, double *part, unsigned *tk) {
+    if (st->done) return;
+    const double alpha = st->alpha;
+    double acc[1] = {0.};
+    GRID_LOOP(i, n) {
+        double si = r[i] - alpha * v[i];
+        s[i] = si;
+        if (JAC) sh[i] = si / d[i];
+        acc[0] += si * si;
+    }
+    if (grid_sum<1>(acc, part, tk)) st->half = rtc(sqrt(acc[0]), st->bnorm, st->eps);
+}
+
+__global__ void k_bi_x(int n, double *__restrict__ x, double *__restrict__ r,
+                       const double *__restrict__ ph, const double *__restrict__ sh,
+                       const double *__restrict__ s, const double *__restrict__ t,
+                       const double *__restrict__ rt, KState *st, double *part, unsigned *tk) {
+    if (st->done) return;
+    const double alpha = st->alpha, om = st->omega;
+    const bool half = st->half;
+    double acc[2] = {0., 0.};
+    GRID_LOOP(i, n) {
+        double ri;
+        if (half) {
+            x[i] += alpha * ph[i];
+            ri = s[i];
+        } else {
+            x[i] = x[i] + alpha * ph[i] + om * sh[i];
+            ri = s[i] - om * t[i];
+        }
+        r[i] = ri;
+        acc[0] += ri * ri;
+        acc[1] += rt[i] * ri;
+    }
+    if (grid_sum<2>(acc, part, tk)) {
+        st->iter += 1;
+        st->rr = acc[0];
+        st->rnorm = sqrt(acc[0]);
+        if (half) {
+            st->done = 1;
+            return;
+        }
+        st->rho_old = st->rho;
+        st->rho = acc[1];
+        st->done = rtc(st->rnorm, st->bnorm, st->eps) || st->iter >= st->maxit;
+        if (!st->done && om == 0.) { st->done = 1; st->brk = 1; }
+        if (!st->done && st->rho == 0.) { st->done = 1; st->brk = 1; st->iter += 1; }
+    }
+}
+
+// ---------------------------------------------------------------- GMRES --
+// v_j = W / h (stored) ; Jacobi: w = A (v_j / d) into Wn ; generic: only v_j
+template <bool JAC>
+__global__ void k_gm_v(Ell A, const double *__restrict__ W, double *__restrict__ Vj,
+                       double *__restrict__ Wn, const KState *st) {
+    if (st->done || st->cycle_done) return;
+    const double h = st->hscale;
+    const int n = A.n;
+    GRID_LOOP(i, n) {
+        double vi = W[i] / h;
+        Vj[i] = vi;
+        if (JAC) {
+            double s = A.diag[i] * (vi / A.diag[i]);
+            for (int k = 0; k < A.K; ++k) {
+                size_t o = (size_t)k * n + i;
+                int c = A.col[o];
+                s += A.val[o] * ((W[c] / h) / A.diag[c]);
+            }
+            Wn[i] = s;
+        }
+    }
+}
+
+__global__ void k_gm_spmv(Ell A, const double *__restrict__ z, double *__restrict__ Wn,
+                          const KState *st) {
+    if (st->done || st->cycle_done) return;
+    GRID_LOOP(i, A.n) Wn[i] = row_dot(A, i, z);
+}
+
+// h[t] = w . V_t, t = 0..j
+__global__ void k_gm_dot(int n, const double *__restrict__ w, const double *__restrict__ V,
+                         KState *st, double *part, unsigned *tk) {
+    if (st->done || st->cycle_done) return;
+    const int j = st->j;
+    double acc[12];
+#pragma unroll
+    for (int t = 0; t < 12; ++t) acc[t] = 0.;
+    GRID_LOOP(i, n) {
+        double wi = w[i];
+#pragma unroll
+        for (int t = 0; t < 12; ++t)
+            if (t <= j) acc[t] += wi * V[(size_t)t * n + i];
+    }
+    if (grid_sum<12>(acc, part, tk))
+        for (int t = 0; t < 12; ++t) st->h[t] = t <= j ? acc[t] : 0.;
+}
+
+// pass 1: w -= V h, then h2 = V^T w ; pass 2: w -= V h2, then ||w||^2 and the
+// Givens update of the Hessenberg column
+template <int PASS>
+__global__ void k_gm_sub(int n, double *__restrict__ w, const double *__restrict__ V,
+                         KState *st, double *part, unsigned *tk) {
+    if (st->done || st->cycle_done) return;
+    const int j = st->j;
+    double c[12];
+#pragma unroll
+    for (int t = 0; t < 12; ++t) c[t] = PASS == 1 ? st->h[t] : st->h2[t];
+    double acc[12];
+#pragma unroll
+    for (int t = 0; t < 12; ++t) acc[t] = 0.;
+    GRID_LOOP(i, n) {
+        double wi = w[i];
+        for (int t = 0; t <= j; ++t) wi -= c[t] * V[(size_t)t * n + i];
+        w[i] = wi;
+        if (PASS == 1) {
+#pragma unroll
+            for (int t = 0; t < 12; ++t)
+                if (t <= j) acc[t] += wi * V[(size_t)t * n + i];
+        } else {
+            acc[0] += wi * wi;
+        }
+    }
+    if (!grid_sum<12>(acc, part, tk)) return;
+    if (PASS == 1) {
+        for (int t = 0; t < 12; ++t) st->h2[t] = t <= j ? acc[t] : 0.;
+        return;
+    }
+    const int m = st->m;
+    double hn = sqrt(acc[0]);
+    for (int t = 0; t <= j; ++t) st->H[t * m + j] = st->h[t] + st->h2[t];
+    st->H[(j + 1) * m + j] = hn;
+    for (int t = 0; t < j; ++t) {
+        double a = st->H[t * m + j], b = st->H[(t + 1) * m + j];
+        st->H[t * m + j] = st->cs[t] * a + st->sn[t] * b;
+        st->H[(t + 1) * m + j] = -st->sn[t] * a + st->cs[t] * b;
+    }
+    double a = st->H[j * m + j], b = st->H[(j + 1) * m + j];
+    if (b == 0.) {
+        st->cs[j] = 1.;
+        st->sn[j] = 0.;
+    } else {
+        double tt = hypot(a, b);
+        st->cs[j] = a / tt;
+        st->sn[j] = b / tt;
+    }
+    st->H[j * m + j] = st->cs[j] * a + st->sn[j] * b;
+    st->H[(j + 1) * m + j] = 0.;
+    st->g[j + 1] = -st->sn[j] * st->g[j];
+    st->g[j] = st->cs[j] * st->g[j];
+    st->rnorm = fabs(st->g[j + 1]);
+    st->hscale = hn;
+    st->iter += 1;
+    st->j = j + 1;
+    if (rtc(st->rnorm, st->bnorm, st->eps) || st->iter >= st->maxit || hn == 0. ||
+        st->j >= m)
+        st->cycle_done = 1;
+}
+
+// y = H^{-1} g (upper triangular, k = st->j)
+__global__ void k_gm_y(KState *st) {
+    if (st->done || threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int k = st->j, m = st->m;
+    for (int i = k - 1; i >= 0; --i) {
+        double s = st->g[i];
+        for (int q = i + 1; q < k; ++q) s -= st->H[i * m + q] * st->y[q];
+        st->y[i] = s / st->H[i * m + i];
+    }
+}
+
+// u = sum_t y_t V_t ; Jacobi: x += u / d ; generic: u stored
+template <bool JAC>
+__global__ void k_gm_u(int n, const double *__restrict__ V, double *__restrict__ u,
+                       double *__restrict__ x, const double *__restrict__ d, const KState *st) {
+    if (st->done) return;
+    const int k = st->j;
+    GRID_LOOP(i, n) {
+        double s = 0.;
+        for (int t = 0; t < k; ++t) s += st->y[t] * V[(size_t)t * n + i];
+        if (JAC) x[i] += s / d[i];
+        else u[i] = s;
+    }
+}
+
+__global__ void k_add(int n, double *__restrict__ x, const double *__restrict__ z, const KState *st) {
+    if (st && st->done) return;
+    GRID_LOOP(i, n) x[i] += z[i];
+}
+
+// ----------------------------------------------- multicolour sweeps --
+// Rows [r0, r1) form one colour block of the permuted matrix; a neighbour
+// is "lower" iff its permuted index is below the row's.
+__global__ void k_ssor_fwd(Ell A, double omega, int r0, int r1, const double *__restrict__ c,
+                           double *__restrict__ t) {
+    const int n = A.n;
+    int p = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= r1) return;
+    double acc = c[p];
+    for (int k = 0; k < A.K; ++k) {
+        size_t o = (size_t)k * n + p;
+        int cl = A.col[o];
+        if (cl < p) acc -= omega * A.val[o] * t[cl];
+    }
+    t[p] = acc / A.diag[p];
+}
+__global__ void k_ssor_bwd(Ell A, double omega, int r0, int r1, const double *__restrict__ t,
+                           double *__restrict__ y) {
+    const int n = A.n;
+    int p = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= r1) return;
+    double acc = A.diag[p] * t[p];
+    for (int k = 0; k < A.K; ++k) {
+        size_t o = (size_t)k * n + p;
+        int cl = A.col[o];
+        if (cl > p) acc -= omega * A.val[o] * y[cl];
+    }
+    y[p] = acc / A.diag[p];
+}
+
+constexpr int kIluMaxK = 16;
+
+// ILU(0) of one colour block, IKJ order: earlier neighbours in ascending
+// permuted index; lu/ud start as copies of the off-diagonals / diagonal.
+__global__ void k_ilu_factor(int n, int K, const int *__restrict__ col, int r0, int r1,
+                             double *__restrict__ lu, double *__restrict__ ud) {
+    int p = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= r1) return;
+    int lc[kIluMaxK], ls[kIluMaxK], nl = 0;
+    for (int k = 0; k < K; ++k) {
+        int cl = col[(size_t)k * n + p];
+        if (cl < p) {
+            int q = nl++;
+            while (q > 0 && lc[q - 1] > cl) {
+                lc[q] = lc[q - 1];
+                ls[q] = ls[q - 1];
+                --q;
+            }
+            lc[q] = cl;
+            ls[q] = k;
+        }
+    }
+    for (int a = 0; a < nl; ++a) {
+        int c = lc[a];
+        size_t o = (size_t)ls[a] * n + p;
+        double l = lu[o] / ud[c];
+        lu[o] = l;
+        for (int q = 0; q < K; ++q) {
+            size_t oc = (size_t)q * n + c;
+            int j = col[oc];
+            if (j <= c) continue;
+            double lv = lu[oc];
+            if (j == p) {
+                ud[p] -= l * lv;
+                continue;
+            }
+            for (int t = 0; t < K; ++t) {
+                size_t op = (size_t)t * n + p;
+                if (col[op] == j) {
+                    lu[op] -= l * lv;
+                    break;
+                }
+            }
+        }
+    }
+}
+__global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col, const double *__restrict__ lu,
+                          int r0, int r1, const double *__restrict__ c, double *__restrict__ t) {
+    int p = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= r1) return;
+    double acc = c[p];
+    for (int k = 0; k < K; ++k) {
+        size_t o = (size_t)k * n + p;
+        int cl = col[o];
+        if (cl < p) acc -= lu[o] * t[cl];
+    }
+    t[p] = acc;
+}
+__global__ void k_ilu_bwd(int n, int K, const int *__restrict__ col, const double *__restrict__ lu,
+                          const double *__restrict__ ud, int r0, int r1,
+                          const double *__restrict__ t, double *__restrict__ y) {
+    int p = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= r1) return;
+    double acc = t[p];
+    for (int k = 0; k < K; ++k) {
+        size_t o = (size_t)k * n + p;
+        int cl = col[o];
+        if (cl > p) acc -= lu[o] * y[cl];
+    }
+    y[p] = acc / ud[p];
+}
+
+// permuted copies: A'[p] = A[perm[p]] (columns via the cached pcol)
+__global__ void k_perm_matrix(int n, int K, const int *__restrict__ perm,
+                              const double *__restrict__ diag, const double *__restrict__ val,
+                              double *__restrict__ pdiag, double *__restrict__ pval) {
+    GRID_LOOP(p, n) {
+        int o = perm[p];
+        pdiag[p] = diag[o];
+        for (int k = 0; k < K; ++k) pval[(size_t)k * n + p] = val[(size_t)k * n + o];
+    }
+}
+__global__ void k_perm_gather(int n, const int *__restrict__ perm, const double *__restrict__ a,
+                              double *__restrict__ b) {
+    GRID_LOOP(p, n) b[p] = a[perm[p]];
+}
+__global__ void k_perm_scatter(int n, const int *__restrict__ perm, const double *__restrict__ a,
+                               double *__restrict__ b) {
+    GRID_LOOP(p, n) b[perm[p]] = a[p];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ Colouring --
+void Colouring::build(const int *hc, int n_, int K_, const int *given, const int *dev_col,
+                      hipStream_t s) {
+    n = n_;
+    K = K_;
+    src_col = dev_col;
+    colour_host.assign(n, 0);
+    if (given) {
+        std::copy(given, given + n, colour_host.begin());
+    } else {
+        // symmetrised adjacency, greedy first-fit in natural order
+        std::vector<int> deg(n + 1, 0);
+        for (int k = 0; k < K; ++k)
+            for (int i = 0; i < n; ++i) {
+                int c = hc[(size_t)k * n + i];
+                if (c != i) { deg[i + 1]++; deg[c + 1]++; }
+            }
+        for (int i = 0; i < n; ++i) deg[i + 1] += deg[i];
+        std::vector<int> adj(deg[n]), fill(deg.begin(), deg.end() - 1);
+        for (int k = 0; k < K; ++k)
+            for (int i = 0; i < n; ++i) {
+                int c = hc[(size_t)k * n + i];
+                if (c != i) { adj[fill[i]++] = c; adj[fill[c]++] = i; }
+            }
+        std::vector<char> used;
+        for (int i = 0; i < n; ++i) {
+            used.assign(deg[i + 1] - deg[i] + 2, 0);
+            for (int e = deg[i]; e < deg[i + 1]; ++e) {
+                int j = adj[e];
+                if (j < i && colour_host[j] < (int)used.size()) used[colour_host[j]] = 1;
+            }
+            int c = 0;
+            while (used[c]) ++c;
+            colour_host[i] = c;
+        }
+    }
+    ncolour = 0;
+    for (int c : colour_host) ncolour = std::max(ncolour, c + 1);
+    off.assign(ncolour + 1, 0);
+    for (int c : colour_host) off[c + 1]++;
+    for (int c = 0; c < ncolour; ++c) off[c + 1] += off[c];
+    std::vector<int> hp(n), hip_(n), fillc(off.begin(), off.end() - 1);
+    for (int i = 0; i < n; ++i) {
+        int p = fillc[colour_host[i]]++;
+        hp[p] = i;
+        hip_[i] = p;
+    }
+    std::vector<int> hpc((size_t)K * n);
+    for (int k = 0; k < K; ++k)
+        for (int p = 0; p < n; ++p) hpc[(size_t)k * n + p] = hip_[hc[(size_t)k * n + hp[p]]];
+    perm.alloc(n);
+    iperm.alloc(n);
+    pcol.alloc((size_t)K * n);
+    perm.upload(hp.data(), n, s);
+    iperm.upload(hip_.data(), n, s);
+    pcol.upload(hpc.data(), (size_t)K * n, s);
+    CFD_HIP(hipStreamSynchronize(s));
+}
+
+// --------------------------------------------------------------- Krylov --
+Krylov::Krylov(hipStream_t s) : st_(s) {
+    ticket_.alloc(1);
+    ticket_.zero(st_);
+    state_.alloc(1);
+    part_.alloc(12 * kRedGrid);
+    CFD_HIP(hipHostMalloc((void **)&hstate_, sizeof(KState), hipHostMallocDefault));
+    std::memset(hstate_, 0, sizeof(KState));
+}
+Krylov::~Krylov() {
+    if (hstate_) (void)hipHostFree(hstate_);
+    for (auto e : evs_) (void)hipEventDestroy(e);
+}
+
+hipEvent_t Krylov::ev(size_t k) {
+    while (evs_.size() <= k) {
+        hipEvent_t e;
+        CFD_HIP(hipEventCreate(&e));
+        evs_.push_back(e);
+    }
+    return evs_[k];
+}
+
+// Iterations to launch before the next host check: the observed log-residual
+// decrease per iteration extrapolated to the target (few idle launches past
+// convergence, few host synchronisations).
+int Krylov::next_batch(int batch, int &prev_it, double &prev_log, int maxit) {
+    const KState &h = *hstate_;
+    double target = h.bnorm > 0 ? h.eps * h.bnorm : h.eps;
+    double lr = std::log(std::max(h.rnorm, 1e-300)), lt = std::log(std::max(target, 1e-300));
+    int nb = batch * 2;
+    if (h.iter > prev_it && prev_log > lr) {
+        double rate = (prev_log - lr) / (h.iter - prev_it);
+        double need = (lr - lt) / rate;
+        nb = need < 1. ? 1 : (int)std::ceil(need);
+    }
+    prev_it = h.iter;
+    prev_log = lr;
+    nb = std::min(std::max(nb, 1), check_max);
+    nb = std::min(nb, std::max(1, maxit - h.iter));
+    return nb;
+}
+
+void Krylov::ensure(int n, int K) {
+    if (n <= nalloc_ && K <= kalloc_) return;
+    nalloc_ = std::max(n, nalloc_);
+    kalloc_ = std::max(K, kalloc_);
+    size_t N = nalloc_;
+    for (auto *b : {&r_, &z_, &p0_, &p1_, &q_, &rt_, &v_, &s_, &t_, &ph_, &sh_, &w0_, &w1_,
+                    &pdiag_, &pb_, &px_, &ud_, &tmp_})
+        b->alloc(N);
+    V_.alloc(11 * N);
+    pval_.alloc((size_t)kalloc_ * N);
+    lu_.alloc((size_t)kalloc_ * N);
+}
+
+void Krylov::sync() { CFD_HIP(hipStreamSynchronize(st_)); }
+
+void Krylov::read_state() {
+    CFD_HIP(hipMemcpyAsync(hstate_, state_.p, sizeof(KState), hipMemcpyDeviceToHost, st_));
+    CFD_HIP(hipStreamSynchronize(st_));
+}
+
+void Krylov::spmv(const Ell &A, const double *x, double *y) {
+    hipLaunchKernelGGL(k_spmv, dim3(grid_for(A.n)), dim3(kBlock), 0, st_, A, x, y);
+    CFD_LAUNCH_CHECK();
+}
+
+double Krylov::relres(const Ell &A, const double *x, const double *b) {
+    ensure(A.n, A.K);
+    hipLaunchKernelGGL(k_residual, dim3(red_grid(A.n)), dim3(kBlock), 0, st_, A, b, x, tmp_.p,
+                       (double *)nullptr, (double *)nullptr, state_.p, part_.p, ticket_.p, 0, 0.,
+                       0);
+    CFD_LAUNCH_CHECK();
+    read_state();
+    double bn = std::sqrt(hstate_->bb), rn = std::sqrt(hstate_->rr);
+    return bn > 0 ? rn / bn : rn;
+}
+
+void Krylov::setup_pc(const Ell &A, int pc, Colouring *col) {
+    if (pc != CFD_PC_ILU) return;
+    const int n = A.n;
+    if (A.K > kIluMaxK) throw HipError("ILU(0): more than 16 off-diagonals per row");
+    CFD_HIP(hipMemcpyAsync(lu_.p, A.val, sizeof(double) * (size_t)A.K * n,
+                           hipMemcpyDeviceToDevice, st_));
+    CFD_HIP(hipMemcpyAsync(ud_.p, A.diag, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
+    for (int c = 0; c < col->ncolour; ++c) {
+        int r0 = col->off[c], r1 = col->off[c + 1];
+        if (r1 > r0)
+            hipLaunchKernelGGL(k_ilu_factor, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_, n,
+                               A.K, A.col, r0, r1, lu_.p, ud_.p);
+    }
+    CFD_LAUNCH_CHECK();
+}
+
+// A is the permuted matrix for SSOR / ILU
+void Krylov::apply_pc(const Ell &A, int pc, double omega, const double *c, double *y,
+                      Colouring *col) {
+    const int n = A.n;
+    if (pc == CFD_PC_NONE) {
+        CFD_HIP(hipMemcpyAsync(y, c, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
+    } else if (pc == CFD_PC_JACOBI) {
+        hipLaunchKernelGGL(k_jacobi, dim3(grid_for(n)), dim3(kBlock), 0, st_, n, A.diag, c, y);
+    } else if (pc == CFD_PC_SSOR) {
+        for (int k = 0; k < col->ncolour; ++k) {
+            int r0 = col->off[k], r1 = col->off[k + 1];
+            if (r1 > r0)
+                hipLaunchKernelGGL(k_ssor_fwd, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_, A,
+                                   omega, r0, r1, c, tmp_.p);
+        }
+        for (int k = col->ncolour - 1; k >= 0; --k) {
+            int r0 = col->off[k], r1 = col->off[k + 1];
+            if (r1 > r0)
+                hipLaunchKernelGGL(k_ssor_bwd, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_, A,
+                                   omega, r0, r1, tmp_.p, y);
+        }
+        hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
+                           omega * (2. - omega), y);
+    } else {
+        for (int k = 0; k < col->ncolour; ++k) {
+            int r0 = col->off[k], r1 = col->off[k + 1];
+            if (r1 > r0)
+                hipLaunchKernelGGL(k_ilu_fwd, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_, n,
+                                   A.K, A.col, lu_.p, r0, r1, c, tmp_.p);
+        }
+        for (int k = col->ncolour - 1; k >= 0; --k) {
+            int r0 = col->off[k], r1 = col->off[k + 1];
+            if (r1 > r0)
+                hipLaunchKernelGGL(k_ilu_bwd, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_, n,
+                                   A.K, A.col, lu_.p, ud_.p, r0, r1, tmp_.p, y);
+        }
+    }
+    CFD_LAUNCH_CHECK();
+}
+
+void Krylov::precond(const Ell &A, int pc, double omega, const double *c, double *y,
+                     Colouring *col) {
+    ensure(A.n, A.K);
+    const int n = A.n;
+    if (pc == CFD_PC_SSOR || pc == CFD_PC_ILU) {
+        if (!col) throw HipError("multicolour preconditioner needs a colouring");
+        hipLaunchKernelGGL(k_perm_matrix, dim3(grid_for(n)), dim3(kBlock), 0, st_, n, A.K,
+                           col->perm.p, A.diag, A.val, pdiag_.p, pval_.p);
+        Ell P{n, A.K, pdiag_.p, pval_.p, col->pcol.p};
+        setup_pc(P, pc, col);
+        hipLaunchKernelGGL(k_perm_gather, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
+                           col->perm.p, c, pb_.p);
+        apply_pc(P, pc, omega, pb_.p, px_.p, col);
+        hipLaunchKernelGGL(k_perm_scatter, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
+                           col->perm.p, px_.p, y);
+        CFD_LAUNCH_CHECK();
+    } else {
+        apply_pc(A, pc, omega, c, y, col);
+    }
+    sync();
+}
+
+SolveStats Krylov::solve(const Ell &A0, const double *b0, double *x0, const cfd_solve_cfg &cfg,
+                         Colouring *col) {
+    const int n = A0.n;
+    ensure(n, A0.K);
+    int pc = cfg.precond;
+    Ell A = A0;
+    const double *b = b0;
+    double *x = x0;
+    bool perm = (pc == CFD_PC_SSOR || pc == CFD_PC_ILU);
+    if (perm) {
+        if (!col || col->n != n) throw HipError("multicolour preconditioner needs a colouring");
+        hipLaunchKernelGGL(k_perm_matrix, dim3(grid_for(n)), dim3(kBlock), 0, st_, n, A0.K,
+                           col->perm.p, A0.diag, A0.val, pdiag_.p, pval_.p);
+        hipLaunchKernelGGL(k_perm_gather, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
+                           col->perm.p, b0, pb_.p);
+        hipLaunchKernelGGL(k_perm_gather, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
+                           col->perm.p, x0, px_.p);
+        CFD_LAUNCH_CHECK();
+        A = Ell{n, A0.K, pdiag_.p, pval_.p, col->pcol.p};
+        b = pb_.p;
+        x = px_.p;
+        setup_pc(A, pc, col);
+    }
+    SolveStats S;
+    if (cfg.method == CFD_CG) S = cg(A, b, x, cfg, col, pc);
+    else if (cfg.method == CFD_BICGSTAB) S = bicgstab(A, b, x, cfg, col, pc);
+    else S = gmres(A, b, x, cfg, col, pc);
+    S.relres = relres(A, x, b);
+    if (perm) {
+        hipLaunchKernelGGL(k_perm_scatter, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
+                           col->perm.p, px_.p, x0);
+        CFD_LAUNCH_CHECK();
+        sync();
+    }
+    return S;
+}
+
+SolveStats Krylov::cg(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
+                      Colouring *col, int pc) {
+    const int n = A.n, G = red_grid(n), g = grid_for(n);
+    const bool jac = pc == CFD_PC_JACOBI;
+    hipLaunchKernelGGL(k_residual, dim3(G), dim3(kBlock), 0, st_, A, b, (const double *)x, r_.p,
+                       p0_.p, p1_.p, state_.p, part_.p, ticket_.p, 1, cfg.tol, cfg.maxit);
+    if (!jac) {
+        apply_pc(A, pc, cfg.omega, r_.p, z_.p, col);
+        hipLaunchKernelGGL(k_cg_rho, dim3(G), dim3(kBlock), 0, st_, n, r_.p, z_.p, state_.p,
+                           part_.p, ticket_.p);
+    }
+    CFD_LAUNCH_CHECK();
+    int batch = check_first, issued = 0, prev_it = 0;
+    double prev_log = 1e300;
+    double *pa = p0_.p, *pb = p1_.p;
+    (void)g;
+    size_t nev = 0;
+    for (;;) {
+        for (int k = 0; k < batch; ++k) {
+            if (prof) CFD_HIP(hipEventRecord(ev(2 * nev), st_));
+            if (jac)
+                hipLaunchKernelGGL(k_cg_pq<true>, dim3(G), dim3(kBlock), 0, st_, A, r_.p,
+                                   (const double *)nullptr, pa, pb, q_.p, state_.p, part_.p,
+                                   ticket_.p);
+            else
+                hipLaunchKernelGGL(k_cg_pq<false>, dim3(G), dim3(kBlock), 0, st_, A, r_.p, z_.p,
+                                   pa, pb, q_.p, state_.p, part_.p, ticket_.p);
+            if (prof) {
+                CFD_HIP(hipEventRecord(ev(2 * nev + 1), st_));
+                ++nev;
+            }
+            if (jac) {
+                hipLaunchKernelGGL(k_cg_update<true>, dim3(G), dim3(kBlock), 0, st_, n, x, r_.p,
+                                   pb, q_.p, A.diag, state_.p, part_.p, ticket_.p);
+            } else {
+                hipLaunchKernelGGL(k_cg_update<false>, dim3(G), dim3(kBlock), 0, st_, n, x,
+                                   r_.p, pb, q_.p, A.diag, state_.p, part_.p, ticket_.p);
+                apply_pc(A, pc, cfg.omega, r_.p, z_.p, col);
+                hipLaunchKernelGGL(k_cg_rho, dim3(G), dim3(kBlock), 0, st_, n, r_.p, z_.p,
+                                   state_.p, part_.p, ticket_.p);
+            }
+            std::swap(pa, pb);
+        }
+        CFD_LAUNCH_CHECK();
+        issued += batch;
+        read_state();
+        if (hstate_->done || issued > cfg.maxit + 2) break;
+        batch = next_batch(batch, prev_it, prev_log, cfg.maxit);
+    }
+    if (prof) {
+        size_t worked = std::min<size_t>(nev, (size_t)hstate_->iter);
+        for (size_t k = 0; k < worked; ++k) {
+            float ms = 0.f;
+            CFD_HIP(hipEventElapsedTime(&ms, ev(2 * k), ev(2 * k + 1)));
+            prof_ms += ms;
+        }
+        prof_launches += (long long)worked;
+    }
+    SolveStats S;
+    S.iters = hstate_->iter;
+    S.rtcres = hstate_->bnorm > 0 ? hstate_->rnorm / hstate_->bnorm : hstate_->rnorm;
+    S.breakdown = hstate_->brk;
+    return S;
+}
+
+SolveStats Krylov::bicgstab(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
+                            Colouring *col, int pc) {
+    const int n = A.n, G = red_grid(n), g = grid_for(n);
+    const bool jac = pc == CFD_PC_JACOBI;
+    hipLaunchKernelGGL(k_residual, dim3(G), dim3(kBlock), 0, st_, A, b, (const double *)x, r_.p,
+                       p0_.p, v_.p, state_.p, part_.p, ticket_.p, 3, cfg.tol, cfg.maxit);
+    hipLaunchKernelGGL(k_copy, dim3(g), dim3(kBlock), 0, st_, n, (const double *)r_.p, rt_.p);
+    CFD_LAUNCH_CHECK();
+    int batch = check_first, issued = 0, prev_it = 0;
+    double prev_log = 1e300;
+    for (;;) {
+        for (int k = 0; k < batch; ++k) {
+            if (jac) {
+                hipLaunchKernelGGL(k_bi_p<true>, dim3(g), dim3(kBlock), 0, st_, n, r_.p, p0_.p,
+                                   v_.p, ph_.p, A.diag, state_.p);
+            } else {
+                hipLaunchKernelGGL(k_bi_p<false>, dim3(g), dim3(kBlock), 0, st_, n, r_.p, p0_.p,
+                                   v_.p, ph_.p, A.diag, state_.p);
+                apply_pc(A, pc, cfg.omega, p0_.p, ph_.p, col);
+            }
+            hipLaunchKernelGGL(k_bi_spmv<0>, dim3(G), dim3(kBlock), 0, st_, A, ph_.p, v_.p, rt_.p,
+                               state_.p, part_.p, ticket_.p);
+            if (jac) {
+                hipLaunchKernelGGL(k_bi_s<true>, dim3(G), dim3(kBlock), 0, st_, n, r_.p, v_.p,
+                                   s_.p, sh_.p, A.diag, state_.p, part_.p, ticket_.p);
+            } else {
+                hipLaunchKernelGGL(k_bi_s<false>, dim3(G), dim3(kBlock), 0, st_, n, r_.p, v_.p,
+                                   s_.p, sh_.p, A.diag, state_.p, part_.p, ticket_.p);
+                apply_pc(A, pc, cfg.omega, s_.p, sh_.p, col);
+            }
+            hipLaunchKernelGGL(k_bi_spmv<1>, dim3(G), dim3(kBlock), 0, st_, A, sh_.p, t_.p, s_.p,
+                               state_.p, part_.p, ticket_.p);
+            hipLaunchKernelGGL(k_bi_x, dim3(G), dim3(kBlock), 0, st_, n, x, r_.p, ph_.p, sh_.p,
+                               s_.p, t_.p, rt_.p, state_.p, part_.p, ticket_.p);
+        }
+        CFD_LAUNCH_CHECK();
+        issued += batch;
+        read_state();
+        if (hstate_->done || issued > cfg.maxit + 2) break;
+        batch = next_batch(batch, prev_it, prev_log, cfg.maxit);
+    }
+    SolveStats S;
+    S.iters = hstate_->iter;
+    S.rtcres = hstate_->bnorm > 0 ? hstate_->rnorm / hstate_->bnorm : hstate_->rnorm;
+    S.breakdown = hstate_->brk;
+    return S;
+}
+
+SolveStats Krylov::gmres(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
+                         Colouring *col, int pc) {
+    const int n = A.n, G = red_grid(n), g = grid_for(n), m = 10;
+    const bool jac = pc == CFD_PC_JACOBI;
+    // set m before the first restart kernel reads it
+    {
+        KState init;
+        std::memset(&init, 0, sizeof(init));
+        init.m = m;
+        CFD_HIP(hipMemcpyAsync(state_.p, &init, sizeof(KState), hipMemcpyHostToDevice, st_));
+        CFD_HIP(hipStreamSynchronize(st_));
+    }
+    double *W = w0_.p, *Wn = w1_.p;
+    hipLaunchKernelGGL(k_residual, dim3(G), dim3(kBlock), 0, st_, A, b, (const double *)x, W,
+                       (double *)nullptr, (double *)nullptr, state_.p, part_.p, ticket_.p, 2,
+                       cfg.tol, cfg.maxit);
+    CFD_LAUNCH_CHECK();
+    read_state();
+    while (!hstate_->done) {
+        for (int j = 0; j < m; ++j) {
+            double *Vj = V_.p + (size_t)j * n;
+            if (jac) {
+                hipLaunchKernelGGL(k_gm_v<true>, dim3(g), dim3(kBlock), 0, st_, A, W, Vj, Wn,
+                                   state_.p);
+            } else {
+                hipLaunchKernelGGL(k_gm_v<false>, dim3(g), dim3(kBlock), 0, st_, A, W, Vj, Wn,
+                                   state_.p);
+                apply_pc(A, pc, cfg.omega, Vj, z_.p, col);
+                hipLaunchKernelGGL(k_gm_spmv, dim3(g), dim3(kBlock), 0, st_, A, z_.p, Wn,
+                                   state_.p);
+            }
+            hipLaunchKernelGGL(k_gm_dot, dim3(G), dim3(kBlock), 0, st_, n, Wn, V_.p, state_.p,
+                               part_.p, ticket_.p);
+            hipLaunchKernelGGL(k_gm_sub<1>, dim3(G), dim3(kBlock), 0, st_, n, Wn, V_.p,
+                               state_.p, part_.p, ticket_.p);
+            hipLaunchKernelGGL(k_gm_sub<2>, dim3(G), dim3(kBlock), 0, st_, n, Wn, V_.p,
+                               state_.p, part_.p, ticket_.p);
+            std::swap(W, Wn);
+        }
+        hipLaunchKernelGGL(k_gm_y, dim3(1), dim3(64), 0, st_, state_.p);
+        if (jac) {
+            hipLaunchKernelGGL(k_gm_u<true>, dim3(g), dim3(kBlock), 0, st_, n, V_.p, z_.p, x,
+                               A.diag, state_.p);
+        } else {
+            hipLaunchKernelGGL(k_gm_u<false>, dim3(g), dim3(kBlock), 0, st_, n, V_.p, q_.p, x,
+                               A.diag, state_.p);
+            apply_pc(A, pc, cfg.omega, q_.p, z_.p, col);
+            hipLaunchKernelGGL(k_add, dim3(g), dim3(kBlock), 0, st_, n, x, z_.p, state_.p);
+        }
+        W = w0_.p;
+        Wn = w1_.p;
+        hipLaunchKernelGGL(k_residual, dim3(G), dim3(kBlock), 0, st_, A, b, (const double *)x, W,
+                           (double *)nullptr, (double *)nullptr, state_.p, part_.p, ticket_.p, 2,
+                           cfg.tol, -1);
+        CFD_LAUNCH_CHECK();
+        read_state();
+    }
+    SolveStats S;
+    S.iters = hstate_->iter;
+    S.rtcres = hstate_->bnorm > 0 ? hstate_->rnorm / hstate_->bnorm : hstate_->rnorm;
+    S.breakdown = hstate_->brk;
+    return S;
+}
+
+}  // namespace cfd

@@ -1,0 +1,111 @@
+// Device-resident SIMPLE engine: host classes that keep the reference's
+// NavierStokesSolver / HeatConductionSolver method names and launch the HIP
+// kernels of assembly.hip / krylov.hip.
+#pragma once
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "assembly.h"
+#include "krylov.h"
+#include "mesh.h"
+
+namespace cfd {
+
+// Device copy of one HostMesh (owns the buffers behind DevMesh's pointers).
+struct DevMeshOwner {
+    DevMesh dm;
+    std::vector<int> ell_col_host;  // [K][C]
+    std::vector<DBuf<int>> ib;
+    std::vector<DBuf<double>> db;
+    DBuf<RegTab> reg;
+    void build(const HostMesh &h, const RegTab &rt, const std::vector<int> &breg, hipStream_t s);
+};
+
+class NSSolver;
+
+// HeatConductionSolver (src/HeatConductionSolver.h:5-64), steady
+class HTSolver {
+  public:
+    NSSolver *ns = nullptr;
+    HostMesh hm;
+    DevMeshOwner dmo;
+    std::vector<int> breg;
+    Colouring colour;
+    bool colour_ready = false;
+    double URF = 1.0;  // src/Preprocessor.cpp:506
+    DBuf<double> T, R, D, g, BT, BndQ, xsol, bs, src, diag, val, gtmp;
+    SolveStats last;
+    long long iters_total = 0;
+    void allocate(hipStream_t s);
+    void InitFlowField();
+    void SetBCTemperature();
+    void BuildScalarMatrix();
+    int UpdateEnergy();
+};
+
+// NavierStokesSolver (src/navier.h:13-127), steady, DensityModel 0
+class NSSolver {
+  public:
+    cfd_params P{};
+    RegTab reg{};
+    std::map<int, int> slot_of;  // region id -> slot
+    HostMesh hm;
+    DevMeshOwner dmo;
+    std::vector<int> breg;
+    std::unique_ptr<HTSolver> ht;
+    hipStream_t s = nullptr;
+    std::unique_ptr<Krylov> kry;
+    Colouring colour;
+    bool colour_ready = false;
+
+    DBuf<double> R, U, V, W, P_, T, TE, ED, VisLam, VisTur, Apr, RUFace;
+    DBuf<double> gU, gV, gW, gP, gPhi;  // [3][C]
+    DBuf<double> BR, BU, BV, BW, BP, BT, BTE, BED, BndQ;
+    DBuf<double> xsol, bu, bv, bw, bp, bs, kcond;
+    DBuf<double> As_diag, As_val, Ap_diag, Ap_val;
+    DBuf<double> Prod, TES, EDS, ApTE, ApED, VisTE, VisED, zeros;
+    DBuf<double> res_dev, gtmp, red_part;
+    DBuf<unsigned> red_tk;
+    DBuf<BcScal> bcs;
+    DBuf<int> errflag;
+    double Residual[10] = {0};
+    SolveStats last_u, last_v, last_w, last_p, last_k, last_e, last_t;
+    std::string last_as;  // which system As holds ("velocity", "k", "eps", "T")
+    double stats[8] = {0};
+    bool prof_pressure = false;  // live timing of the pressure CG SpMV kernel
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    ~NSSolver();
+    int setup(const cfd_raw_mesh &raw, const cfd_params &p, std::string &err);
+    Red red() { return Red{red_part.p, red_tk.p}; }
+    const DevMesh &dm() const { return dmo.dm; }
+    Ell As() { return Ell{hm.ncel, dmo.dm.K, As_diag.p, As_val.p, dmo.dm.ell_col}; }
+    Ell Ap() { return Ell{hm.ncel, dmo.dm.K, Ap_diag.p, Ap_val.p, dmo.dm.ell_col}; }
+    Colouring *colouring_for(const cfd_solve_cfg &c);
+    SolveStats solve(const Ell &A, const double *b, double *x, const cfd_solve_cfg &c,
+                     Colouring *col, int precision);
+
+    void InitFlowField();
+    void SetBCVelocity();
+    void SetBCPressure();
+    void SetBCDeltaP();
+    void SetBCTemperature();
+    void GradientUVWP();
+    void BuildVelocityMatrix();
+    int CalculateVelocity();
+    void CalRUFace2();
+    void BuildPressureMatrix();
+    void CorrectRUFace2();
+    int CalculatePressure();
+    void BuildScalarMatrix(const double *Phi, const double *BPhi, const double *Dc,
+                           const double *src, const double *App, int mode);
+    int UpdateTurKEpsilon();
+    int UpdateEnergy();
+    int Iterate(double &resmax);
+    int NSSolve(int &iters);
+    int check_flags(std::string &err);
+};
+
+}  // namespace cfd

@@ -1,0 +1,813 @@
+// Device-resident SIMPLE engine (host orchestration + C-ABI cfd_ns_*).
+#include "navier.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace cfd {
+
+// ---------------------------------------------------------- device mesh --
+void DevMeshOwner::build(const HostMesh &h, const RegTab &rt, const std::vector<int> &breg,
+                         hipStream_t s) {
+    ib.clear();
+    db.clear();
+    const int C = h.ncel, F = h.nfac, B = h.nbnd, V = h.nvrt;
+    auto upi = [&](const std::vector<int> &v) -> const int * {
+        ib.emplace_back(std::max<size_t>(v.size(), 1));
+        ib.back().upload(v.data(), v.size(), s);
+        return ib.back().p;
+    };
+    auto upd = [&](const std::vector<double> &v) -> const double * {
+        db.emplace_back(std::max<size_t>(v.size(), 1));
+        db.back().upload(v.data(), v.size(), s);
+        return db.back().p;
+    };
+    auto soa3 = [&](const std::vector<double> &v, int n, const double *out[3]) {
+        for (int k = 0; k < 3; ++k) {
+            std::vector<double> c(n);
+            for (int i = 0; i < n; ++i) c[i] = v[3 * (size_t)i + k];
+            out[k] = upd(c);
+        }
+    };
+    DevMesh &m = dm;
+    m = DevMesh();
+    m.ncel = C;
+    m.nfac = F;
+    m.nbnd = B;
+    m.ncb = h.ncb;
+    m.nvrt = V;
+    m.f_c1 = upi(h.face_c1);
+    m.f_c2 = upi(h.face_c2);
+    m.f_bnd = upi(h.face_bnd);
+    m.f_lambda = upd(h.face_lambda);
+    m.f_area = upd(h.face_area);
+    m.f_rlencos = upd(h.face_rlencos);
+    soa3(h.face_n, F, m.f_n);
+    soa3(h.face_x, F, m.f_x);
+    soa3(h.face_xpac, F, m.f_xp);
+    soa3(h.face_xnac, F, m.f_xn);
+    m.c_vol = upd(h.cell_vol);
+    soa3(h.cell_x, C, m.c_x);
+    soa3(h.vert, V, m.v_x);
+    m.c_nface = upi(h.cell_nface);
+    std::vector<int> cf(6 * (size_t)C, -1), gf(6 * (size_t)C, -1), bf(6 * (size_t)C, -1),
+        cv(8 * (size_t)C);
+    int K = 0;
+    for (int i = 0; i < C; ++i) {
+        int nf = h.cell_nface[i], nint = 0, nb = 0;
+        int srt[6], bs[6];
+        for (int j = 0; j < nf; ++j) {
+            int f = h.cell_face[6 * (size_t)i + j];
+            cf[(size_t)j * C + i] = f;
+            srt[j] = f;
+            if (h.face_c2[f] >= 0) nint++;
+            else bs[nb++] = h.face_bnd[f];
+        }
+        std::sort(srt, srt + nf);
+        std::sort(bs, bs + nb);
+        for (int j = 0; j < nf; ++j) gf[(size_t)j * C + i] = srt[j];
+        for (int j = 0; j < nb; ++j) bf[(size_t)j * C + i] = bs[j];
+        for (int j = 0; j < 8; ++j) cv[(size_t)j * C + i] = h.cell_vert[8 * (size_t)i + j];
+        K = std::max(K, nint);
+    }
+    if (K < 1) K = 1;
+    m.K = K;
+    m.c_face = upi(cf);
+    m.c_gface = upi(gf);
+    m.c_bface = upi(bf);
+    m.c_vert = upi(cv);
+    ell_col_host.assign((size_t)K * C, 0);
+    for (int i = 0; i < C; ++i) {
+        int nj = 0;
+        for (int j = 0; j < h.cell_nface[i]; ++j) {
+            int f = h.cell_face[6 * (size_t)i + j];
+            if (h.face_c2[f] < 0) continue;
+            int in = h.face_c1[f] == i ? h.face_c2[f] : h.face_c1[f];
+            ell_col_host[(size_t)nj * C + i] = in;
+            nj++;
+        }
+        for (; nj < K; ++nj) ell_col_host[(size_t)nj * C + i] = i;
+    }
+    m.ell_col = upi(ell_col_host);
+    // vertex -> cells (as listed in the 8-vertex form)
+    std::vector<int> ptr(V + 1, 0), lst(8 * (size_t)C);
+    for (int v : h.cell_vert) ptr[v + 1]++;
+    for (int v = 0; v < V; ++v) ptr[v + 1] += ptr[v];
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < C; ++i)
+        for (int j = 0; j < 8; ++j) lst[fill[h.cell_vert[8 * (size_t)i + j]]++] = i;
+    m.v2c_ptr = upi(ptr);
+    m.v2c = upi(lst);
+    std::vector<int> bcell(B);
+    for (int b = 0; b < B; ++b) bcell[b] = h.face_c1[h.bnd_face[b]];
+    m.b_face = upi(h.bnd_face);
+    m.b_cell = upi(bcell);
+    m.b_reg = upi(breg);
+    m.b_dist = upd(h.bnd_dist);
+    reg.alloc(1);
+    reg.upload(&rt, 1, s);
+    m.reg = reg.p;
+    CFD_HIP(hipStreamSynchronize(s));
+}
+
+static void alloc_n(std::initializer_list<DBuf<double> *> bufs, size_t n) {
+    for (auto *b : bufs) b->alloc(std::max<size_t>(n, 1));
+}
+
+// ------------------------------------------------------------- solver --
+NSSolver::~NSSolver() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+}
+
+int NSSolver::setup(const cfd_raw_mesh &raw, const cfd_params &p, std::string &err) {
+    P = p;
+    if (P.density_model != 0) { err = "only DensityModel 0 (constant density) is supported"; return -1; }
+    if (!P.steady) { err = "only steady runs are supported"; return -1; }
+    if (P.nregion > kMaxRegions - 1) { err = "too many regions"; return -1; }
+    reg = RegTab();
+    for (int r = 0; r < P.nregion; ++r) {
+        const cfd_region &R0 = P.region[r];
+        if (slot_of.count(R0.id)) { err = "repeating region id"; return -1; }
+        int k = reg.n++;
+        slot_of[R0.id] = k;
+        reg.id[k] = R0.id;
+        reg.type1[k] = R0.type1;
+        reg.type2[k] = R0.type2;
+        reg.fixed[k] = R0.fixed;
+        for (int t = 0; t < 10; ++t) reg.init[k][t] = R0.init[t];
+    }
+    int coupled_id = -1;
+    bool has_solid = false;
+    HostMesh solid;
+    if (build_parts(raw, P, hm, solid, has_solid, coupled_id, err)) return -1;
+    if (has_solid) {
+        // buildSolver (src/Preprocessor.cpp:453-464): the coupled region is a
+        // wall with type2 = 2 under the first unused id
+        int k = reg.n++;
+        slot_of[coupled_id] = k;
+        reg.id[k] = coupled_id;
+        reg.type1[k] = 1;
+        reg.type2[k] = 2;
+        reg.fixed[k] = 2147483647.0;
+        ht.reset(new HTSolver);
+        ht->ns = this;
+        ht->hm = std::move(solid);
+    }
+    for (int f = 0; f < hm.nfac; ++f)
+        if (hm.face_c2[f] < 0 && hm.face_bnd[f] < 0) { err = "boundary face not listed in the mesh"; return -1; }
+    auto map_bnd = [&](const HostMesh &h, std::vector<int> &br, bool solid) -> int {
+        br.resize(h.nbnd);
+        for (int b = 0; b < h.nbnd; ++b) {
+            auto it = slot_of.find(h.bnd_rid[b]);
+            if (it == slot_of.end()) { err = "boundary region not defined: " + std::to_string(h.bnd_rid[b]); return -1; }
+            int t1 = reg.type1[it->second];
+            if (solid ? !(t1 == 1 || t1 == 4) : (t1 < 1 || t1 > 4)) { err = "no such boundary type"; return -1; }
+            br[b] = it->second;
+        }
+        return 0;
+    };
+    if (map_bnd(hm, breg, false)) return -1;
+    for (int i = 0; i < hm.ncel; ++i) {
+        auto it = slot_of.find(hm.cell_rid[i]);
+        if (it == slot_of.end() || reg.type1[it->second] != 5 || reg.type2[it->second] != 0) {
+            err = "fluid cell region must be a fluid body";
+            return -1;
+        }
+    }
+    if (ht) {
+        if (map_bnd(ht->hm, ht->breg, true)) return -1;
+        if (ht->hm.nbnd - ht->hm.ncb != hm.nbnd - hm.ncb) { err = "coupled face counts differ"; return -1; }
+    }
+    CFD_HIP(hipStreamCreate(&s));
+    CFD_HIP(hipEventCreate(&ev0));
+    CFD_HIP(hipEventCreate(&ev1));
+    kry.reset(new Krylov(s));
+    dmo.build(hm, reg, breg, s);
+    const size_t C = hm.ncel, B = hm.nbnd, F = hm.nfac, K = dmo.dm.K;
+    alloc_n({&R, &U, &V, &W, &P_, &T, &TE, &ED, &VisLam, &VisTur, &Apr, &xsol, &bu, &bv, &bw, &bp,
+             &bs, &kcond, &As_diag, &Ap_diag, &Prod, &TES, &EDS, &ApTE, &ApED, &VisTE, &VisED,
+             &zeros},
+            C);
+    alloc_n({&gU, &gV, &gW, &gP, &gPhi}, 3 * C);
+    alloc_n({&BR, &BU, &BV, &BW, &BP, &BT, &BTE, &BED, &BndQ}, B);
+    alloc_n({&As_val, &Ap_val}, K * C);
+    RUFace.alloc(std::max<size_t>(F, 1));
+    res_dev.alloc(10);
+    gtmp.alloc(std::max<size_t>(3 * C, 2 * (size_t)hm.nvrt) + 1);
+    red_part.alloc(12 * kRedGrid);
+    red_tk.alloc(1);
+    red_tk.zero(s);
+    bcs.alloc(1);
+    errflag.alloc(1);
+    errflag.zero(s);
+    for (auto *b : {&R, &U, &V, &W, &P_, &T, &TE, &ED, &VisLam, &VisTur, &Apr, &xsol, &bu, &bv,
+                    &bw, &bp, &bs, &kcond, &As_diag, &Ap_diag, &gU, &gV, &gW, &gP, &gPhi, &BR,
+                    &BU, &BV, &BW, &BP, &BT, &BTE, &BED, &BndQ, &As_val, &Ap_val, &RUFace,
+                    &res_dev, &zeros, &Prod, &TES, &EDS, &ApTE, &ApED, &VisTE, &VisED})
+        b->zero(s);
+    if (ht) {
+        ht->dmo.build(ht->hm, reg, ht->breg, s);
+        ht->allocate(s);
+    }
+    InitFlowField();
+    CFD_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+// InitFlowField (src/navier.cpp:113-188), regions only (no initflow line)
+void NSSolver::InitFlowField() {
+    const int C = hm.ncel, B = hm.nbnd;
+    std::vector<double> u(C), v(C), w(C), r(C), t(C), pz(C, 0.), ml(C), mt(C, 0.), k(C, 0.),
+        e(C, 0.);
+    for (int i = 0; i < C; ++i) {
+        const double *iv = reg.init[slot_of[hm.cell_rid[i]]];
+        u[i] = iv[0]; v[i] = iv[1]; w[i] = iv[2]; r[i] = iv[3]; t[i] = iv[4];
+        ml[i] = iv[5];
+        if (P.turb_model == 1) {
+            k[i] = iv[6];
+            e[i] = iv[7];
+            mt[i] = r[i] * 0.09 * k[i] * k[i] / (e[i] + 1.e-16);
+        }
+    }
+    std::vector<double> bt(B, 0.), q(B, 0.);
+    for (int b = 0; b < B; ++b) {
+        int sl = breg[b];
+        if (reg.type1[sl] == 1) {
+            if (reg.type2[sl] == 0) bt[b] = reg.fixed[sl];
+            else if (reg.type2[sl] == 1) q[b] = reg.fixed[sl];
+        } else if (reg.type1[sl] == 2) {
+            bt[b] = reg.init[sl][5];
+        }
+    }
+    U.upload(u.data(), C, s); V.upload(v.data(), C, s); W.upload(w.data(), C, s);
+    R.upload(r.data(), C, s); T.upload(t.data(), C, s); P_.upload(pz.data(), C, s);
+    VisLam.upload(ml.data(), C, s); VisTur.upload(mt.data(), C, s);
+    TE.upload(k.data(), C, s); ED.upload(e.data(), C, s);
+    BT.upload(bt.data(), B, s); BndQ.upload(q.data(), B, s);
+    RUFace.zero(s);
+    CFD_HIP(hipStreamSynchronize(s));
+    if (ht) ht->InitFlowField();
+}
+
+Colouring *NSSolver::colouring_for(const cfd_solve_cfg &c) {
+    if (c.precond != CFD_PC_SSOR && c.precond != CFD_PC_ILU) return nullptr;
+    if (!colour_ready) {
+        colour.build(dmo.ell_col_host.data(), hm.ncel, dmo.dm.K, hm.colour.data(), dmo.dm.ell_col, s);
+        colour_ready = true;
+    }
+    return &colour;
+}
+
+SolveStats NSSolver::solve(const Ell &A, const double *b, double *x, const cfd_solve_cfg &c,
+                           Colouring *col, int precision) {
+    (void)precision;
+    return kry->solve(A, b, x, c, col);
+}
+
+void NSSolver::SetBCVelocity() {
+    launch_bc_velocity(dm(), R, U, V, W, BR, BU, BV, BW, bcs.p, red(), s);
+}
+void NSSolver::SetBCPressure() { launch_bc_pressure(dm(), P_, BP, s); }
+void NSSolver::SetBCDeltaP() { launch_bc_pressure(dm(), xsol, BP, s); }
+void NSSolver::SetBCTemperature() { launch_bc_temperature(dm(), T, BT, s); }
+
+void NSSolver::GradientUVWP() {
+    const double *phi[4] = {U, V, W, P_}, *bphi[4] = {BU, BV, BW, BP};
+    double *out[4] = {gU, gV, gW, gP};
+    launch_gradient(dm(), 4, phi, bphi, out, P.limiter, gtmp, s);
+}
+
+// BuildVelocityMatrix (src/navier_velocity.cpp:77-357)
+void NSSolver::BuildVelocityMatrix() {
+    SetBCVelocity();
+    SetBCPressure();
+    GradientUVWP();
+    VelArgs a;
+    a.R = R; a.U = U; a.V = V; a.W = W; a.VisLam = VisLam; a.VisTur = VisTur; a.RUFace = RUFace;
+    a.gU = gU; a.gV = gV; a.gW = gW; a.gP = gP; a.BU = BU; a.BV = BV; a.BW = BW;
+    a.urf = P.urf[0];
+    for (int k = 0; k < 3; ++k) a.grav[k] = P.gravity[k];
+    a.diag = As_diag; a.val = As_val; a.bu = bu; a.bv = bv; a.bw = bw; a.Apr = Apr;
+    launch_build_velocity(dm(), a, errflag, s);
+    last_as = "velocity";
+}
+
+// CalculateVelocity (src/navier_velocity.cpp:12-75)
+int NSSolver::CalculateVelocity() {
+    launch_fill(res_dev, 10, 0., s);
+    BuildVelocityMatrix();
+    Colouring *col = colouring_for(P.mom);
+    double *fld[3] = {U, V, W};
+    const double *rhs[3] = {bu, bv, bw};
+    SolveStats *out[3] = {&last_u, &last_v, &last_w};
+    for (int c = 0; c < 3; ++c) {
+        CFD_HIP(hipMemcpyAsync(xsol, fld[c], sizeof(double) * hm.ncel, hipMemcpyDeviceToDevice, s));
+        *out[c] = solve(As(), rhs[c], xsol, P.mom, col, 0);
+        stats[2] += out[c]->iters;
+        if (out[c]->iters >= P.mom.maxit && out[c]->relres > P.mom.tol) {
+            last_error() = "velocity cannot converge";
+            return -1;
+        }
+        launch_take_solution(dm(), xsol, fld[c], res_dev.p + c, 0., 0, red(), s);
+    }
+    return 0;
+}
+
+void NSSolver::CalRUFace2() {
+    launch_rhie_chow(dm(), R, U, V, W, P_, gP, Apr, BR, BU, BV, BW, RUFace, s);
+}
+
+void NSSolver::BuildPressureMatrix() {
+    SetBCVelocity();
+    CalRUFace2();
+    launch_build_pressure(dm(), R, Apr, RUFace, Ap_diag, Ap_val, bp, s);
+}
+
+void NSSolver::CorrectRUFace2() {
+    launch_correct_ruface(dm(), R, Apr, xsol, gP, BR, BU, BV, BW, P.urf[3], RUFace, s);
+}
+
+// CalculatePressure (src/navier_pressure.cpp:9-76), DensityModel 0
+int NSSolver::CalculatePressure() {
+    launch_fill(xsol, hm.ncel, 0., s);
+    BuildPressureMatrix();
+    kry->prof = prof_pressure;
+    last_p = solve(Ap(), bp, xsol, P.pres, colouring_for(P.pres), P.pres_precision);
+    kry->prof = false;
+    stats[3] += last_p.iters;
+    if (!(last_p.relres <= 1.)) {
+        last_error() = "pressure iteration cannot converge";
+        return -1;
+    }
+    SetBCDeltaP();
+    const double *phi[1] = {xsol};
+    const double *bphi[1] = {BP};
+    double *out[1] = {gP};
+    launch_gradient(dm(), 1, phi, bphi, out, P.limiter, gtmp, s);
+    launch_pressure_update(dm(), xsol, P.cell_pressure_ref, P.urf[3], Apr, gP, P_, U, V, W,
+                           res_dev.p + 3, red(), s);
+    SetBCVelocity();
+    CorrectRUFace2();
+    return 0;
+}
+
+// NavierStokesSolver::BuildScalarMatrix (src/navier_scalar.cpp:466-647)
+void NSSolver::BuildScalarMatrix(const double *Phi, const double *BPhi, const double *Dc,
+                                 const double *src, const double *App, int mode) {
+    const double *phi[1] = {Phi}, *bphi[1] = {BPhi};
+    double *out[1] = {gPhi};
+    launch_gradient(dm(), 1, phi, bphi, out, P.limiter, gtmp, s);
+    ScaArgs a;
+    a.Phi = Phi; a.BPhi = BPhi; a.D = Dc; a.src = src; a.App = App; a.RUFace = RUFace; a.g = gPhi;
+    a.BndQ = BndQ; a.urf = P.urf[5]; a.mode = mode;
+    a.diag = As_diag; a.val = As_val; a.bs = bs;
+    launch_build_scalar(dm(), a, errflag, s);
+}
+
+// UpdateTurKEpsilon (src/navier_scalar.cpp:14-172), steady
+int NSSolver::UpdateTurKEpsilon() {
+    KeArgs a;
+    a.R = R; a.VisLam = VisLam; a.gU = gU; a.gV = gV; a.gW = gW; a.U = U; a.V = V; a.W = W;
+    a.VisTur = VisTur; a.TE = TE; a.ED = ED; a.BTE = BTE; a.BED = BED;
+    a.Prod = Prod; a.TES = TES; a.EDS = EDS; a.ApTE = ApTE; a.ApED = ApED; a.VisTE = VisTE;
+    a.VisED = VisED;
+    launch_ke_sources(dm(), a, s);
+    launch_ke_wall(dm(), a, s);
+    Colouring *col = colouring_for(P.ke);
+    BuildScalarMatrix(TE, BTE, VisTE, TES, ApTE, 0);
+    last_as = "k";
+    CFD_HIP(hipMemcpyAsync(xsol, TE, sizeof(double) * hm.ncel, hipMemcpyDeviceToDevice, s));
+    last_k = solve(As(), bs, xsol, P.ke, col, 0);
+    stats[4] += last_k.iters;
+    if (last_k.iters >= P.ke.maxit && last_k.relres > P.ke.tol) {
+        last_error() = "k cannot converge";
+        return -1;
+    }
+    launch_take_solution(dm(), xsol, TE, nullptr, 1.e-9, 1, red(), s);
+    launch_ke_eps_sources(dm(), a, s);
+    BuildScalarMatrix(ED, BED, VisED, EDS, ApED, 0);
+    launch_eps_wall_rows(dm(), ED, As_diag, As_val, bs, s);
+    last_as = "eps";
+    CFD_HIP(hipMemcpyAsync(xsol, ED, sizeof(double) * hm.ncel, hipMemcpyDeviceToDevice, s));
+    last_e = solve(As(), bs, xsol, P.ke, col, 0);
+    stats[4] += last_e.iters;
+    if (last_e.iters >= P.ke.maxit && last_e.relres > P.ke.tol) {
+        last_error() = "epsilon cannot converge";
+        return -1;
+    }
+    launch_take_solution(dm(), xsol, ED, nullptr, 1.e-12, 1, red(), s);
+    launch_ke_visc(dm(), a, s);
+    return 0;
+}
+
+// UpdateEnergy (src/navier_scalar.cpp:257-397), steady, DensityModel 0
+int NSSolver::UpdateEnergy() {
+    launch_kcond(dm(), VisLam, VisTur, P.cp, P.prl, P.prte, kcond, s);
+    SetBCTemperature();
+    if (ht) launch_solid2fluid(ht->dmo.dm, ht->T, ht->g, hm.ncb, BT, s);
+    BuildScalarMatrix(T, BT, kcond, zeros, zeros, 1);
+    last_as = "T";
+    CFD_HIP(hipMemcpyAsync(xsol, T, sizeof(double) * hm.ncel, hipMemcpyDeviceToDevice, s));
+    last_t = solve(As(), bs, xsol, P.energy, colouring_for(P.energy), 0);
+    stats[5] += last_t.iters;
+    if (last_t.iters >= P.energy.maxit && last_t.relres > P.energy.tol) {
+        last_error() = "energy cannot converge";
+        return -1;
+    }
+    CFD_HIP(hipMemcpyAsync(T, xsol, sizeof(double) * hm.ncel, hipMemcpyDeviceToDevice, s));
+    if (ht) launch_fluid2solid(dm(), ht->dmo.dm, gPhi, kcond, BndQ, ht->BndQ, s);
+    return 0;
+}
+
+int NSSolver::check_flags(std::string &err) {
+    int f = 0;
+    CFD_HIP(hipMemcpyAsync(&f, errflag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    CFD_HIP(hipStreamSynchronize(s));
+    if (f == 1) { err = "no such boundary type"; return -1; }
+    if (f == 2) { err = "reverse flow get out of inlet"; return -1; }
+    return 0;
+}
+
+// NSSolve loop body (src/navier.cpp:36-68), steady
+int NSSolver::Iterate(double &resmax) {
+    CFD_HIP(hipEventRecord(ev0, s));
+    if (CalculateVelocity()) return -1;
+    if (CalculatePressure()) return -1;
+    if (P.turb_model == 1 && UpdateTurKEpsilon()) return -1;
+    if (P.solve_energy) {
+        if (UpdateEnergy()) return -1;
+        if (ht && ht->UpdateEnergy()) return -1;
+    }
+    CFD_HIP(hipEventRecord(ev1, s));
+    res_dev.download(Residual, 10, s);
+    std::string err;
+    if (check_flags(err)) {
+        last_error() = err;
+        return -1;
+    }
+    float ms = 0.f;
+    CFD_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+    stats[0] += 1;
+    stats[1] += ms * 1e-3;
+    double m = 0.;
+    for (int i = 0; i < 10; ++i) m = Residual[i] > m ? Residual[i] : m;  // vec_max
+    resmax = m;
+    return 0;
+}
+
+int NSSolver::NSSolve(int &iters) {
+    iters = 0;
+    for (int it = 1; it < P.max_step; ++it) {
+        double r;
+        if (Iterate(r)) return -1;
+        iters = it;
+        if (r < P.residual_steady) break;
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------ solid --
+void HTSolver::allocate(hipStream_t s) {
+    const size_t C = hm.ncel, B = hm.nbnd, K = dmo.dm.K;
+    alloc_n({&T, &R, &D, &xsol, &bs, &src, &diag}, C);
+    alloc_n({&g}, 3 * C);
+    alloc_n({&BT, &BndQ}, B);
+    val.alloc(std::max<size_t>(K * C, 1));
+    gtmp.alloc(std::max<size_t>(3 * C, 2 * (size_t)hm.nvrt) + 1);
+    for (auto *b : {&T, &R, &D, &xsol, &bs, &src, &diag, &g, &BT, &BndQ, &val}) b->zero(s);
+}
+
+// HeatConductionSolver::InitFlowField (src/HeatConductionSolver.cpp:269-304)
+void HTSolver::InitFlowField() {
+    NSSolver &n = *ns;
+    const int C = hm.ncel, B = hm.nbnd;
+    std::vector<double> t(C), r(C), d(C), bt(B, 0.), q(B, 0.);
+    for (int i = 0; i < C; ++i) {
+        const double *iv = n.reg.init[n.slot_of[hm.cell_rid[i]]];
+        t[i] = iv[0]; r[i] = iv[1]; d[i] = iv[2];
+    }
+    for (int b = 0; b < B; ++b) {
+        int sl = breg[b];
+        if (n.reg.type1[sl] == 1) {
+            if (n.reg.type2[sl] == 0) bt[b] = n.reg.fixed[sl];
+            else if (n.reg.type2[sl] == 1) q[b] = n.reg.fixed[sl];
+        }
+    }
+    T.upload(t.data(), C, n.s); R.upload(r.data(), C, n.s); D.upload(d.data(), C, n.s);
+    BT.upload(bt.data(), B, n.s); BndQ.upload(q.data(), B, n.s);
+    SetBCTemperature();
+    const double *phi[1] = {T}, *bphi[1] = {BT};
+    double *out[1] = {g};
+    launch_gradient(dmo.dm, 1, phi, bphi, out, n.P.limiter, gtmp, n.s);
+    CFD_HIP(hipStreamSynchronize(n.s));
+}
+
+void HTSolver::SetBCTemperature() { launch_solid_bc_temperature(dmo.dm, T, D, BndQ, BT, ns->s); }
+
+// HeatConductionSolver::BuildScalarMatrix (src/HeatConductionSolver.cpp:110-223)
+void HTSolver::BuildScalarMatrix() {
+    const double *phi[1] = {T}, *bphi[1] = {BT};
+    double *out[1] = {g};
+    launch_gradient(dmo.dm, 1, phi, bphi, out, ns->P.limiter, gtmp, ns->s);
+    ScaArgs a;
+    a.Phi = T; a.BPhi = BT; a.D = D; a.src = src; a.App = nullptr; a.RUFace = nullptr; a.g = g;
+    a.BndQ = BndQ; a.urf = URF; a.mode = 2;
+    a.diag = diag; a.val = val; a.bs = bs;
+    launch_build_scalar(dmo.dm, a, ns->errflag, ns->s);
+}
+
+// HeatConductionSolver::UpdateEnergy (src/HeatConductionSolver.cpp:47-106)
+int HTSolver::UpdateEnergy() {
+    hipStream_t s = ns->s;
+    launch_scale_source(dmo.dm, ns->P.solid_heat_source, src, s);
+    SetBCTemperature();
+    BuildScalarMatrix();
+    CFD_HIP(hipMemcpyAsync(xsol, T, sizeof(double) * hm.ncel, hipMemcpyDeviceToDevice, s));
+    Colouring *col = nullptr;
+    if (ns->P.solid.precond == CFD_PC_SSOR || ns->P.solid.precond == CFD_PC_ILU) {
+        if (!colour_ready) {
+            colour.build(dmo.ell_col_host.data(), hm.ncel, dmo.dm.K, hm.colour.data(),
+                         dmo.dm.ell_col, s);
+            colour_ready = true;
+        }
+        col = &colour;
+    }
+    Ell A{hm.ncel, dmo.dm.K, diag.p, val.p, dmo.dm.ell_col};
+    last = ns->kry->solve(A, bs, xsol, ns->P.solid, col);
+    ns->stats[6] += last.iters;
+    if (last.iters >= ns->P.solid.maxit && last.relres > ns->P.solid.tol) {
+        last_error() = "solid energy cannot converge";
+        return -1;
+    }
+    CFD_HIP(hipMemcpyAsync(T, xsol, sizeof(double) * hm.ncel, hipMemcpyDeviceToDevice, s));
+    return 0;
+}
+
+}  // namespace cfd
+
+// ============================================================ C-ABI ==
+using namespace cfd;
+
+struct cfd_ns_s {
+    NSSolver ns;
+};
+
+namespace {
+
+template <class F>
+int guarded(F f) {
+    try {
+        return f();
+    } catch (const std::exception &e) {
+        last_error() = e.what();
+        return -1;
+    }
+}
+
+struct FieldRef {
+    DBuf<double> *buf = nullptr;
+    size_t n = 0;
+    bool vec3 = false;  // host [n][3] <-> device [3][n]
+};
+
+FieldRef field_of(cfd_ns_s *h, int part, const std::string &s) {
+    NSSolver &n = h->ns;
+    FieldRef f;
+    if (part == 1) {
+        if (!n.ht) return f;
+        HTSolver &t = *n.ht;
+        size_t C = t.hm.ncel, B = t.hm.nbnd;
+        if (s == "Tn") f = {&t.T, C};
+        else if (s == "Rn") f = {&t.R, C};
+        else if (s == "diffCoef") f = {&t.D, C};
+        else if (s == "dPhidX") f = {&t.g, C, true};
+        else if (s == "BTem") f = {&t.BT, B};
+        else if (s == "BndQ") f = {&t.BndQ, B};
+        else if (s == "xsol") f = {&t.xsol, C};
+        else if (s == "bs") f = {&t.bs, C};
+        return f;
+    }
+    size_t C = n.hm.ncel, B = n.hm.nbnd, F = n.hm.nfac;
+    struct E {
+        const char *nm;
+        DBuf<double> *b;
+        size_t n;
+        bool v3;
+    } tab[] = {{"Rn", &n.R, C, false},       {"Un", &n.U, C, false},
+               {"Vn", &n.V, C, false},       {"Wn", &n.W, C, false},
+               {"Pn", &n.P_, C, false},      {"Tn", &n.T, C, false},
+               {"TE", &n.TE, C, false},      {"ED", &n.ED, C, false},
+               {"VisLam", &n.VisLam, C, false}, {"VisTur", &n.VisTur, C, false},
+               {"Apr", &n.Apr, C, false},    {"RUFace", &n.RUFace, F, false},
+               {"dPdX", &n.gP, C, true},     {"dUdX", &n.gU, C, true},
+               {"dVdX", &n.gV, C, true},     {"dWdX", &n.gW, C, true},
+               {"dPhidX", &n.gPhi, C, true}, {"BRo", &n.BR, B, false},
+               {"BU", &n.BU, B, false},      {"BV", &n.BV, B, false},
+               {"BW", &n.BW, B, false},      {"BPre", &n.BP, B, false},
+               {"BTem", &n.BT, B, false},    {"BTE", &n.BTE, B, false},
+               {"BED", &n.BED, B, false},    {"BndQ", &n.BndQ, B, false},
+               {"xsol", &n.xsol, C, false},  {"bu", &n.bu, C, false},
+               {"bv", &n.bv, C, false},      {"bw", &n.bw, C, false},
+               {"bp", &n.bp, C, false},      {"bs", &n.bs, C, false}};
+    for (auto &e : tab)
+        if (s == e.nm) return FieldRef{e.b, e.n, e.v3};
+    return f;
+}
+
+}  // namespace
+
+extern "C" {
+
+cfd_ns *cfd_ns_create(const cfd_raw_mesh *m, const cfd_params *p) {
+    cfd_ns *h = new cfd_ns;
+    std::string err;
+    int rc = guarded([&] { return h->ns.setup(*m, *p, err); });
+    if (rc) {
+        if (!err.empty()) last_error() = err;
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+void cfd_ns_free(cfd_ns *h) {
+    if (!h) return;
+    try {
+        if (h->ns.s) (void)hipStreamSynchronize(h->ns.s);
+        hipStream_t s = h->ns.s;
+        h->ns.kry.reset();
+        delete h;
+        if (s) (void)hipStreamDestroy(s);
+    } catch (...) {
+    }
+}
+
+int cfd_ns_counts(const cfd_ns *h, int part, int c[5]) {
+    const HostMesh *m = part == 1 ? (h->ns.ht ? &h->ns.ht->hm : nullptr) : &h->ns.hm;
+    if (!m) { last_error() = "no solid part"; return -1; }
+    c[0] = m->nvrt; c[1] = m->ncel; c[2] = m->nfac; c[3] = m->nbnd; c[4] = m->ncb;
+    return 0;
+}
+
+int cfd_ns_call(cfd_ns *h, const char *method) {
+    return guarded([&]() -> int {
+        NSSolver &n = h->ns;
+        std::string m(method);
+        int rc = 0;
+        if (m == "NSSolve") { int it; rc = n.NSSolve(it); }
+        else if (m == "Iterate") { double r; rc = n.Iterate(r); }
+        else if (m == "CalculateVelocity") rc = n.CalculateVelocity();
+        else if (m == "BuildVelocityMatrix") n.BuildVelocityMatrix();
+        else if (m == "CalRUFace2") n.CalRUFace2();
+        else if (m == "CalculatePressure") rc = n.CalculatePressure();
+        else if (m == "BuildPressureMatrix") n.BuildPressureMatrix();
+        else if (m == "CorrectRUFace2") n.CorrectRUFace2();
+        else if (m == "SetBCVelocity") n.SetBCVelocity();
+        else if (m == "SetBCPressure") n.SetBCPressure();
+        else if (m == "SetBCDeltaP") n.SetBCDeltaP();
+        else if (m == "SetBCTemperature") n.SetBCTemperature();
+        else if (m == "UpdateTurKEpsilon") rc = n.UpdateTurKEpsilon();
+        else if (m == "UpdateEnergy") rc = n.UpdateEnergy();
+        else if (m == "SolidSolve") rc = n.ht ? n.ht->UpdateEnergy() : -1;
+        else if (m == "InitFlowField") n.InitFlowField();
+        else if (m == "GradientU" || m == "GradientV" || m == "GradientW" || m == "GradientP" ||
+                 m == "GradientT") {
+            char c = m[8];
+            const double *phi = c == 'U' ? n.U.p : c == 'V' ? n.V.p : c == 'W' ? n.W.p : c == 'P' ? n.P_.p : n.T.p;
+            const double *bphi = c == 'U' ? n.BU.p : c == 'V' ? n.BV.p : c == 'W' ? n.BW.p : c == 'P' ? n.BP.p : n.BT.p;
+            double *out = c == 'U' ? n.gU.p : c == 'V' ? n.gV.p : c == 'W' ? n.gW.p : c == 'P' ? n.gP.p : n.gPhi.p;
+            launch_gradient(n.dm(), 1, &phi, &bphi, &out, n.P.limiter, n.gtmp, n.s);
+        } else if (m == "SolidGradientT") {
+            if (!n.ht) return -1;
+            const double *phi = n.ht->T.p, *bphi = n.ht->BT.p;
+            double *out = n.ht->g.p;
+            launch_gradient(n.ht->dmo.dm, 1, &phi, &bphi, &out, n.P.limiter, n.ht->gtmp, n.s);
+        } else {
+            last_error() = "unknown method " + m;
+            return -2;
+        }
+        CFD_HIP(hipStreamSynchronize(n.s));
+        if (rc == 0) {
+            std::string err;
+            if (n.check_flags(err)) { last_error() = err; return -1; }
+        }
+        return rc;
+    });
+}
+
+int cfd_ns_field(cfd_ns *h, int part, const char *name, double *host, size_t count, int dir) {
+    return guarded([&]() -> int {
+        NSSolver &n = h->ns;
+        std::string s(name);
+        if (s == "Residual" && part == 0) {
+            if (count != 10) { last_error() = "Residual has 10 entries"; return -1; }
+            if (dir == 0) std::memcpy(host, n.Residual, sizeof(n.Residual));
+            else std::memcpy(n.Residual, host, sizeof(n.Residual));
+            return 0;
+        }
+        FieldRef f = field_of(h, part, s);
+        if (!f.buf) { last_error() = "unknown field " + s; return -1; }
+        size_t tot = f.vec3 ? 3 * f.n : f.n;
+        if (count != tot) { last_error() = "size mismatch for " + s; return -1; }
+        std::vector<double> tmp(tot);
+        if (dir == 0) {
+            f.buf->download(tmp.data(), tot, n.s);
+            CFD_HIP(hipStreamSynchronize(n.s));
+            if (f.vec3)
+                for (size_t i = 0; i < f.n; ++i)
+                    for (int k = 0; k < 3; ++k) host[3 * i + k] = tmp[k * f.n + i];
+            else
+                std::memcpy(host, tmp.data(), sizeof(double) * tot);
+        } else {
+            if (f.vec3)
+                for (size_t i = 0; i < f.n; ++i)
+                    for (int k = 0; k < 3; ++k) tmp[k * f.n + i] = host[3 * i + k];
+            else
+                std::memcpy(tmp.data(), host, sizeof(double) * tot);
+            f.buf->upload(tmp.data(), tot, n.s);
+            CFD_HIP(hipStreamSynchronize(n.s));
+        }
+        return 0;
+    });
+}
+
+int cfd_ns_matrix(cfd_ns *h, const char *which, int *row_len, int *col, double *val) {
+    return guarded([&]() -> int {
+        NSSolver &n = h->ns;
+        std::string w(which);
+        const HostMesh *hm = &n.hm;
+        const DevMeshOwner *dmo = &n.dmo;
+        DBuf<double> *dg, *vl;
+        bool upper = false;
+        if (w == "As") { dg = &n.As_diag; vl = &n.As_val; }
+        else if (w == "Ap") { dg = &n.Ap_diag; vl = &n.Ap_val; upper = true; }
+        else if (w == "SolidAs" && n.ht) { dg = &n.ht->diag; vl = &n.ht->val; hm = &n.ht->hm; dmo = &n.ht->dmo; }
+        else { last_error() = "unknown matrix"; return -1; }
+        const int C = hm->ncel, K = dmo->dm.K;
+        std::vector<double> d(C), v((size_t)K * C);
+        dg->download(d.data(), C, n.s);
+        vl->download(v.data(), (size_t)K * C, n.s);
+        CFD_HIP(hipStreamSynchronize(n.s));
+        size_t k = 0;
+        for (int i = 0; i < C; ++i) {
+            int len = 1;
+            if (col) { col[k] = i; val[k] = d[i]; ++k; }
+            for (int t = 0; t < K; ++t) {
+                int c = dmo->ell_col_host[(size_t)t * C + i];
+                if (c == i || (upper && c < i)) continue;
+                if (col) { col[k] = c; val[k] = v[(size_t)t * C + i]; ++k; }
+                ++len;
+            }
+            row_len[i] = len;
+        }
+        return 0;
+    });
+}
+
+int cfd_ns_last_solve(const cfd_ns *h, const char *eq, int *iters, double *relres) {
+    const NSSolver &n = h->ns;
+    std::string e(eq);
+    const SolveStats *s = e == "u" ? &n.last_u : e == "v" ? &n.last_v : e == "w" ? &n.last_w
+                        : e == "p" ? &n.last_p : e == "k" ? &n.last_k : e == "eps" ? &n.last_e
+                        : e == "T" ? &n.last_t : (e == "solid" && n.ht) ? &n.ht->last : nullptr;
+    if (!s) { last_error() = "unknown equation"; return -1; }
+    *iters = s->iters;
+    *relres = s->relres;
+    return 0;
+}
+
+int cfd_ns_profile(cfd_ns *h, int enable) {
+    h->ns.prof_pressure = enable != 0;
+    h->ns.kry->prof_ms = 0.;
+    h->ns.kry->prof_launches = 0;
+    return 0;
+}
+
+int cfd_ns_profile_read(const cfd_ns *h, double out[4]) {
+    const NSSolver &n = h->ns;
+    long long nnz = 0;
+    for (size_t k = 0; k < n.dmo.ell_col_host.size(); ++k)
+        if (n.dmo.ell_col_host[k] != (int)(k % n.hm.ncel)) ++nnz;
+    // k_cg_pq per launch: reads r|z, diag, p_old (3 x 8 B per row), writes
+    // p_new, q (2 x 8 B per row); 8 B value + 4 B column per stored
+    // off-diagonal; neighbour gathers of the vectors hit cache and are not
+    // counted (each vector is read once)
+    out[0] = (double)n.kry->prof_launches;
+    out[1] = n.kry->prof_ms;
+    out[2] = 40.0 * n.hm.ncel + 12.0 * (double)nnz;
+    out[3] = (double)nnz;
+    return 0;
+}
+
+int cfd_ns_stats(const cfd_ns *h, double out[8]) {
+    std::memcpy(out, h->ns.stats, sizeof(h->ns.stats));
+    return 0;
+}
+int cfd_ns_reset_stats(cfd_ns *h) {
+    std::memset(h->ns.stats, 0, sizeof(h->ns.stats));
+    return 0;
+}
+
+}  // extern "C"

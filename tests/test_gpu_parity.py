@@ -1,0 +1,141 @@
+"""GPU parity: each device method against the oracle on the same state.
+
+Assembly, gradients, Rhie-Chow and corrections are compared bit for bit on
+meshes without an outlet (no global reduction enters them); with an outlet
+the mass-balance sums of SetBCVelocity are tree reductions on the device and
+the tolerance is 1e-13 relative. Krylov solves are compared within their
+stopping tolerance.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _solvers():
+    import sysu_cfd_heat_amd as P
+    from oracle import oracle as O
+    return P, O
+
+
+def _random_state(gpu, ref, seed, names=("Un", "Vn", "Wn", "Pn", "VisTur"), faces=True):
+    rng = np.random.default_rng(seed)
+    n = gpu.counts()[1]
+    for nm in names:
+        v = rng.standard_normal(n) * (0.1 if nm == "VisTur" else 1.0)
+        if nm == "VisTur":
+            v = np.abs(v) * 1e-3
+        gpu.set(nm, v)
+        ref.set(nm, v)
+    if faces:
+        f = rng.standard_normal(gpu.counts()[2]) * 0.05
+        gpu.set("RUFace", f)
+        ref.set("RUFace", f)
+    apr = 0.5 + rng.random(n)
+    gpu.set("Apr", apr)
+    ref.set("Apr", apr)
+
+
+def _cases():
+    from sysu_cfd_heat_amd import cases
+    return {
+        "cavity_hex": cases.cavity(n=6, max_step=4),
+        "cavity_tet": cases.cavity(n=4, tets=True, jitter=0.1, seed=5, max_step=4),
+        "channel_cht": cases.channel(nx=8, ny=4, nz=4, tets=True, jitter=0.1, seed=2,
+                                     max_step=4),
+    }
+
+
+CASES = ["cavity_hex", "cavity_tet", "channel_cht"]
+
+
+def _pair(name):
+    P, O = _solvers()
+    raw, p = _cases()[name]
+    return P.NSSolver(raw, p), O.NSSolver(raw, p, mc=1)
+
+
+def _cmp(a, b, exact, rtol=1e-13, what=""):
+    if exact:
+        assert np.array_equal(a, b), f"{what}: max diff {np.abs(a - b).max():.3e}"
+    else:
+        scale = max(np.abs(b).max(), 1e-300)
+        assert np.abs(a - b).max() <= rtol * scale, f"{what}: {np.abs(a - b).max() / scale:.3e}"
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_mesh_and_split_identical(case):
+    gpu, ref = _pair(case)
+    assert gpu.counts(0) == ref.counts(0)
+    if gpu.has_solid:
+        assert gpu.counts(1) == ref.counts(1)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_build_velocity_matrix(case):
+    gpu, ref = _pair(case)
+    _random_state(gpu, ref, 1)
+    gpu.call("BuildVelocityMatrix")
+    ref.call("BuildVelocityMatrix")
+    exact = case != "channel_cht"
+    for nm in ("BU", "BV", "BW", "BRo", "BPre", "dUdX", "dVdX", "dWdX", "dPdX", "Apr", "bu",
+               "bv", "bw"):
+        _cmp(gpu.get(nm), ref.get(nm), exact, what=nm)
+    ga, ra = gpu.matrix("As"), ref.matrix("As")
+    assert np.array_equal(ga[0], ra[0]) and np.array_equal(ga[1], ra[1])
+    _cmp(ga[2], ra[2], exact, what="As values")
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_rhie_chow_and_pressure_matrix(case):
+    gpu, ref = _pair(case)
+    _random_state(gpu, ref, 2)
+    for s in (gpu, ref):
+        s.call("BuildVelocityMatrix")  # Apr, dPdX from the same state
+        s.call("BuildPressureMatrix")
+    exact = case != "channel_cht"
+    _cmp(gpu.get("RUFace"), ref.get("RUFace"), exact, what="RUFace")
+    _cmp(gpu.get("bp"), ref.get("bp"), exact, rtol=1e-12, what="bp")
+    ga, ra = gpu.matrix("Ap"), ref.matrix("Ap")
+    assert np.array_equal(ga[0], ra[0]) and np.array_equal(ga[1], ra[1])
+    _cmp(ga[2], ra[2], exact, what="Ap values")
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_gradient_and_correction(case):
+    gpu, ref = _pair(case)
+    _random_state(gpu, ref, 3)
+    rng = np.random.default_rng(7)
+    dp = rng.standard_normal(gpu.counts()[1])
+    for s in (gpu, ref):
+        s.call("BuildVelocityMatrix")
+        s.set("xsol", dp)
+        s.call("SetBCDeltaP")
+        s.call("GradientP")
+        s.call("SetBCVelocity")
+        s.call("CorrectRUFace2")
+    exact = case != "channel_cht"
+    _cmp(gpu.get("dPdX"), ref.get("dPdX"), exact, what="grad p'")
+    _cmp(gpu.get("RUFace"), ref.get("RUFace"), exact, what="RUFace corrected")
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_simple_iterations_match_oracle(case):
+    gpu, ref = _pair(case)
+    for it in range(3):
+        gpu.call("Iterate")
+        ref.call("Iterate")
+    names = ["Un", "Vn", "Wn", "Pn"]
+    if case == "channel_cht":
+        names += ["TE", "ED", "Tn"]
+    for nm in names:
+        e = rel_l2(gpu.get(nm), ref.get(nm))
+        assert e <= 1e-6, f"{nm}: rel-L2 {e:.3e}"
+    if gpu.has_solid:
+        e = rel_l2(gpu.get("Tn", part=1), ref.get("Tn", part=1))
+        assert e <= 1e-6, f"solid T: rel-L2 {e:.3e}"
+    for eq in ("u", "p"):
+        it, rr = gpu.last_solve(eq)
+        assert rr < 1e-7, (eq, it, rr)

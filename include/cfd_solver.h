@@ -212,10 +212,12 @@ int cfd_ns_reset_stats(cfd_ns *h);
 
 /* Live HIP-event timing of the pressure CG's SpMV kernel (the dominant
  * kernel of the step), recorded on the solver's own stream around each
- * launch. enable resets the counters. out = {launches that did work,
- * total ms, algorithmic bytes per launch, stored off-diagonals}. */
+ * launch. enable resets the counters. out = {FP64 launches that did work,
+ * their total ms, FP64 algorithmic bytes per launch, stored off-diagonals,
+ * FP32 launches (mixed-precision inner solves), their total ms, FP32 bytes
+ * per launch, 0}. */
 int cfd_ns_profile(cfd_ns *h, int enable);
-int cfd_ns_profile_read(const cfd_ns *h, double out[4]);
+int cfd_ns_profile_read(const cfd_ns *h, double out[8]);
 
 const char *cfd_last_error(void);
 

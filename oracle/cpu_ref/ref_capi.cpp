@@ -94,7 +94,6 @@ void *oracle_mesh_build(const cfd_raw_mesh *raw) {
     Mesh *m = new Mesh;
     load_raw(*raw, *m);
     if (create_faces(*m)) { g_err = "face construction failed"; delete m; return nullptr; }
-    m->ncb = m->nbnd;
     int rc = cell_face_info(*m);
     if (rc) { g_err = "geometry check failed " + std::to_string(rc); delete m; return nullptr; }
     return m;

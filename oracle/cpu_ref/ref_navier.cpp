@@ -645,6 +645,14 @@ int RefNS::CorrectRUFace2(const double *dp) {
 int RefNS::CalculatePressure() {
     for (int i = 0; i < M.ncel; ++i) xsol[i] = 0.;
     if (BuildPressureMatrix()) return -1;
+    if (P.pres_project) {
+        // extension: the pure-Neumann system is consistent only if the
+        // imbalances sum to zero; remove their mean (no reference counterpart)
+        double s = 0.;
+        for (int i = 0; i < M.ncel; ++i) s += bp[i];
+        s /= M.ncel;
+        for (int i = 0; i < M.ncel; ++i) bp[i] -= s;
+    }
     SolveCfg c = cfg_of(P.pres);
     last_pres = las_solve(Ap, xsol, bp, c);
     if (last_pres.relres > 1.) return err("pressure iteration cannot converge");

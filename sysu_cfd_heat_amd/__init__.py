@@ -348,7 +348,7 @@ class NSSolver:
         self._L.cfd_ns_profile(self.h, int(enable))
 
     def profile_read(self):
-        out = np.zeros(4)
+        out = np.zeros(8)
         self._L.cfd_ns_profile_read(self.h, _dp(out))
         return out
 

@@ -31,6 +31,9 @@ def config_solvers(p, tol=1e-8):
     p.ke = solve_cfg(CFD_BICGSTAB, CFD_PC_JACOBI, 1.0, tol, 500)
     p.energy = solve_cfg(CFD_BICGSTAB, CFD_PC_JACOBI, 1.0, tol, 500)
     p.solid = solve_cfg(CFD_CG, CFD_PC_ILU, 1.0, tol, 1000)
+    # at 1e-8 the pure-Neumann pressure system must be made consistent: its
+    # imbalances are projected to zero mean before each solve
+    p.pres_project = 1
 
 
 def cavity(n=32, re=100.0, tets=False, jitter=0.0, seed=1, max_step=201, solvers="config"):

@@ -115,6 +115,8 @@ void launch_fluid2solid(const DevMesh &fm, const DevMesh &sm, const double *gTf,
 void launch_solid_bc_temperature(const DevMesh &m, const double *T, const double *D,
                                  const double *q, double *BT, hipStream_t s);
 void launch_fill(double *p, size_t n, double v, hipStream_t s);
+// b -= mean(b): consistency projection of a pure-Neumann right-hand side
+void launch_remove_mean(double *b, int n, Red red, hipStream_t s);
 void launch_scale_source(const DevMesh &m, double q, double *src, hipStream_t s);
 
 }  // namespace cfd

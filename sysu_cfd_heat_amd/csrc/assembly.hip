@@ -794,6 +794,17 @@ __global__ void k_solid_bc_temperature(DevMesh m, const double *__restrict__ T,
     }
 }
 
+__global__ void k_sum(int n, const double *__restrict__ b, double *out, double *part,
+                      unsigned *tk) {
+    double acc[1] = {0.};
+    CELL_LOOP(i, n) acc[0] += b[i];
+    if (grid_sum<1>(acc, part, tk)) *out = acc[0] / n;
+}
+__global__ void k_sub_scalar(int n, const double *__restrict__ m, double *__restrict__ b) {
+    const double v = *m;
+    CELL_LOOP(i, n) b[i] -= v;
+}
+
 __global__ void k_scale_source(int C, const double *__restrict__ vol, double q,
                                double *__restrict__ src) {
     CELL_LOOP(i, C) src[i] = q * vol[i];
@@ -974,6 +985,17 @@ void launch_solid_bc_temperature(const DevMesh &m, const double *T, const double
                        D, q, BT);
     CFD_LAUNCH_CHECK();
 }
+void launch_remove_mean(double *b, int n, Red red, hipStream_t s) {
+    // the mean lands in the last reduction slot (part[12*kRedGrid-1] is never
+    // a partial: reductions use at most 12*gridDim.x < 12*kRedGrid slots)
+    double *mean = red.part + 12 * kRedGrid - 1;
+    hipLaunchKernelGGL(k_sum, dim3(std::min(red_grid(n), kRedGrid - 1)), dim3(kBlock), 0, s, n,
+                       (const double *)b, mean, red.part, red.tk);
+    hipLaunchKernelGGL(k_sub_scalar, dim3(grid_for(n)), dim3(kBlock), 0, s, n,
+                       (const double *)mean, b);
+    CFD_LAUNCH_CHECK();
+}
+
 void launch_scale_source(const DevMesh &m, double q, double *src, hipStream_t s) {
     hipLaunchKernelGGL(k_scale_source, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m.ncel,
                        m.c_vol, q, src);

@@ -10,13 +10,16 @@ namespace cfd {
 // Sparse matrix on the device: diagonal separate, K off-diagonal slots stored
 // slot-major (val[k*n + i], col[k*n + i]) so a wave64 reads 64 consecutive
 // rows of one slot in one coalesced access. Padding slots hold col = row,
-// val = 0.
-struct Ell {
+// val = 0. T = double (FP64 path) or float (inner solves of the mixed path).
+template <class T>
+struct EllT {
     int n = 0, K = 0;
-    double *diag = nullptr;
-    double *val = nullptr;
+    T *diag = nullptr;
+    T *val = nullptr;
     const int *col = nullptr;
 };
+using Ell = EllT<double>;
+using EllF = EllT<float>;
 
 // Multicolour ordering of one sparsity structure: rows permuted into colour
 // blocks (greedy first-fit over the symmetrised graph in natural order,
@@ -47,6 +50,14 @@ struct KState {
     int iter, maxit, done, brk, half, j, m, cycle_done;
 };
 
+// Work vectors of one precision.
+template <class T>
+struct KWork {
+    DBuf<T> r, z, p0, p1, q, tmp, lu, ud, pdiag, pval, pb, px;
+    int n = 0, K = 0;
+    void ensure(int n, int K);
+};
+
 class Krylov {
   public:
     explicit Krylov(hipStream_t s = nullptr);
@@ -58,6 +69,11 @@ class Krylov {
     // col must be given for SSOR / ILU.
     SolveStats solve(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
                      Colouring *col);
+    // Mixed precision CG: FP32 matrix copy, FP32 inner CG (+ FP32 Jacobi /
+    // ILU(0) / SSOR) to inner_tol, FP64 residual b - A x and FP64 update of x
+    // until the true FP64 relative residual is below cfg.tol.
+    SolveStats solve_mixed(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
+                           Colouring *col, double inner_tol);
     // y = M^{-1} c with the preconditioner the solver uses (natural order)
     void precond(const Ell &A, int pc, double omega, const double *c, double *y, Colouring *col);
     void spmv(const Ell &A, const double *x, double *y);
@@ -72,29 +88,50 @@ class Krylov {
     bool prof = false;
     double prof_ms = 0.;
     long long prof_launches = 0;
+    double prof_bytes_scale = 1.0;  // 0.5 while the FP32 inner solve runs
 
   private:
     hipStream_t st_;
-    DBuf<double> r_, z_, p0_, p1_, q_, rt_, v_, s_, t_, ph_, sh_, w0_, w1_, V_, part_;
-    DBuf<double> pdiag_, pval_, pb_, px_, lu_, ud_, tmp_;
+    KWork<double> wd_;
+    KWork<float> wf_;
+    DBuf<double> rt_, v_, s_, t_, ph_, sh_, w0_, w1_, V_, part_;
     DBuf<unsigned> ticket_;
     DBuf<KState> state_;
     KState *hstate_ = nullptr;
-    int nalloc_ = 0, kalloc_ = 0;
     std::vector<hipEvent_t> evs_;
+    double prof_ms_f32_ = 0.;
+    long long prof_launches_f32_ = 0;
     hipEvent_t ev(size_t k);
     int next_batch(int batch, int &prev_it, double &prev_log, int maxit);
     void ensure(int n, int K);
     void read_state();
-    void setup_pc(const Ell &A, int pc, Colouring *col);
-    void apply_pc(const Ell &A, int pc, double omega, const double *c, double *y,
-                  Colouring *col);
-    SolveStats cg(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
-                  Colouring *col, int pc);
+    template <class T>
+    void setup_pc(const EllT<T> &A, int pc, Colouring *col, KWork<T> &w);
+    template <class T>
+    void apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, Colouring *col,
+                  KWork<T> &w);
+    template <class T>
+    SolveStats cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &cfg, Colouring *col,
+                  int pc, KWork<T> &w);
     SolveStats bicgstab(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
                         Colouring *col, int pc);
     SolveStats gmres(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
                      Colouring *col, int pc);
+    DBuf<double> mrhs_;   // multi-RHS BiCGSTAB work vectors
+    DBuf<KState> mstate_;
+    KState *hmstate_ = nullptr;
+
+  public:
+    // Multi-RHS BiCGSTAB + Jacobi: nb <= 3 systems A x_k = b_k that share A
+    // (momentum u, v, w); each SpMV reads the matrix once for all systems,
+    // each system keeps its own recurrence scalars and stopping test.
+    void solve_multi(const Ell &A, int nb, const double *const *b, double *const *x,
+                     const cfd_solve_cfg &cfg, SolveStats *out);
+
+    // per-precision live timing (FP32 inner launches are counted separately)
+    double prof_ms_f32() const { return prof_ms_f32_; }
+    long long prof_launches_f32() const { return prof_launches_f32_; }
+    void prof_reset() { prof_ms = prof_ms_f32_ = 0.; prof_launches = prof_launches_f32_ = 0; }
 };
 
 }  // namespace cfd

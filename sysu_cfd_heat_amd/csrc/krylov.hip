@@ -1,5 +1,6 @@
 // Device Krylov solvers: CG, BiCGSTAB, restarted GMRES(10) with Jacobi,
-// multicolour SSOR and multicolour ILU(0) preconditioners, FP64.
+// multicolour SSOR and multicolour ILU(0) preconditioners, FP64; and a
+// mixed-precision CG (FP32 inner solves, FP64 iterative refinement).
 //
 // The algorithms are those the oracle restates from LASPack / Templates
 // (src/tools.cpp:351-359 is the call boundary): same recurrences, same
@@ -10,8 +11,8 @@
 // so results are run-to-run reproducible. Scalars (alpha, beta, ...) never
 // leave the device: the last workgroup of each reducing kernel updates a
 // KState record that the following kernels read. The host reads KState only
-// every few iterations to decide when to stop; kernels launched past
-// convergence return immediately.
+// when a convergence estimate says the solve may be done; kernels launched
+// past convergence return immediately.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -27,9 +28,10 @@ __device__ __forceinline__ bool rtc(double rn, double bn, double eps) {
     return bn > 0. ? rn < eps * bn : rn < eps;
 }
 
-__device__ __forceinline__ double row_dot(const Ell &A, int i, const double *__restrict__ x) {
+template <class T>
+__device__ __forceinline__ T row_dot(const EllT<T> &A, int i, const T *__restrict__ x) {
     const int n = A.n;
-    double s = A.diag[i] * x[i];
+    T s = A.diag[i] * x[i];
     for (int k = 0; k < A.K; ++k) {
         size_t o = (size_t)k * n + i;
         s += A.val[o] * x[A.col[o]];
@@ -40,28 +42,30 @@ __device__ __forceinline__ double row_dot(const Ell &A, int i, const double *__r
 #define GRID_LOOP(i, n) \
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 
-__global__ void k_spmv(Ell A, const double *__restrict__ x, double *__restrict__ y) {
+template <class T>
+__global__ void k_spmv(EllT<T> A, const T *__restrict__ x, T *__restrict__ y) {
     GRID_LOOP(i, A.n) y[i] = row_dot(A, i, x);
 }
 
 // r = b - A x ; partials ||r||^2, ||b||^2, r.(r/d) (Jacobi rho)
-__global__ void k_residual(Ell A, const double *__restrict__ b, const double *__restrict__ x,
-                           double *__restrict__ r, double *__restrict__ zero1,
-                           double *__restrict__ zero2, KState *st, double *part, unsigned *tk,
-                           int mode, double eps, int maxit) {
+template <class T>
+__global__ void k_residual(EllT<T> A, const T *__restrict__ b, const T *__restrict__ x,
+                           T *__restrict__ r, T *__restrict__ zero1, T *__restrict__ zero2,
+                           KState *st, double *part, unsigned *tk, int mode, double eps,
+                           int maxit) {
     double acc[3] = {0., 0., 0.};
     GRID_LOOP(i, A.n) {
-        double ri = b[i] - row_dot(A, i, x);
+        T ri = b[i] - row_dot(A, i, x);
         r[i] = ri;
-        if (zero1) zero1[i] = 0.;
-        if (zero2) zero2[i] = 0.;
-        acc[0] += ri * ri;
-        acc[1] += b[i] * b[i];
-        acc[2] += ri * (ri / A.diag[i]);
+        if (zero1) zero1[i] = T(0);
+        if (zero2) zero2[i] = T(0);
+        acc[0] += (double)ri * ri;
+        acc[1] += (double)b[i] * b[i];
+        acc[2] += (double)ri * (ri / A.diag[i]);
     }
     if (grid_sum<3>(acc, part, tk)) {
-        // mode 0: plain residual (no state change); 1: CG/BiCGSTAB start;
-        // 2: GMRES (re)start
+        // mode 0: plain residual (no state change); 1: CG start;
+        // 3: BiCGSTAB start; 2: GMRES (re)start
         if (mode == 0) {
             st->rr = acc[0];
             st->bb = acc[1];
@@ -84,7 +88,7 @@ __global__ void k_residual(Ell A, const double *__restrict__ b, const double *__
             st->beta = 0.;
             st->done = rtc(st->rnorm, st->bnorm, eps) || maxit <= 0;
             if (mode == 3 && !st->done && st->rho == 0.) { st->done = 1; st->brk = 1; }
-        } else {  // GMRES: r goes to W; beta = ||r||
+        } else {
             if (maxit >= 0) {  // first start
                 st->eps = eps;
                 st->maxit = maxit;
@@ -109,37 +113,54 @@ __global__ void k_copy(int n, const double *__restrict__ a, double *__restrict__
     GRID_LOOP(i, n) b[i] = a[i];
 }
 
-__global__ void k_scale(int n, double f, double *__restrict__ y) { GRID_LOOP(i, n) y[i] = f * y[i]; }
+template <class T>
+__global__ void k_scale(int n, double f, T *__restrict__ y) { GRID_LOOP(i, n) y[i] = T(f) * y[i]; }
 
-__global__ void k_jacobi(int n, const double *__restrict__ d, const double *__restrict__ c,
-                         double *__restrict__ y) {
+template <class T>
+__global__ void k_jacobi(int n, const T *__restrict__ d, const T *__restrict__ c,
+                         T *__restrict__ y) {
     GRID_LOOP(i, n) y[i] = c[i] / d[i];
+}
+
+__global__ void k_to_float(int n, const double *__restrict__ a, float *__restrict__ b) {
+    GRID_LOOP(i, n) b[i] = (float)a[i];
+}
+__global__ void k_matrix_to_float(int n, int K, const double *__restrict__ d,
+                                  const double *__restrict__ v, float *__restrict__ fd,
+                                  float *__restrict__ fv) {
+    GRID_LOOP(i, n) {
+        fd[i] = (float)d[i];
+        for (int k = 0; k < K; ++k) fv[(size_t)k * n + i] = (float)v[(size_t)k * n + i];
+    }
+}
+__global__ void k_add_f(int n, double *__restrict__ x, const float *__restrict__ e) {
+    GRID_LOOP(i, n) x[i] += (double)e[i];
 }
 
 // ------------------------------------------------------------------- CG --
 // p_new = z + beta p_old ; q = A p_new ; partial p.q ; alpha = rho / p.q.
 // Jacobi: z = r / d is formed on the fly for the row and its neighbours.
-template <bool JAC>
-__global__ void k_cg_pq(Ell A, const double *__restrict__ r, const double *__restrict__ z,
-                        const double *__restrict__ pold, double *__restrict__ pnew,
-                        double *__restrict__ q, KState *st, double *part, unsigned *tk) {
+template <bool JAC, class T>
+__global__ void k_cg_pq(EllT<T> A, const T *__restrict__ r, const T *__restrict__ z,
+                        const T *__restrict__ pold, T *__restrict__ pnew, T *__restrict__ q,
+                        KState *st, double *part, unsigned *tk) {
     if (st->done) return;
-    const double beta = st->beta;
+    const T beta = (T)st->beta;
     const int n = A.n;
     double acc[1] = {0.};
     GRID_LOOP(i, n) {
-        double zi = JAC ? r[i] / A.diag[i] : z[i];
-        double pi = zi + beta * pold[i];
-        double s = A.diag[i] * pi;
+        T zi = JAC ? r[i] / A.diag[i] : z[i];
+        T pi = zi + beta * pold[i];
+        T s = A.diag[i] * pi;
         for (int k = 0; k < A.K; ++k) {
             size_t o = (size_t)k * n + i;
             int c = A.col[o];
-            double zc = JAC ? r[c] / A.diag[c] : z[c];
+            T zc = JAC ? r[c] / A.diag[c] : z[c];
             s += A.val[o] * (zc + beta * pold[c]);
         }
         pnew[i] = pi;
         q[i] = s;
-        acc[0] += pi * s;
+        acc[0] += (double)pi * s;
     }
     if (grid_sum<1>(acc, part, tk)) {
         st->pq = acc[0];
@@ -148,20 +169,19 @@ __global__ void k_cg_pq(Ell A, const double *__restrict__ r, const double *__res
 }
 
 // x += alpha p ; r -= alpha q ; partial ||r||^2 (and Jacobi rho = r.(r/d))
-template <bool JAC>
-__global__ void k_cg_update(int n, double *__restrict__ x, double *__restrict__ r,
-                            const double *__restrict__ p, const double *__restrict__ q,
-                            const double *__restrict__ d, KState *st, double *part,
-                            unsigned *tk) {
+template <bool JAC, class T>
+__global__ void k_cg_update(int n, T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
+                            const T *__restrict__ q, const T *__restrict__ d, KState *st,
+                            double *part, unsigned *tk) {
     if (st->done) return;
-    const double alpha = st->alpha;
+    const T alpha = (T)st->alpha;
     double acc[2] = {0., 0.};
     GRID_LOOP(i, n) {
         x[i] += alpha * p[i];
-        double ri = r[i] - alpha * q[i];
+        T ri = r[i] - alpha * q[i];
         r[i] = ri;
-        acc[0] += ri * ri;
-        if (JAC) acc[1] += ri * (ri / d[i]);
+        acc[0] += (double)ri * ri;
+        if (JAC) acc[1] += (double)ri * (ri / d[i]);
     }
     if (grid_sum<2>(acc, part, tk)) {
         st->iter += 1;
@@ -177,11 +197,12 @@ __global__ void k_cg_update(int n, double *__restrict__ x, double *__restrict__ 
 }
 
 // rho = r.z ; beta = rho / rho_old (0 on the first iteration)
-__global__ void k_cg_rho(int n, const double *__restrict__ r, const double *__restrict__ z,
-                         KState *st, double *part, unsigned *tk) {
+template <class T>
+__global__ void k_cg_rho(int n, const T *__restrict__ r, const T *__restrict__ z, KState *st,
+                         double *part, unsigned *tk) {
     if (st->done) return;
     double acc[1] = {0.};
-    GRID_LOOP(i, n) acc[0] += r[i] * z[i];
+    GRID_LOOP(i, n) acc[0] += (double)r[i] * z[i];
     if (grid_sum<1>(acc, part, tk)) {
         st->rho = acc[0];
         st->beta = st->iter == 0 ? 0. : acc[0] / st->rho_old;
@@ -204,7 +225,7 @@ __global__ void k_bi_p(int n, const double *__restrict__ r, double *__restrict__
     }
 }
 
-// y = A x ; partial rt.y ; mode 0: alpha = rho / rt.v ; mode 1: (t.s, t.t) -> omega
+// y = A x ; mode 0: partial rt.v -> alpha ; mode 1: (t.s, t.t) -> omega
 template <int MODE>
 __global__ void k_bi_spmv(Ell A, const double *__restrict__ x, double *__restrict__ y,
                           const double *__restrict__ other, KState *st, double *part,
@@ -339,7 +360,9 @@ __global__ void k_gm_sub(int n, double *__restrict__ w, const double *__restrict
     for (int t = 0; t < 12; ++t) acc[t] = 0.;
     GRID_LOOP(i, n) {
         double wi = w[i];
-        for (int t = 0; t <= j; ++t) wi -= c[t] * V[(size_t)t * n + i];
+#pragma unroll
+        for (int t = 0; t < 12; ++t)
+            if (t <= j) wi -= c[t] * V[(size_t)t * n + i];
         w[i] = wi;
         if (PASS == 1) {
 #pragma unroll
@@ -418,12 +441,13 @@ __global__ void k_add(int n, double *__restrict__ x, const double *__restrict__ 
 // ----------------------------------------------- multicolour sweeps --
 // Rows [r0, r1) form one colour block of the permuted matrix; a neighbour
 // is "lower" iff its permuted index is below the row's.
-__global__ void k_ssor_fwd(Ell A, double omega, int r0, int r1, const double *__restrict__ c,
-                           double *__restrict__ t) {
+template <class T>
+__global__ void k_ssor_fwd(EllT<T> A, T omega, int r0, int r1, const T *__restrict__ c,
+                           T *__restrict__ t) {
     const int n = A.n;
     int p = r0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= r1) return;
-    double acc = c[p];
+    T acc = c[p];
     for (int k = 0; k < A.K; ++k) {
         size_t o = (size_t)k * n + p;
         int cl = A.col[o];
@@ -431,12 +455,13 @@ __global__ void k_ssor_fwd(Ell A, double omega, int r0, int r1, const double *__
     }
     t[p] = acc / A.diag[p];
 }
-__global__ void k_ssor_bwd(Ell A, double omega, int r0, int r1, const double *__restrict__ t,
-                           double *__restrict__ y) {
+template <class T>
+__global__ void k_ssor_bwd(EllT<T> A, T omega, int r0, int r1, const T *__restrict__ t,
+                           T *__restrict__ y) {
     const int n = A.n;
     int p = r0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= r1) return;
-    double acc = A.diag[p] * t[p];
+    T acc = A.diag[p] * t[p];
     for (int k = 0; k < A.K; ++k) {
         size_t o = (size_t)k * n + p;
         int cl = A.col[o];
@@ -449,8 +474,9 @@ constexpr int kIluMaxK = 16;
 
 // ILU(0) of one colour block, IKJ order: earlier neighbours in ascending
 // permuted index; lu/ud start as copies of the off-diagonals / diagonal.
+template <class T>
 __global__ void k_ilu_factor(int n, int K, const int *__restrict__ col, int r0, int r1,
-                             double *__restrict__ lu, double *__restrict__ ud) {
+                             T *__restrict__ lu, T *__restrict__ ud) {
     int p = r0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= r1) return;
     int lc[kIluMaxK], ls[kIluMaxK], nl = 0;
@@ -470,13 +496,13 @@ __global__ void k_ilu_factor(int n, int K, const int *__restrict__ col, int r0, 
     for (int a = 0; a < nl; ++a) {
         int c = lc[a];
         size_t o = (size_t)ls[a] * n + p;
-        double l = lu[o] / ud[c];
+        T l = lu[o] / ud[c];
         lu[o] = l;
         for (int q = 0; q < K; ++q) {
             size_t oc = (size_t)q * n + c;
             int j = col[oc];
             if (j <= c) continue;
-            double lv = lu[oc];
+            T lv = lu[oc];
             if (j == p) {
                 ud[p] -= l * lv;
                 continue;
@@ -491,11 +517,12 @@ __global__ void k_ilu_factor(int n, int K, const int *__restrict__ col, int r0, 
         }
     }
 }
-__global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col, const double *__restrict__ lu,
-                          int r0, int r1, const double *__restrict__ c, double *__restrict__ t) {
+template <class T>
+__global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col, const T *__restrict__ lu,
+                          int r0, int r1, const T *__restrict__ c, T *__restrict__ t) {
     int p = r0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= r1) return;
-    double acc = c[p];
+    T acc = c[p];
     for (int k = 0; k < K; ++k) {
         size_t o = (size_t)k * n + p;
         int cl = col[o];
@@ -503,12 +530,13 @@ __global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col, const doubl
     }
     t[p] = acc;
 }
-__global__ void k_ilu_bwd(int n, int K, const int *__restrict__ col, const double *__restrict__ lu,
-                          const double *__restrict__ ud, int r0, int r1,
-                          const double *__restrict__ t, double *__restrict__ y) {
+template <class T>
+__global__ void k_ilu_bwd(int n, int K, const int *__restrict__ col, const T *__restrict__ lu,
+                          const T *__restrict__ ud, int r0, int r1, const T *__restrict__ t,
+                          T *__restrict__ y) {
     int p = r0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= r1) return;
-    double acc = t[p];
+    T acc = t[p];
     for (int k = 0; k < K; ++k) {
         size_t o = (size_t)k * n + p;
         int cl = col[o];
@@ -594,10 +622,21 @@ void Colouring::build(const int *hc, int n_, int K_, const int *given, const int
     perm.upload(hp.data(), n, s);
     iperm.upload(hip_.data(), n, s);
     pcol.upload(hpc.data(), (size_t)K * n, s);
-    CFD_HIP(hipStreamSynchronize(s));
 }
 
 // --------------------------------------------------------------- Krylov --
+template <class T>
+void KWork<T>::ensure(int n_, int K_) {
+    if (n_ <= n && K_ <= K) return;
+    n = std::max(n, n_);
+    K = std::max(K, K_);
+    for (auto *b : {&r, &z, &p0, &p1, &q, &tmp, &ud, &pdiag, &pb, &px}) b->alloc(n);
+    lu.alloc((size_t)K * n);
+    pval.alloc((size_t)K * n);
+}
+template struct KWork<double>;
+template struct KWork<float>;
+
 Krylov::Krylov(hipStream_t s) : st_(s) {
     ticket_.alloc(1);
     ticket_.zero(st_);
@@ -641,16 +680,11 @@ int Krylov::next_batch(int batch, int &prev_it, double &prev_log, int maxit) {
 }
 
 void Krylov::ensure(int n, int K) {
-    if (n <= nalloc_ && K <= kalloc_) return;
-    nalloc_ = std::max(n, nalloc_);
-    kalloc_ = std::max(K, kalloc_);
-    size_t N = nalloc_;
-    for (auto *b : {&r_, &z_, &p0_, &p1_, &q_, &rt_, &v_, &s_, &t_, &ph_, &sh_, &w0_, &w1_,
-                    &pdiag_, &pb_, &px_, &ud_, &tmp_})
-        b->alloc(N);
-    V_.alloc(11 * N);
-    pval_.alloc((size_t)kalloc_ * N);
-    lu_.alloc((size_t)kalloc_ * N);
+    wd_.ensure(n, K);
+    if ((int)rt_.n < n) {
+        for (auto *b : {&rt_, &v_, &s_, &t_, &ph_, &sh_, &w0_, &w1_}) b->alloc(n);
+        V_.alloc(11 * (size_t)n);
+    }
 }
 
 void Krylov::sync() { CFD_HIP(hipStreamSynchronize(st_)); }
@@ -661,72 +695,76 @@ void Krylov::read_state() {
 }
 
 void Krylov::spmv(const Ell &A, const double *x, double *y) {
-    hipLaunchKernelGGL(k_spmv, dim3(grid_for(A.n)), dim3(kBlock), 0, st_, A, x, y);
+    hipLaunchKernelGGL(k_spmv<double>, dim3(grid_for(A.n)), dim3(kBlock), 0, st_, A, x, y);
     CFD_LAUNCH_CHECK();
 }
 
 double Krylov::relres(const Ell &A, const double *x, const double *b) {
     ensure(A.n, A.K);
-    hipLaunchKernelGGL(k_residual, dim3(red_grid(A.n)), dim3(kBlock), 0, st_, A, b, x, tmp_.p,
-                       (double *)nullptr, (double *)nullptr, state_.p, part_.p, ticket_.p, 0, 0.,
-                       0);
+    hipLaunchKernelGGL(k_residual<double>, dim3(red_grid(A.n)), dim3(kBlock), 0, st_, A, b, x,
+                       wd_.tmp.p, (double *)nullptr, (double *)nullptr, state_.p, part_.p,
+                       ticket_.p, 0, 0., 0);
     CFD_LAUNCH_CHECK();
     read_state();
     double bn = std::sqrt(hstate_->bb), rn = std::sqrt(hstate_->rr);
     return bn > 0 ? rn / bn : rn;
 }
 
-void Krylov::setup_pc(const Ell &A, int pc, Colouring *col) {
+template <class T>
+void Krylov::setup_pc(const EllT<T> &A, int pc, Colouring *col, KWork<T> &w) {
     if (pc != CFD_PC_ILU) return;
     const int n = A.n;
     if (A.K > kIluMaxK) throw HipError("ILU(0): more than 16 off-diagonals per row");
-    CFD_HIP(hipMemcpyAsync(lu_.p, A.val, sizeof(double) * (size_t)A.K * n,
-                           hipMemcpyDeviceToDevice, st_));
-    CFD_HIP(hipMemcpyAsync(ud_.p, A.diag, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
+    CFD_HIP(hipMemcpyAsync(w.lu.p, A.val, sizeof(T) * (size_t)A.K * n, hipMemcpyDeviceToDevice,
+                           st_));
+    CFD_HIP(hipMemcpyAsync(w.ud.p, A.diag, sizeof(T) * n, hipMemcpyDeviceToDevice, st_));
     for (int c = 0; c < col->ncolour; ++c) {
         int r0 = col->off[c], r1 = col->off[c + 1];
         if (r1 > r0)
-            hipLaunchKernelGGL(k_ilu_factor, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_, n,
-                               A.K, A.col, r0, r1, lu_.p, ud_.p);
+            hipLaunchKernelGGL(k_ilu_factor<T>, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_,
+                               n, A.K, A.col, r0, r1, w.lu.p, w.ud.p);
     }
     CFD_LAUNCH_CHECK();
 }
 
 // A is the permuted matrix for SSOR / ILU
-void Krylov::apply_pc(const Ell &A, int pc, double omega, const double *c, double *y,
-                      Colouring *col) {
+template <class T>
+void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, Colouring *col,
+                      KWork<T> &w) {
     const int n = A.n;
     if (pc == CFD_PC_NONE) {
-        CFD_HIP(hipMemcpyAsync(y, c, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
+        CFD_HIP(hipMemcpyAsync(y, c, sizeof(T) * n, hipMemcpyDeviceToDevice, st_));
     } else if (pc == CFD_PC_JACOBI) {
-        hipLaunchKernelGGL(k_jacobi, dim3(grid_for(n)), dim3(kBlock), 0, st_, n, A.diag, c, y);
+        hipLaunchKernelGGL(k_jacobi<T>, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
+                           (const T *)A.diag, c, y);
     } else if (pc == CFD_PC_SSOR) {
         for (int k = 0; k < col->ncolour; ++k) {
             int r0 = col->off[k], r1 = col->off[k + 1];
             if (r1 > r0)
-                hipLaunchKernelGGL(k_ssor_fwd, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_, A,
-                                   omega, r0, r1, c, tmp_.p);
+                hipLaunchKernelGGL(k_ssor_fwd<T>, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_,
+                                   A, (T)omega, r0, r1, c, w.tmp.p);
         }
         for (int k = col->ncolour - 1; k >= 0; --k) {
             int r0 = col->off[k], r1 = col->off[k + 1];
             if (r1 > r0)
-                hipLaunchKernelGGL(k_ssor_bwd, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_, A,
-                                   omega, r0, r1, tmp_.p, y);
+                hipLaunchKernelGGL(k_ssor_bwd<T>, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_,
+                                   A, (T)omega, r0, r1, (const T *)w.tmp.p, y);
         }
-        hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
+        hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
                            omega * (2. - omega), y);
     } else {
         for (int k = 0; k < col->ncolour; ++k) {
             int r0 = col->off[k], r1 = col->off[k + 1];
             if (r1 > r0)
-                hipLaunchKernelGGL(k_ilu_fwd, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_, n,
-                                   A.K, A.col, lu_.p, r0, r1, c, tmp_.p);
+                hipLaunchKernelGGL(k_ilu_fwd<T>, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_, n,
+                                   A.K, A.col, (const T *)w.lu.p, r0, r1, c, w.tmp.p);
         }
         for (int k = col->ncolour - 1; k >= 0; --k) {
             int r0 = col->off[k], r1 = col->off[k + 1];
             if (r1 > r0)
-                hipLaunchKernelGGL(k_ilu_bwd, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_, n,
-                                   A.K, A.col, lu_.p, ud_.p, r0, r1, tmp_.p, y);
+                hipLaunchKernelGGL(k_ilu_bwd<T>, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_, n,
+                                   A.K, A.col, (const T *)w.lu.p, (const T *)w.ud.p, r0, r1,
+                                   (const T *)w.tmp.p, y);
         }
     }
     CFD_LAUNCH_CHECK();
@@ -739,17 +777,17 @@ void Krylov::precond(const Ell &A, int pc, double omega, const double *c, double
     if (pc == CFD_PC_SSOR || pc == CFD_PC_ILU) {
         if (!col) throw HipError("multicolour preconditioner needs a colouring");
         hipLaunchKernelGGL(k_perm_matrix, dim3(grid_for(n)), dim3(kBlock), 0, st_, n, A.K,
-                           col->perm.p, A.diag, A.val, pdiag_.p, pval_.p);
-        Ell P{n, A.K, pdiag_.p, pval_.p, col->pcol.p};
-        setup_pc(P, pc, col);
+                           col->perm.p, A.diag, A.val, wd_.pdiag.p, wd_.pval.p);
+        Ell P{n, A.K, wd_.pdiag.p, wd_.pval.p, col->pcol.p};
+        setup_pc(P, pc, col, wd_);
         hipLaunchKernelGGL(k_perm_gather, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
-                           col->perm.p, c, pb_.p);
-        apply_pc(P, pc, omega, pb_.p, px_.p, col);
+                           col->perm.p, c, wd_.pb.p);
+        apply_pc(P, pc, omega, (const double *)wd_.pb.p, wd_.px.p, col, wd_);
         hipLaunchKernelGGL(k_perm_scatter, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
-                           col->perm.p, px_.p, y);
+                           col->perm.p, (const double *)wd_.px.p, y);
         CFD_LAUNCH_CHECK();
     } else {
-        apply_pc(A, pc, omega, c, y, col);
+        apply_pc(A, pc, omega, c, y, col, wd_);
     }
     sync();
 }
@@ -766,71 +804,143 @@ SolveStats Krylov::solve(const Ell &A0, const double *b0, double *x0, const cfd_
     if (perm) {
         if (!col || col->n != n) throw HipError("multicolour preconditioner needs a colouring");
         hipLaunchKernelGGL(k_perm_matrix, dim3(grid_for(n)), dim3(kBlock), 0, st_, n, A0.K,
-                           col->perm.p, A0.diag, A0.val, pdiag_.p, pval_.p);
+                           col->perm.p, A0.diag, A0.val, wd_.pdiag.p, wd_.pval.p);
         hipLaunchKernelGGL(k_perm_gather, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
-                           col->perm.p, b0, pb_.p);
+                           col->perm.p, b0, wd_.pb.p);
         hipLaunchKernelGGL(k_perm_gather, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
-                           col->perm.p, x0, px_.p);
+                           col->perm.p, (const double *)x0, wd_.px.p);
         CFD_LAUNCH_CHECK();
-        A = Ell{n, A0.K, pdiag_.p, pval_.p, col->pcol.p};
-        b = pb_.p;
-        x = px_.p;
-        setup_pc(A, pc, col);
+        A = Ell{n, A0.K, wd_.pdiag.p, wd_.pval.p, col->pcol.p};
+        b = wd_.pb.p;
+        x = wd_.px.p;
+        setup_pc(A, pc, col, wd_);
     }
     SolveStats S;
-    if (cfg.method == CFD_CG) S = cg(A, b, x, cfg, col, pc);
+    if (cfg.method == CFD_CG) S = cg(A, b, x, cfg, col, pc, wd_);
     else if (cfg.method == CFD_BICGSTAB) S = bicgstab(A, b, x, cfg, col, pc);
     else S = gmres(A, b, x, cfg, col, pc);
     S.relres = relres(A, x, b);
     if (perm) {
         hipLaunchKernelGGL(k_perm_scatter, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
-                           col->perm.p, px_.p, x0);
+                           col->perm.p, (const double *)wd_.px.p, x0);
         CFD_LAUNCH_CHECK();
         sync();
     }
     return S;
 }
 
-SolveStats Krylov::cg(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
-                      Colouring *col, int pc) {
-    const int n = A.n, G = red_grid(n), g = grid_for(n);
+SolveStats Krylov::solve_mixed(const Ell &A0, const double *b0, double *x0,
+                               const cfd_solve_cfg &cfg, Colouring *col, double inner_tol) {
+    if (cfg.method != CFD_CG) throw HipError("mixed precision is implemented for CG");
+    const int n = A0.n;
+    ensure(n, A0.K);
+    wf_.ensure(n, A0.K);
+    int pc = cfg.precond;
+    Ell A = A0;
+    const double *b = b0;
+    double *x = x0;
+    bool perm = (pc == CFD_PC_SSOR || pc == CFD_PC_ILU);
+    if (perm) {
+        if (!col || col->n != n) throw HipError("multicolour preconditioner needs a colouring");
+        hipLaunchKernelGGL(k_perm_matrix, dim3(grid_for(n)), dim3(kBlock), 0, st_, n, A0.K,
+                           col->perm.p, A0.diag, A0.val, wd_.pdiag.p, wd_.pval.p);
+        hipLaunchKernelGGL(k_perm_gather, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
+                           col->perm.p, b0, wd_.pb.p);
+        hipLaunchKernelGGL(k_perm_gather, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
+                           col->perm.p, (const double *)x0, wd_.px.p);
+        A = Ell{n, A0.K, wd_.pdiag.p, wd_.pval.p, col->pcol.p};
+        b = wd_.pb.p;
+        x = wd_.px.p;
+    }
+    // FP32 copy of the (permuted) matrix and its preconditioner
+    hipLaunchKernelGGL(k_matrix_to_float, dim3(grid_for(n)), dim3(kBlock), 0, st_, n, A.K,
+                       (const double *)A.diag, (const double *)A.val, wf_.pdiag.p, wf_.pval.p);
+    CFD_LAUNCH_CHECK();
+    EllF Af{n, A.K, wf_.pdiag.p, wf_.pval.p, A.col};
+    setup_pc(Af, pc, col, wf_);
+    SolveStats S;
+    int total = 0;
+    double rel = 0.;
+    for (int outer = 0; outer < 100; ++outer) {
+        // FP64 residual r = b - A x
+        hipLaunchKernelGGL(k_residual<double>, dim3(red_grid(n)), dim3(kBlock), 0, st_, A, b,
+                           (const double *)x, wd_.r.p, (double *)nullptr, (double *)nullptr,
+                           state_.p, part_.p, ticket_.p, 0, 0., 0);
+        CFD_LAUNCH_CHECK();
+        read_state();
+        double bn = std::sqrt(hstate_->bb), rn = std::sqrt(hstate_->rr);
+        rel = bn > 0 ? rn / bn : rn;
+        if ((bn > 0 ? rn < cfg.tol * bn : rn < cfg.tol) || total >= cfg.maxit) break;
+        // FP32 correction: A e = r to inner_tol relative to ||r||
+        hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
+                           (const double *)wd_.r.p, wf_.pb.p);
+        CFD_HIP(hipMemsetAsync(wf_.px.p, 0, sizeof(float) * n, st_));
+        cfd_solve_cfg ic = cfg;
+        ic.tol = std::max(inner_tol, 0.5 * cfg.tol * bn / rn);
+        ic.maxit = cfg.maxit - total;
+        SolveStats si = cg(Af, (const float *)wf_.pb.p, wf_.px.p, ic, col, pc, wf_);
+        total += si.iters;
+        hipLaunchKernelGGL(k_add_f, dim3(grid_for(n)), dim3(kBlock), 0, st_, n, x,
+                           (const float *)wf_.px.p);
+        S.refinements++;
+        if (si.iters == 0) break;
+    }
+    S.iters = total;
+    S.relres = rel;
+    S.rtcres = rel;
+    if (perm) {
+        hipLaunchKernelGGL(k_perm_scatter, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
+                           col->perm.p, (const double *)wd_.px.p, x0);
+        CFD_LAUNCH_CHECK();
+    }
+    sync();
+    return S;
+}
+
+template <class T>
+SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &cfg,
+                      Colouring *col, int pc, KWork<T> &w) {
+    const int n = A.n, G = red_grid(n);
     const bool jac = pc == CFD_PC_JACOBI;
-    hipLaunchKernelGGL(k_residual, dim3(G), dim3(kBlock), 0, st_, A, b, (const double *)x, r_.p,
-                       p0_.p, p1_.p, state_.p, part_.p, ticket_.p, 1, cfg.tol, cfg.maxit);
+    hipLaunchKernelGGL(k_residual<T>, dim3(G), dim3(kBlock), 0, st_, A, b, (const T *)x, w.r.p,
+                       w.p0.p, w.p1.p, state_.p, part_.p, ticket_.p, 1, cfg.tol, cfg.maxit);
     if (!jac) {
-        apply_pc(A, pc, cfg.omega, r_.p, z_.p, col);
-        hipLaunchKernelGGL(k_cg_rho, dim3(G), dim3(kBlock), 0, st_, n, r_.p, z_.p, state_.p,
-                           part_.p, ticket_.p);
+        apply_pc(A, pc, cfg.omega, (const T *)w.r.p, w.z.p, col, w);
+        hipLaunchKernelGGL(k_cg_rho<T>, dim3(G), dim3(kBlock), 0, st_, n, (const T *)w.r.p,
+                           (const T *)w.z.p, state_.p, part_.p, ticket_.p);
     }
     CFD_LAUNCH_CHECK();
     int batch = check_first, issued = 0, prev_it = 0;
     double prev_log = 1e300;
-    double *pa = p0_.p, *pb = p1_.p;
-    (void)g;
+    T *pa = w.p0.p, *pb = w.p1.p;
     size_t nev = 0;
     for (;;) {
         for (int k = 0; k < batch; ++k) {
             if (prof) CFD_HIP(hipEventRecord(ev(2 * nev), st_));
             if (jac)
-                hipLaunchKernelGGL(k_cg_pq<true>, dim3(G), dim3(kBlock), 0, st_, A, r_.p,
-                                   (const double *)nullptr, pa, pb, q_.p, state_.p, part_.p,
-                                   ticket_.p);
+                hipLaunchKernelGGL((k_cg_pq<true, T>), dim3(G), dim3(kBlock), 0, st_, A,
+                                   (const T *)w.r.p, (const T *)nullptr, (const T *)pa, pb, w.q.p,
+                                   state_.p, part_.p, ticket_.p);
             else
-                hipLaunchKernelGGL(k_cg_pq<false>, dim3(G), dim3(kBlock), 0, st_, A, r_.p, z_.p,
-                                   pa, pb, q_.p, state_.p, part_.p, ticket_.p);
+                hipLaunchKernelGGL((k_cg_pq<false, T>), dim3(G), dim3(kBlock), 0, st_, A,
+                                   (const T *)w.r.p, (const T *)w.z.p, (const T *)pa, pb, w.q.p,
+                                   state_.p, part_.p, ticket_.p);
             if (prof) {
                 CFD_HIP(hipEventRecord(ev(2 * nev + 1), st_));
                 ++nev;
             }
             if (jac) {
-                hipLaunchKernelGGL(k_cg_update<true>, dim3(G), dim3(kBlock), 0, st_, n, x, r_.p,
-                                   pb, q_.p, A.diag, state_.p, part_.p, ticket_.p);
-            } else {
-                hipLaunchKernelGGL(k_cg_update<false>, dim3(G), dim3(kBlock), 0, st_, n, x,
-                                   r_.p, pb, q_.p, A.diag, state_.p, part_.p, ticket_.p);
-                apply_pc(A, pc, cfg.omega, r_.p, z_.p, col);
-                hipLaunchKernelGGL(k_cg_rho, dim3(G), dim3(kBlock), 0, st_, n, r_.p, z_.p,
+                hipLaunchKernelGGL((k_cg_update<true, T>), dim3(G), dim3(kBlock), 0, st_, n, x,
+                                   w.r.p, (const T *)pb, (const T *)w.q.p, (const T *)A.diag,
                                    state_.p, part_.p, ticket_.p);
+            } else {
+                hipLaunchKernelGGL((k_cg_update<false, T>), dim3(G), dim3(kBlock), 0, st_, n, x,
+                                   w.r.p, (const T *)pb, (const T *)w.q.p, (const T *)A.diag,
+                                   state_.p, part_.p, ticket_.p);
+                apply_pc(A, pc, cfg.omega, (const T *)w.r.p, w.z.p, col, w);
+                hipLaunchKernelGGL(k_cg_rho<T>, dim3(G), dim3(kBlock), 0, st_, n,
+                                   (const T *)w.r.p, (const T *)w.z.p, state_.p, part_.p,
+                                   ticket_.p);
             }
             std::swap(pa, pb);
         }
@@ -842,12 +952,19 @@ SolveStats Krylov::cg(const Ell &A, const double *b, double *x, const cfd_solve_
     }
     if (prof) {
         size_t worked = std::min<size_t>(nev, (size_t)hstate_->iter);
+        double ms_sum = 0.;
         for (size_t k = 0; k < worked; ++k) {
             float ms = 0.f;
             CFD_HIP(hipEventElapsedTime(&ms, ev(2 * k), ev(2 * k + 1)));
-            prof_ms += ms;
+            ms_sum += ms;
         }
-        prof_launches += (long long)worked;
+        if (sizeof(T) == 8) {
+            prof_ms += ms_sum;
+            prof_launches += (long long)worked;
+        } else {
+            prof_ms_f32_ += ms_sum;
+            prof_launches_f32_ += (long long)worked;
+        }
     }
     SolveStats S;
     S.iters = hstate_->iter;
@@ -855,40 +972,44 @@ SolveStats Krylov::cg(const Ell &A, const double *b, double *x, const cfd_solve_
     S.breakdown = hstate_->brk;
     return S;
 }
+template SolveStats Krylov::cg<double>(const Ell &, const double *, double *,
+                                       const cfd_solve_cfg &, Colouring *, int, KWork<double> &);
+template SolveStats Krylov::cg<float>(const EllF &, const float *, float *, const cfd_solve_cfg &,
+                                      Colouring *, int, KWork<float> &);
 
 SolveStats Krylov::bicgstab(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
                             Colouring *col, int pc) {
     const int n = A.n, G = red_grid(n), g = grid_for(n);
     const bool jac = pc == CFD_PC_JACOBI;
-    hipLaunchKernelGGL(k_residual, dim3(G), dim3(kBlock), 0, st_, A, b, (const double *)x, r_.p,
-                       p0_.p, v_.p, state_.p, part_.p, ticket_.p, 3, cfg.tol, cfg.maxit);
-    hipLaunchKernelGGL(k_copy, dim3(g), dim3(kBlock), 0, st_, n, (const double *)r_.p, rt_.p);
+    hipLaunchKernelGGL(k_residual<double>, dim3(G), dim3(kBlock), 0, st_, A, b, (const double *)x, wd_.r.p,
+                       wd_.p0.p, v_.p, state_.p, part_.p, ticket_.p, 3, cfg.tol, cfg.maxit);
+    hipLaunchKernelGGL(k_copy, dim3(g), dim3(kBlock), 0, st_, n, (const double *)wd_.r.p, rt_.p);
     CFD_LAUNCH_CHECK();
     int batch = check_first, issued = 0, prev_it = 0;
     double prev_log = 1e300;
     for (;;) {
         for (int k = 0; k < batch; ++k) {
             if (jac) {
-                hipLaunchKernelGGL(k_bi_p<true>, dim3(g), dim3(kBlock), 0, st_, n, r_.p, p0_.p,
+                hipLaunchKernelGGL(k_bi_p<true>, dim3(g), dim3(kBlock), 0, st_, n, wd_.r.p, wd_.p0.p,
                                    v_.p, ph_.p, A.diag, state_.p);
             } else {
-                hipLaunchKernelGGL(k_bi_p<false>, dim3(g), dim3(kBlock), 0, st_, n, r_.p, p0_.p,
+                hipLaunchKernelGGL(k_bi_p<false>, dim3(g), dim3(kBlock), 0, st_, n, wd_.r.p, wd_.p0.p,
                                    v_.p, ph_.p, A.diag, state_.p);
-                apply_pc(A, pc, cfg.omega, p0_.p, ph_.p, col);
+                apply_pc(A, pc, cfg.omega, (const double *)wd_.p0.p, ph_.p, col, wd_);
             }
             hipLaunchKernelGGL(k_bi_spmv<0>, dim3(G), dim3(kBlock), 0, st_, A, ph_.p, v_.p, rt_.p,
                                state_.p, part_.p, ticket_.p);
             if (jac) {
-                hipLaunchKernelGGL(k_bi_s<true>, dim3(G), dim3(kBlock), 0, st_, n, r_.p, v_.p,
+                hipLaunchKernelGGL(k_bi_s<true>, dim3(G), dim3(kBlock), 0, st_, n, wd_.r.p, v_.p,
                                    s_.p, sh_.p, A.diag, state_.p, part_.p, ticket_.p);
             } else {
-                hipLaunchKernelGGL(k_bi_s<false>, dim3(G), dim3(kBlock), 0, st_, n, r_.p, v_.p,
+                hipLaunchKernelGGL(k_bi_s<false>, dim3(G), dim3(kBlock), 0, st_, n, wd_.r.p, v_.p,
                                    s_.p, sh_.p, A.diag, state_.p, part_.p, ticket_.p);
-                apply_pc(A, pc, cfg.omega, s_.p, sh_.p, col);
+                apply_pc(A, pc, cfg.omega, (const double *)s_.p, sh_.p, col, wd_);
             }
             hipLaunchKernelGGL(k_bi_spmv<1>, dim3(G), dim3(kBlock), 0, st_, A, sh_.p, t_.p, s_.p,
                                state_.p, part_.p, ticket_.p);
-            hipLaunchKernelGGL(k_bi_x, dim3(G), dim3(kBlock), 0, st_, n, x, r_.p, ph_.p, sh_.p,
+            hipLaunchKernelGGL(k_bi_x, dim3(G), dim3(kBlock), 0, st_, n, x, wd_.r.p, ph_.p, sh_.p,
                                s_.p, t_.p, rt_.p, state_.p, part_.p, ticket_.p);
         }
         CFD_LAUNCH_CHECK();
@@ -917,7 +1038,7 @@ SolveStats Krylov::gmres(const Ell &A, const double *b, double *x, const cfd_sol
         CFD_HIP(hipStreamSynchronize(st_));
     }
     double *W = w0_.p, *Wn = w1_.p;
-    hipLaunchKernelGGL(k_residual, dim3(G), dim3(kBlock), 0, st_, A, b, (const double *)x, W,
+    hipLaunchKernelGGL(k_residual<double>, dim3(G), dim3(kBlock), 0, st_, A, b, (const double *)x, W,
                        (double *)nullptr, (double *)nullptr, state_.p, part_.p, ticket_.p, 2,
                        cfg.tol, cfg.maxit);
     CFD_LAUNCH_CHECK();
@@ -931,8 +1052,8 @@ SolveStats Krylov::gmres(const Ell &A, const double *b, double *x, const cfd_sol
             } else {
                 hipLaunchKernelGGL(k_gm_v<false>, dim3(g), dim3(kBlock), 0, st_, A, W, Vj, Wn,
                                    state_.p);
-                apply_pc(A, pc, cfg.omega, Vj, z_.p, col);
-                hipLaunchKernelGGL(k_gm_spmv, dim3(g), dim3(kBlock), 0, st_, A, z_.p, Wn,
+                apply_pc(A, pc, cfg.omega, (const double *)Vj, wd_.z.p, col, wd_);
+                hipLaunchKernelGGL(k_gm_spmv, dim3(g), dim3(kBlock), 0, st_, A, wd_.z.p, Wn,
                                    state_.p);
             }
             hipLaunchKernelGGL(k_gm_dot, dim3(G), dim3(kBlock), 0, st_, n, Wn, V_.p, state_.p,
@@ -945,17 +1066,17 @@ SolveStats Krylov::gmres(const Ell &A, const double *b, double *x, const cfd_sol
         }
         hipLaunchKernelGGL(k_gm_y, dim3(1), dim3(64), 0, st_, state_.p);
         if (jac) {
-            hipLaunchKernelGGL(k_gm_u<true>, dim3(g), dim3(kBlock), 0, st_, n, V_.p, z_.p, x,
+            hipLaunchKernelGGL(k_gm_u<true>, dim3(g), dim3(kBlock), 0, st_, n, V_.p, wd_.z.p, x,
                                A.diag, state_.p);
         } else {
-            hipLaunchKernelGGL(k_gm_u<false>, dim3(g), dim3(kBlock), 0, st_, n, V_.p, q_.p, x,
+            hipLaunchKernelGGL(k_gm_u<false>, dim3(g), dim3(kBlock), 0, st_, n, V_.p, wd_.q.p, x,
                                A.diag, state_.p);
-            apply_pc(A, pc, cfg.omega, q_.p, z_.p, col);
-            hipLaunchKernelGGL(k_add, dim3(g), dim3(kBlock), 0, st_, n, x, z_.p, state_.p);
+            apply_pc(A, pc, cfg.omega, (const double *)wd_.q.p, wd_.z.p, col, wd_);
+            hipLaunchKernelGGL(k_add, dim3(g), dim3(kBlock), 0, st_, n, x, wd_.z.p, state_.p);
         }
         W = w0_.p;
         Wn = w1_.p;
-        hipLaunchKernelGGL(k_residual, dim3(G), dim3(kBlock), 0, st_, A, b, (const double *)x, W,
+        hipLaunchKernelGGL(k_residual<double>, dim3(G), dim3(kBlock), 0, st_, A, b, (const double *)x, W,
                            (double *)nullptr, (double *)nullptr, state_.p, part_.p, ticket_.p, 2,
                            cfg.tol, -1);
         CFD_LAUNCH_CHECK();
@@ -967,5 +1088,6 @@ SolveStats Krylov::gmres(const Ell &A, const double *b, double *x, const cfd_sol
     S.breakdown = hstate_->brk;
     return S;
 }
+
 
 }  // namespace cfd

@@ -1,0 +1,272 @@
+// Multi-RHS BiCGSTAB + Jacobi for momentum: up to three systems A x_k = b_k
+// with one matrix (u, v, w of CalculateVelocity, src/navier_velocity.cpp:
+// 23-58, which solve the same As three times). Every kernel streams the
+// matrix once for all systems; the recurrences, reductions and stopping tests
+// of each system are those of the single-system BiCGSTAB, in the same order,
+// so each solution is bit-identical to solving the systems one by one.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "krylov.h"
+
+namespace cfd {
+
+namespace {
+
+constexpr int NB = 3;
+
+__device__ __forceinline__ bool rtc_m(double rn, double bn, double eps) {
+    return bn > 0. ? rn < eps * bn : rn < eps;
+}
+
+struct MVec {  // per-system vectors
+    double *r[NB], *rt[NB], *p[NB], *v[NB], *s[NB], *t[NB], *ph[NB], *sh[NB];
+    double *x[NB];
+    const double *b[NB];
+};
+
+#define MLOOP(i, n) \
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+
+__global__ void k_m_init(Ell A, MVec V, int nb, KState *st, double *part, unsigned *tk,
+                         double eps, int maxit) {
+    double acc[2 * NB];
+    for (int k = 0; k < 2 * NB; ++k) acc[k] = 0.;
+    const int n = A.n;
+    MLOOP(i, n) {
+        double s[NB];
+        const double d = A.diag[i];
+        for (int q = 0; q < NB; ++q) if (q < nb) s[q] = d * V.x[q][i];
+        for (int k = 0; k < A.K; ++k) {
+            size_t o = (size_t)k * n + i;
+            const double a = A.val[o];
+            const int c = A.col[o];
+            for (int q = 0; q < NB; ++q) if (q < nb) s[q] += a * V.x[q][c];
+        }
+        for (int q = 0; q < NB; ++q) if (q < nb) {
+            double ri = V.b[q][i] - s[q];
+            V.r[q][i] = ri;
+            V.rt[q][i] = ri;
+            V.p[q][i] = 0.;
+            V.v[q][i] = 0.;
+            acc[q] += ri * ri;
+            acc[NB + q] += V.b[q][i] * V.b[q][i];
+        }
+    }
+    if (grid_sum<2 * NB>(acc, part, tk))
+        for (int q = 0; q < NB; ++q) if (q < nb) {
+            KState &S = st[q];
+            S.eps = eps;
+            S.maxit = maxit;
+            S.iter = 0;
+            S.brk = 0;
+            S.half = 0;
+            S.rr = acc[q];
+            S.bb = acc[NB + q];
+            S.bnorm = sqrt(acc[NB + q]);
+            S.rnorm = sqrt(acc[q]);
+            S.rho = acc[q];
+            S.rho_old = 1.;
+            S.alpha = 1.;
+            S.omega = 1.;
+            S.beta = 0.;
+            S.done = rtc_m(S.rnorm, S.bnorm, eps) || maxit <= 0;
+            if (!S.done && S.rho == 0.) { S.done = 1; S.brk = 1; }
+        }
+}
+
+__global__ void k_m_p(int n, MVec V, int nb, const double *__restrict__ d, const KState *st) {
+    bool act[NB];
+    double beta[NB], om[NB];
+    bool first[NB];
+    for (int q = 0; q < NB; ++q) if (q < nb) {
+        act[q] = !st[q].done;
+        first[q] = st[q].iter == 0;
+        beta[q] = (st[q].rho / st[q].rho_old) * (st[q].alpha / st[q].omega);
+        om[q] = st[q].omega;
+    }
+    MLOOP(i, n) {
+        const double di = d[i];
+        for (int q = 0; q < NB; ++q) if (q < nb) {
+            if (!act[q]) continue;
+            double pi = first[q] ? V.r[q][i] : V.r[q][i] + beta[q] * (V.p[q][i] - om[q] * V.v[q][i]);
+            V.p[q][i] = pi;
+            V.ph[q][i] = pi / di;
+        }
+    }
+}
+
+// MODE 0: v = A ph, rt.v -> alpha ; MODE 1: t = A sh, (t.s, t.t) -> omega
+template <int MODE>
+__global__ void k_m_spmv(Ell A, MVec V, int nb, KState *st, double *part, unsigned *tk) {
+    bool act[NB];
+    for (int q = 0; q < NB; ++q) if (q < nb) act[q] = !st[q].done && (MODE == 0 || !st[q].half);
+    double acc[2 * NB];
+    for (int k = 0; k < 2 * NB; ++k) acc[k] = 0.;
+    const int n = A.n;
+    MLOOP(i, n) {
+        double s[NB];
+        const double d = A.diag[i];
+        for (int q = 0; q < NB; ++q) if (q < nb) s[q] = act[q] ? d * (MODE == 0 ? V.ph[q][i] : V.sh[q][i]) : 0.;
+        for (int k = 0; k < A.K; ++k) {
+            size_t o = (size_t)k * n + i;
+            const double a = A.val[o];
+            const int c = A.col[o];
+            for (int q = 0; q < NB; ++q) if (q < nb)
+                if (act[q]) s[q] += a * (MODE == 0 ? V.ph[q][c] : V.sh[q][c]);
+        }
+        for (int q = 0; q < NB; ++q) if (q < nb) {
+            if (!act[q]) continue;
+            if (MODE == 0) {
+                V.v[q][i] = s[q];
+                acc[q] += V.rt[q][i] * s[q];
+            } else {
+                V.t[q][i] = s[q];
+                acc[q] += s[q] * V.s[q][i];
+                acc[NB + q] += s[q] * s[q];
+            }
+        }
+    }
+    if (grid_sum<2 * NB>(acc, part, tk))
+        for (int q = 0; q < NB; ++q) if (q < nb) {
+            if (!act[q]) continue;
+            if (MODE == 0) st[q].alpha = st[q].rho / acc[q];
+            else st[q].omega = acc[q] / acc[NB + q];
+        }
+}
+
+__global__ void k_m_s(int n, MVec V, int nb, const double *__restrict__ d, KState *st,
+                      double *part, unsigned *tk) {
+    bool act[NB];
+    double al[NB];
+    for (int q = 0; q < NB; ++q) if (q < nb) {
+        act[q] = !st[q].done;
+        al[q] = st[q].alpha;
+    }
+    double acc[NB];
+    for (int q = 0; q < NB; ++q) acc[q] = 0.;
+    MLOOP(i, n) {
+        const double di = d[i];
+        for (int q = 0; q < NB; ++q) if (q < nb) {
+            if (!act[q]) continue;
+            double si = V.r[q][i] - al[q] * V.v[q][i];
+            V.s[q][i] = si;
+            V.sh[q][i] = si / di;
+            acc[q] += si * si;
+        }
+    }
+    if (grid_sum<NB>(acc, part, tk))
+        for (int q = 0; q < NB; ++q) if (q < nb)
+            if (act[q]) st[q].half = rtc_m(sqrt(acc[q]), st[q].bnorm, st[q].eps);
+}
+
+__global__ void k_m_x(int n, MVec V, int nb, KState *st, double *part, unsigned *tk) {
+    bool act[NB], half[NB];
+    double al[NB], om[NB];
+    for (int q = 0; q < NB; ++q) if (q < nb) {
+        act[q] = !st[q].done;
+        half[q] = st[q].half;
+        al[q] = st[q].alpha;
+        om[q] = st[q].omega;
+    }
+    double acc[2 * NB];
+    for (int k = 0; k < 2 * NB; ++k) acc[k] = 0.;
+    MLOOP(i, n) {
+        for (int q = 0; q < NB; ++q) if (q < nb) {
+            if (!act[q]) continue;
+            double ri;
+            if (half[q]) {
+                V.x[q][i] += al[q] * V.ph[q][i];
+                ri = V.s[q][i];
+            } else {
+                V.x[q][i] = V.x[q][i] + al[q] * V.ph[q][i] + om[q] * V.sh[q][i];
+                ri = V.s[q][i] - om[q] * V.t[q][i];
+            }
+            V.r[q][i] = ri;
+            acc[q] += ri * ri;
+            acc[NB + q] += V.rt[q][i] * ri;
+        }
+    }
+    if (grid_sum<2 * NB>(acc, part, tk))
+        for (int q = 0; q < NB; ++q) if (q < nb) {
+            if (!act[q]) continue;
+            KState &S = st[q];
+            S.iter += 1;
+            S.rr = acc[q];
+            S.rnorm = sqrt(acc[q]);
+            if (half[q]) {
+                S.done = 1;
+                continue;
+            }
+            S.rho_old = S.rho;
+            S.rho = acc[NB + q];
+            S.done = rtc_m(S.rnorm, S.bnorm, S.eps) || S.iter >= S.maxit;
+            if (!S.done && om[q] == 0.) { S.done = 1; S.brk = 1; }
+            if (!S.done && S.rho == 0.) { S.done = 1; S.brk = 1; S.iter += 1; }
+        }
+}
+
+}  // namespace
+
+void Krylov::solve_multi(const Ell &A, int nb, const double *const *b, double *const *x,
+                         const cfd_solve_cfg &cfg, SolveStats *out) {
+    if (nb < 1 || nb > NB) throw HipError("multi-RHS solve takes 1..3 systems");
+    if (cfg.method != CFD_BICGSTAB || cfg.precond != CFD_PC_JACOBI)
+        throw HipError("multi-RHS solve is BiCGSTAB + Jacobi");
+    const int n = A.n, G = red_grid(n), g = grid_for(n);
+    ensure(n, A.K);
+    if (mrhs_.n < (size_t)8 * NB * n) mrhs_.alloc((size_t)8 * NB * n);
+    if (!mstate_.p) {
+        mstate_.alloc(NB);
+        CFD_HIP(hipHostMalloc((void **)&hmstate_, NB * sizeof(KState), hipHostMallocDefault));
+    }
+    MVec V;
+    double *base = mrhs_.p;
+    for (int q = 0; q < NB; ++q) {
+        double **slots[8] = {&V.r[q], &V.rt[q], &V.p[q], &V.v[q], &V.s[q], &V.t[q], &V.ph[q], &V.sh[q]};
+        for (int k = 0; k < 8; ++k) *slots[k] = base + ((size_t)(8 * q + k)) * n;
+        V.x[q] = q < nb ? x[q] : nullptr;
+        V.b[q] = q < nb ? b[q] : nullptr;
+    }
+    hipLaunchKernelGGL(k_m_init, dim3(G), dim3(kBlock), 0, st_, A, V, nb, mstate_.p, part_.p,
+                       ticket_.p, cfg.tol, cfg.maxit);
+    CFD_LAUNCH_CHECK();
+    int batch = check_first, issued = 0;
+    for (;;) {
+        for (int k = 0; k < batch; ++k) {
+            hipLaunchKernelGGL(k_m_p, dim3(g), dim3(kBlock), 0, st_, n, V, nb,
+                               (const double *)A.diag, (const KState *)mstate_.p);
+            hipLaunchKernelGGL(k_m_spmv<0>, dim3(G), dim3(kBlock), 0, st_, A, V, nb, mstate_.p,
+                               part_.p, ticket_.p);
+            hipLaunchKernelGGL(k_m_s, dim3(G), dim3(kBlock), 0, st_, n, V, nb,
+                               (const double *)A.diag, mstate_.p, part_.p, ticket_.p);
+            hipLaunchKernelGGL(k_m_spmv<1>, dim3(G), dim3(kBlock), 0, st_, A, V, nb, mstate_.p,
+                               part_.p, ticket_.p);
+            hipLaunchKernelGGL(k_m_x, dim3(G), dim3(kBlock), 0, st_, n, V, nb, mstate_.p,
+                               part_.p, ticket_.p);
+        }
+        CFD_LAUNCH_CHECK();
+        issued += batch;
+        CFD_HIP(hipMemcpyAsync(hmstate_, mstate_.p, nb * sizeof(KState), hipMemcpyDeviceToHost, st_));
+        CFD_HIP(hipStreamSynchronize(st_));
+        bool all = true;
+        int maxiter = 0;
+        for (int q = 0; q < NB; ++q) if (q < nb) {
+            all = all && hmstate_[q].done;
+            maxiter = std::max(maxiter, hmstate_[q].iter);
+        }
+        if (all || issued > cfg.maxit + 2) break;
+        batch = std::min(std::max(2, batch * 2), check_max);
+        (void)maxiter;
+    }
+    for (int q = 0; q < NB; ++q) if (q < nb) {
+        out[q].iters = hmstate_[q].iter;
+        out[q].breakdown = hmstate_[q].brk;
+        out[q].rtcres = hmstate_[q].bnorm > 0 ? hmstate_[q].rnorm / hmstate_[q].bnorm
+                                              : hmstate_[q].rnorm;
+        out[q].relres = relres(A, x[q], b[q]);
+    }
+}
+
+}  // namespace cfd

@@ -153,7 +153,7 @@ int build_faces(HostMesh &m, std::string &err) {
         }
     }
     m.nfac = (int)m.face_c1.size();
-    m.ncb = m.nbnd;
+    m.ncb = 0;  // NCoupledBnd stays 0 unless a split sets it (src/BasicSolver.cpp:14)
     return 0;
 }
 

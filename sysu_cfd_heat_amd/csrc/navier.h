@@ -64,6 +64,7 @@ class NSSolver {
     DBuf<double> gU, gV, gW, gP, gPhi;  // [3][C]
     DBuf<double> BR, BU, BV, BW, BP, BT, BTE, BED, BndQ;
     DBuf<double> xsol, bu, bv, bw, bp, bs, kcond;
+    DBuf<double> xv, xw;  // extra solution vectors of the multi-RHS momentum solve
     DBuf<double> As_diag, As_val, Ap_diag, Ap_val;
     DBuf<double> Prod, TES, EDS, ApTE, ApED, VisTE, VisED, zeros;
     DBuf<double> res_dev, gtmp, red_part;

@@ -262,7 +262,8 @@ Colouring *NSSolver::colouring_for(const cfd_solve_cfg &c) {
 
 SolveStats NSSolver::solve(const Ell &A, const double *b, double *x, const cfd_solve_cfg &c,
                            Colouring *col, int precision) {
-    (void)precision;
+    // FP32 inner CG to 1e-4 per refinement, FP64 residual and update
+    if (precision == 1 && c.method == CFD_CG) return kry->solve_mixed(A, b, x, c, col, 1e-4);
     return kry->solve(A, b, x, c, col);
 }
 
@@ -302,6 +303,29 @@ int NSSolver::CalculateVelocity() {
     double *fld[3] = {U, V, W};
     const double *rhs[3] = {bu, bv, bw};
     SolveStats *out[3] = {&last_u, &last_v, &last_w};
+    if (P.multi_rhs && P.mom.method == CFD_BICGSTAB && P.mom.precond == CFD_PC_JACOBI) {
+        // one matrix, three right-hand sides: the matrix streams once per SpMV
+        if (!xv.p) {
+            xv.alloc(hm.ncel);
+            xw.alloc(hm.ncel);
+        }
+        double *xs[3] = {xsol, xv, xw};
+        for (int c = 0; c < 3; ++c)
+            CFD_HIP(hipMemcpyAsync(xs[c], fld[c], sizeof(double) * hm.ncel,
+                                   hipMemcpyDeviceToDevice, s));
+        SolveStats st3[3];
+        kry->solve_multi(As(), 3, rhs, xs, P.mom, st3);
+        for (int c = 0; c < 3; ++c) {
+            *out[c] = st3[c];
+            stats[2] += st3[c].iters;
+            if (st3[c].iters >= P.mom.maxit && st3[c].relres > P.mom.tol) {
+                last_error() = "velocity cannot converge";
+                return -1;
+            }
+            launch_take_solution(dm(), xs[c], fld[c], res_dev.p + c, 0., 0, red(), s);
+        }
+        return 0;
+    }
     for (int c = 0; c < 3; ++c) {
         CFD_HIP(hipMemcpyAsync(xsol, fld[c], sizeof(double) * hm.ncel, hipMemcpyDeviceToDevice, s));
         *out[c] = solve(As(), rhs[c], xsol, P.mom, col, 0);
@@ -333,6 +357,7 @@ void NSSolver::CorrectRUFace2() {
 int NSSolver::CalculatePressure() {
     launch_fill(xsol, hm.ncel, 0., s);
     BuildPressureMatrix();
+    if (P.pres_project) launch_remove_mean(bp, hm.ncel, red(), s);
     kry->prof = prof_pressure;
     last_p = solve(Ap(), bp, xsol, P.pres, colouring_for(P.pres), P.pres_precision);
     kry->prof = false;
@@ -780,24 +805,27 @@ int cfd_ns_last_solve(const cfd_ns *h, const char *eq, int *iters, double *relre
 
 int cfd_ns_profile(cfd_ns *h, int enable) {
     h->ns.prof_pressure = enable != 0;
-    h->ns.kry->prof_ms = 0.;
-    h->ns.kry->prof_launches = 0;
+    h->ns.kry->prof_reset();
     return 0;
 }
 
-int cfd_ns_profile_read(const cfd_ns *h, double out[4]) {
+int cfd_ns_profile_read(const cfd_ns *h, double out[8]) {
     const NSSolver &n = h->ns;
     long long nnz = 0;
     for (size_t k = 0; k < n.dmo.ell_col_host.size(); ++k)
         if (n.dmo.ell_col_host[k] != (int)(k % n.hm.ncel)) ++nnz;
-    // k_cg_pq per launch: reads r|z, diag, p_old (3 x 8 B per row), writes
-    // p_new, q (2 x 8 B per row); 8 B value + 4 B column per stored
+    // k_cg_pq per launch: reads r|z, diag, p_old (3 words per row), writes
+    // p_new, q (2 words per row); one value word + a 4 B column per stored
     // off-diagonal; neighbour gathers of the vectors hit cache and are not
-    // counted (each vector is read once)
+    // counted (each vector is read once). Words are 8 B (FP64) or 4 B (FP32).
     out[0] = (double)n.kry->prof_launches;
     out[1] = n.kry->prof_ms;
     out[2] = 40.0 * n.hm.ncel + 12.0 * (double)nnz;
     out[3] = (double)nnz;
+    out[4] = (double)n.kry->prof_launches_f32();
+    out[5] = n.kry->prof_ms_f32();
+    out[6] = 20.0 * n.hm.ncel + 8.0 * (double)nnz;
+    out[7] = 0.;
     return 0;
 }
 

@@ -1,0 +1,76 @@
+/*
+ * cfd_gnn.h -- learned initial guess for the pressure-correction solve.
+ *
+ * No counterpart in the reference: CalculatePressure starts CG from x0 = 0
+ * (V_SetAllCmp(&xsol, 0.), src/navier_pressure.cpp:21). This extension
+ * predicts x0 with a small message-passing graph network on the cell graph
+ * of the pressure matrix; the plain Krylov solve then always finishes the
+ * job to the same true FP64 residual, so a bad prediction costs iterations,
+ * never accuracy.
+ *
+ * Network (width H = 64):
+ *   f_i   = [vol_i/mean vol, x_i, y_i, z_i (scaled to [0,1]), b_i/rms b,
+ *            p_prev_i/rms p_prev, a_ii/rms diag, 1]                (8 inputs)
+ *   h1    = relu(f W0 + c0)                                         8 -> 64
+ *   m_l   = sum_k w_ik h_l[col_ik],  w_ik = a_ik / |a_ii|           (message)
+ *   h_l+1 = relu([h_l | m_l] W_l + c_l),  l = 1..3                128 -> 64
+ *   y     = h4 w_out + c_out                                        64 -> 1
+ * The per-cell products run on BF16 MFMA (v_mfma_f32_16x16x16_bf16) with
+ * FP32 accumulation; messages and element-wise work are FP32 HIP kernels.
+ * The guess is the A-norm-optimal combination of the mean-free y and the
+ * previous outer iteration's p' (two SpMVs, a 2x2 Galerkin system).
+ * Training: Adam on mean((y - mean y - t)^2), t = the converged p' made
+ * mean-free and scaled to unit RMS, on (system, solution) pairs captured
+ * from the solver's own pressure solves.
+ */
+#ifndef SYSU_CFD_HEAT_AMD_CFD_GNN_H
+#define SYSU_CFD_HEAT_AMD_CFD_GNN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "cfd_solver.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFD_GNN_IN 8
+#define CFD_GNN_H 64
+#define CFD_GNN_LAYERS 3
+
+typedef struct cfd_gnn_s cfd_gnn;
+
+cfd_gnn *cfd_gnn_create(uint64_t seed);
+void cfd_gnn_free(cfd_gnn *g);
+/* parameters, FP32, packed W0[8][64] c0[64] (W_l[128][64] c_l[64]) x3
+ * w_out[64] c_out[1] */
+size_t cfd_gnn_num_params(void);
+int cfd_gnn_get_params(const cfd_gnn *g, float *out);
+int cfd_gnn_set_params(cfd_gnn *g, const float *in);
+
+/* One graph given on the host (tests): n cells, K slots per cell,
+ * col[K][n] (pad = self), w[K][n] message weights, feat[n][8].
+ * forward: y[n]. loss_grad: loss and d loss / d params for target t[n]
+ * (t already mean-free, unit RMS). */
+int cfd_gnn_forward_host(cfd_gnn *g, int n, int K, const int *col, const float *w,
+                         const float *feat, float *y);
+int cfd_gnn_loss_grad_host(cfd_gnn *g, int n, int K, const int *col, const float *w,
+                           const float *feat, const float *t, float *loss, float *grad);
+
+/* Capture (pressure system, converged p') pairs from a solver's pressure
+ * solves into g's device-resident data set (up to max_samples). */
+int cfd_ns_capture(cfd_ns *h, cfd_gnn *g, int max_samples);
+int cfd_gnn_num_samples(const cfd_gnn *g);
+/* Adam training on the captured samples; out[0] = first-epoch loss,
+ * out[1] = last-epoch loss. */
+int cfd_gnn_train(cfd_gnn *g, int epochs, float lr, uint64_t seed, double out[2]);
+/* mode 0: x0 = 0 (reference); 1: x0 = the previous p' scaled (A-norm
+ * optimal); 2: the learned guess combined with the previous p'. */
+int cfd_ns_set_guess(cfd_ns *h, cfd_gnn *g, int mode);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SYSU_CFD_HEAT_AMD_CFD_GNN_H */

@@ -95,6 +95,19 @@ def lib() -> C.CDLL:
     L.cfd_ns_reset_stats.argtypes = [vp]
     L.cfd_ns_profile.argtypes = [vp, C.c_int]
     L.cfd_ns_profile_read.argtypes = [vp, dp]
+    fp = C.POINTER(C.c_float)
+    L.cfd_gnn_create.restype = vp
+    L.cfd_gnn_create.argtypes = [C.c_uint64]
+    L.cfd_gnn_free.argtypes = [vp]
+    L.cfd_gnn_num_params.restype = C.c_size_t
+    L.cfd_gnn_get_params.argtypes = [vp, fp]
+    L.cfd_gnn_set_params.argtypes = [vp, fp]
+    L.cfd_gnn_forward_host.argtypes = [vp, C.c_int, C.c_int, ip, fp, fp, fp]
+    L.cfd_gnn_loss_grad_host.argtypes = [vp, C.c_int, C.c_int, ip, fp, fp, fp, fp, fp]
+    L.cfd_ns_capture.argtypes = [vp, vp, C.c_int]
+    L.cfd_gnn_num_samples.argtypes = [vp]
+    L.cfd_gnn_train.argtypes = [vp, C.c_int, C.c_float, C.c_uint64, dp]
+    L.cfd_ns_set_guess.argtypes = [vp, vp, C.c_int]
     _lib = L
     return L
 
@@ -109,6 +122,10 @@ def _dp(a: np.ndarray):
 
 def _ip(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 # ------------------------------------------------------------------ mesh --
@@ -352,6 +369,21 @@ class NSSolver:
         self._L.cfd_ns_profile_read(self.h, _dp(out))
         return out
 
+    # -- pressure-correction initial guess (include/cfd_gnn.h) --
+    def capture(self, net: Optional["Gnn"], max_samples: int) -> None:
+        """Append (pressure system, converged p') pairs of later pressure
+        solves to net's device-resident data set, up to max_samples."""
+        self._cap_net = net
+        if self._L.cfd_ns_capture(self.h, net.h if net else None, int(max_samples)):
+            raise RuntimeError(self._error())
+
+    def set_guess(self, mode: int, net: Optional["Gnn"] = None) -> None:
+        """0: x0 = 0 (reference); 1: previous p' (A-norm optimal scale);
+        2: the network's prediction combined with the previous p'."""
+        self._guess_net = net
+        if self._L.cfd_ns_set_guess(self.h, net.h if net else None, int(mode)):
+            raise RuntimeError(self._error())
+
     def close(self):
         if getattr(self, "h", None):
             self._fn("free")(self.h)
@@ -375,3 +407,80 @@ def dense_from_rows(row_len, col, val, n: int, sym: bool = False) -> np.ndarray:
                 A[j, i] += val[k]
             k += 1
     return A
+
+
+# ------------------------------------------------------------------- GNN --
+GNN_IN, GNN_H, GNN_LAYERS = 8, 64, 3
+
+
+class Gnn:
+    """Message-passing network predicting the pressure correction (device
+    parameters, BF16 MFMA forward/backward, Adam training on the GPU)."""
+
+    def __init__(self, seed: int = 0):
+        self._L = lib()
+        self.h = self._L.cfd_gnn_create(int(seed))
+        if not self.h:
+            raise RuntimeError(_err(self._L))
+        self.num_params = int(self._L.cfd_gnn_num_params())
+
+    def _chk(self, rc):
+        if rc:
+            raise RuntimeError(_err(self._L))
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.num_params, np.float32)
+        self._chk(self._L.cfd_gnn_get_params(self.h, _fp(out)))
+        return out
+
+    def set_params(self, p) -> None:
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        if a.size != self.num_params:
+            raise ValueError("wrong parameter count")
+        self._chk(self._L.cfd_gnn_set_params(self.h, _fp(a)))
+
+    @staticmethod
+    def _graph(col, w, feat):
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        K, n = col.shape
+        w = np.ascontiguousarray(w, dtype=np.float32).reshape(K, n)
+        feat = np.ascontiguousarray(feat, dtype=np.float32).reshape(n, GNN_IN)
+        return n, K, col, w, feat
+
+    def forward(self, col, w, feat) -> np.ndarray:
+        """col, w: [K][n] neighbour slots (pad = self) and message weights;
+        feat: [n][8]. Returns y[n]."""
+        n, K, col, w, feat = self._graph(col, w, feat)
+        y = np.zeros(n, np.float32)
+        self._chk(self._L.cfd_gnn_forward_host(self.h, n, K, _ip(col), _fp(w), _fp(feat),
+                                               _fp(y)))
+        return y
+
+    def loss_grad(self, col, w, feat, t):
+        n, K, col, w, feat = self._graph(col, w, feat)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_float()
+        g = np.zeros(self.num_params, np.float32)
+        self._chk(self._L.cfd_gnn_loss_grad_host(self.h, n, K, _ip(col), _fp(w), _fp(feat),
+                                                 _fp(t), C.byref(loss), _fp(g)))
+        return float(loss.value), g
+
+    def num_samples(self) -> int:
+        return int(self._L.cfd_gnn_num_samples(self.h))
+
+    def train(self, epochs: int, lr: float = 1e-3, seed: int = 0):
+        """Adam over the captured samples; returns (first, last) epoch loss."""
+        out = np.zeros(2)
+        self._chk(self._L.cfd_gnn_train(self.h, int(epochs), float(lr), int(seed), _dp(out)))
+        return float(out[0]), float(out[1])
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.cfd_gnn_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

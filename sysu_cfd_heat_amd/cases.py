@@ -54,9 +54,10 @@ def cavity(n=32, re=100.0, tets=False, jitter=0.0, seed=1, max_step=201, solvers
 
 
 def channel(nx=128, ny=64, nz=64, tets=True, jitter=0.1, seed=1, re=1e4, turbulence=True,
-            energy=True, cht=True, heat=1.0, max_step=2, solvers="config"):
+            energy=True, cht=True, heat=1.0, max_step=2, solvers="config", u_in=1.0):
     """Conjugate-heat-transfer channel over [0,2]x[0,1]x[0,1]: inlet x=0
-    (u=1, T=0), outlet x=2, adiabatic top wall, bottom wall T=0, symmetric
+    (u=u_in, T=0; the viscosity is fixed by re at u=1, so the Reynolds number
+    scales with u_in), outlet x=2, adiabatic top wall, bottom wall T=0, symmetric
     z sides; a solid block x in [0.4,0.6], y < 0.3 with k_s = 100 k_f and a
     volumetric heat source `heat` (a build extension; the reference hard-codes
     0, src/HeatConductionSolver.cpp:60-62)."""
@@ -69,11 +70,11 @@ def channel(nx=128, ny=64, nz=64, tets=True, jitter=0.1, seed=1, re=1e4, turbule
     mu = 1.0 / re
     # 5% intensity, turbulent length scale 1% of the height: with a 10% scale
     # the reference's k-epsilon + wall-function model runs away on this case
-    k_in = 1.5 * (0.05 * 1.0) ** 2
+    k_in = 1.5 * (0.05 * u_in) ** 2
     e_in = 0.09 ** 0.75 * k_in ** 1.5 / 0.01
     add_region(p, TOP, 1, 1, 0.0)                 # Hwall q = 0
     add_region(p, BOTTOM, 1, 0, 0.0)              # Twall T = 0
-    add_region(p, INLET, 2, 0, 0.0, (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, k_in, e_in))
+    add_region(p, INLET, 2, 0, 0.0, (u_in, 0.0, 0.0, 0.0, 1.0, 0.0, k_in, e_in))
     add_region(p, OUTLET, 3, 0, 0.0)
     add_region(p, SYM, 4, 0, 0.0)
     add_region(p, 8, 5, 0, 0.0, (1.0, 0.0, 0.0, 1.0, 0.0, mu, k_in, e_in))

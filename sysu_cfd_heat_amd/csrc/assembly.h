@@ -44,10 +44,10 @@ struct BcScal {
     int mode;  // 1: outlet rescale, 2: additive correction
 };
 
-// All launches go to stream s. part/tk: reduction scratch (12*kRedGrid).
+// All launches go to stream s. part: reduction partials (12*kRedGrid
+// doubles; a reducing kernel uses NV * grid of them).
 struct Red {
     double *part;
-    unsigned *tk;
 };
 
 void launch_gradient(const DevMesh &m, int nf, const double *const *phi,

@@ -201,9 +201,9 @@ __global__ void k_bc_velocity(DevMesh m, const double *__restrict__ R,
                               const double *__restrict__ U, const double *__restrict__ V,
                               const double *__restrict__ W, double *__restrict__ BR,
                               double *__restrict__ BU, double *__restrict__ BV,
-                              double *__restrict__ BW, BcScal *sc, double *part, unsigned *tk) {
+                              double *__restrict__ BW, BcScal *sc, double *part, int fin) {
     double acc[3] = {0., 0., 0.};
-    CELL_LOOP(b, m.nbnd) {
+    RED_LOOP(b, m.nbnd) {
         const int r = m.b_reg[b], t1 = m.reg->type1[r], f = m.b_face[b], ic = m.b_cell[b];
         double br, bu, bv, bw;
         const double n0 = m.f_n[0][f], n1 = m.f_n[1][f], n2 = m.f_n[2][f];
@@ -230,7 +230,7 @@ __global__ void k_bc_velocity(DevMesh m, const double *__restrict__ R,
             acc[2] += br * m.f_area[b];  // Face[i] with the boundary index, as the reference
         }
     }
-    if (grid_sum<3>(acc, part, tk)) {
+    if (grid_reduce<3>(acc, part, fin)) {
         sc->massin = acc[0];
         sc->massout = acc[1];
         sc->areaout = acc[2];
@@ -511,11 +511,11 @@ __global__ void k_pressure_update(DevMesh m, const double *__restrict__ x, int r
                                   const double *__restrict__ Apr, const double *__restrict__ g,
                                   double *__restrict__ P, double *__restrict__ U,
                                   double *__restrict__ V, double *__restrict__ W,
-                                  double *res_slot, double *part, unsigned *tk) {
+                                  double *res_slot, double *part, int fin) {
     const int C = m.ncel;
     const double xr = x[ref];
     double acc[1] = {0.};
-    CELL_LOOP(i, C) {
+    RED_LOOP(i, C) {
         const double vol = m.c_vol[i], d = x[i] - xr;
         acc[0] += fabs(d) * vol;
         P[i] = P[i] + urf * d;
@@ -524,20 +524,20 @@ __global__ void k_pressure_update(DevMesh m, const double *__restrict__ x, int r
         V[i] = V[i] - coef * g[C + i];
         W[i] = W[i] - coef * g[2 * C + i];
     }
-    if (grid_sum<1>(acc, part, tk)) *res_slot += acc[0];
+    if (grid_reduce<1>(acc, part, fin)) *res_slot += acc[0];
 }
 
 __global__ void k_take_solution(int C, const double *__restrict__ vol, const double *__restrict__ x,
                                 double *__restrict__ fld, double *res_slot, double lo, int clip,
-                                double *part, unsigned *tk) {
+                                double *part, int fin) {
     double acc[1] = {0.};
-    CELL_LOOP(i, C) {
+    RED_LOOP(i, C) {
         double xi = x[i];
         if (clip) xi = cmax(lo, xi);
         if (res_slot) acc[0] += fabs(fld[i] - xi) * vol[i];
         fld[i] = xi;
     }
-    if (res_slot && grid_sum<1>(acc, part, tk)) *res_slot += acc[0];
+    if (res_slot && grid_reduce<1>(acc, part, fin)) *res_slot += acc[0];
 }
 
 // BuildScalarMatrix: NavierStokesSolver's (src/navier_scalar.cpp:480-645),
@@ -795,10 +795,10 @@ __global__ void k_solid_bc_temperature(DevMesh m, const double *__restrict__ T,
 }
 
 __global__ void k_sum(int n, const double *__restrict__ b, double *out, double *part,
-                      unsigned *tk) {
+                      int fin) {
     double acc[1] = {0.};
-    CELL_LOOP(i, n) acc[0] += b[i];
-    if (grid_sum<1>(acc, part, tk)) *out = acc[0] / n;
+    RED_LOOP(i, n) acc[0] += b[i];
+    if (grid_reduce<1>(acc, part, fin)) *out = acc[0] / n;
 }
 __global__ void k_sub_scalar(int n, const double *__restrict__ m, double *__restrict__ b) {
     const double v = *m;
@@ -862,8 +862,8 @@ void launch_bc_velocity(const DevMesh &m, const double *R, const double *U, cons
                         const double *W, double *BR, double *BU, double *BV, double *BW,
                         BcScal *sc, Red red, hipStream_t s) {
     if (m.nbnd == 0) return;
-    hipLaunchKernelGGL(k_bc_velocity, dim3(red_grid(m.nbnd)), dim3(kBlock), 0, s, m, R, U, V, W,
-                       BR, BU, BV, BW, sc, red.part, red.tk);
+    CFD_REDUCE_LAUNCH(k_bc_velocity, red_grid(m.nbnd), s, m, R, U, V, W,
+                       BR, BU, BV, BW, sc, red.part, fin);
     hipLaunchKernelGGL(k_bc_outlet, dim3(grid_for(m.nbnd)), dim3(kBlock), 0, s, m,
                        (const BcScal *)sc, BU, BV, BW);
     CFD_LAUNCH_CHECK();
@@ -915,15 +915,15 @@ void launch_correct_ruface(const DevMesh &m, const double *R, const double *Apr,
 void launch_pressure_update(const DevMesh &m, const double *x, int ref, double urf_p,
                             const double *Apr, const double *gx, double *P, double *U,
                             double *V, double *W, double *res_slot, Red red, hipStream_t s) {
-    hipLaunchKernelGGL(k_pressure_update, dim3(red_grid(m.ncel)), dim3(kBlock), 0, s, m, x, ref,
-                       urf_p, Apr, gx, P, U, V, W, res_slot, red.part, red.tk);
+    CFD_REDUCE_LAUNCH(k_pressure_update, red_grid(m.ncel), s, m, x, ref,
+                       urf_p, Apr, gx, P, U, V, W, res_slot, red.part, fin);
     CFD_LAUNCH_CHECK();
 }
 
 void launch_take_solution(const DevMesh &m, const double *x, double *fld, double *res_slot,
                           double clip_lo, int clip, Red red, hipStream_t s) {
-    hipLaunchKernelGGL(k_take_solution, dim3(red_grid(m.ncel)), dim3(kBlock), 0, s, m.ncel,
-                       m.c_vol, x, fld, res_slot, clip_lo, clip, red.part, red.tk);
+    CFD_REDUCE_LAUNCH(k_take_solution, red_grid(m.ncel), s, m.ncel,
+                       m.c_vol, x, fld, res_slot, clip_lo, clip, red.part, fin);
     CFD_LAUNCH_CHECK();
 }
 
@@ -989,8 +989,8 @@ void launch_remove_mean(double *b, int n, Red red, hipStream_t s) {
     // the mean lands in the last reduction slot (part[12*kRedGrid-1] is never
     // a partial: reductions use at most 12*gridDim.x < 12*kRedGrid slots)
     double *mean = red.part + 12 * kRedGrid - 1;
-    hipLaunchKernelGGL(k_sum, dim3(std::min(red_grid(n), kRedGrid - 1)), dim3(kBlock), 0, s, n,
-                       (const double *)b, mean, red.part, red.tk);
+    CFD_REDUCE_LAUNCH(k_sum, std::min(red_grid(n), kRedGrid - 1), s, n,
+                       (const double *)b, mean, red.part, fin);
     hipLaunchKernelGGL(k_sub_scalar, dim3(grid_for(n)), dim3(kBlock), 0, s, n,
                        (const double *)mean, b);
     CFD_LAUNCH_CHECK();

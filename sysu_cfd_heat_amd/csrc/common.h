@@ -116,42 +116,52 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
         }
 }
 
-// Grid-wide deterministic reduction: each block writes its partials; the
-// last block to arrive (atomic ticket) sums them in block order and returns
-// true in thread 0 with the totals in v. part must hold NV*gridDim.x doubles.
+// Grid-wide deterministic reduction in two launches of the same kernel (a
+// last-workgroup ticket would need a device-scope release fence per
+// workgroup, i.e. an L2 write-back on every XCD, which costs more than the
+// data it reduces). Producer launch (fin == 0): every workgroup writes its NV
+// block sums to part[k * gridDim.x + blockIdx.x]; returns false. Finishing
+// launch (one workgroup, fin = the producer's grid size): the workgroup sums
+// the fin partials of each value in fixed order and returns true in thread 0
+// with the totals in v. Results are bit-reproducible for a fixed grid size.
+// part must hold NV * (producer grid) doubles.
 template <int NV>
-__device__ __forceinline__ bool grid_sum(double (&v)[NV], double *part, unsigned *ticket) {
-    block_sum<NV>(v);
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
+__device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *part, int fin) {
+    if (fin == 0) {
+        block_sum<NV>(v);
+        if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < NV; ++k) part[k * gridDim.x + blockIdx.x] = v[k];
-        __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
-        last = (t == gridDim.x - 1);
+            for (int k = 0; k < NV; ++k) part[k * gridDim.x + blockIdx.x] = v[k];
+        return false;
     }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc[k] = 0.;
-    // all threads of the last block cooperate: strided partial sums, then a
-    // fixed-order block reduction (deterministic for a fixed grid size)
-    // agent-scope loads bypass this CU's L1, which may not hold other
-    // blocks' freshly written partials
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+    for (int b = threadIdx.x; b < fin; b += blockDim.x)
 #pragma unroll
-        for (int k = 0; k < NV; ++k)
-            acc[k] += __hip_atomic_load(&part[k * gridDim.x + b], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < NV; ++k) acc[k] += part[k * fin + b];
     block_sum<NV>(acc);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
         for (int k = 0; k < NV; ++k) v[k] = acc[k];
-        *ticket = 0u;
-    }
     return threadIdx.x == 0;
 }
+
+// Grid-stride loop of a reducing kernel (parameter `fin` in scope): empty in
+// the finishing launch.
+#define RED_LOOP(i, n)                                                              \
+    for (int i = fin ? (int)(n) : (int)(blockIdx.x * blockDim.x + threadIdx.x); i < (n); \
+         i += gridDim.x * blockDim.x)
+
+// Launch a reducing kernel: producer on G workgroups, then the finishing
+// workgroup. The argument list names `fin` where the kernel takes it.
+#define CFD_REDUCE_LAUNCH(kern, G, s, ...)                                     \
+    do {                                                                       \
+        const int g_ = (G);                                                    \
+        int fin = 0;                                                           \
+        hipLaunchKernelGGL(kern, dim3(g_), dim3(::cfd::kBlock), 0, s, __VA_ARGS__); \
+        fin = g_;                                                              \
+        hipLaunchKernelGGL(kern, dim3(1), dim3(::cfd::kBlock), 0, s, __VA_ARGS__); \
+    } while (0)
 
 }  // namespace cfd

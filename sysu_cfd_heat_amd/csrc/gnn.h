@@ -1,0 +1,92 @@
+// Learned pressure initial guess (see include/cfd_gnn.h for the network).
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "cfd_gnn.h"
+#include "common.h"
+#include "assembly.h"
+#include "krylov.h"
+
+namespace cfd {
+
+constexpr int GIN = 16;  // input features padded to one MFMA K-step
+constexpr int GH = CFD_GNN_H;
+constexpr int GNL = CFD_GNN_LAYERS;
+
+// parameter offsets in the packed FP32 vector
+struct GnnLayout {
+    static constexpr size_t W0 = 0, C0 = W0 + CFD_GNN_IN * GH;
+    static constexpr size_t L0 = C0 + GH;                  // first message layer
+    static constexpr size_t LSZ = 2 * GH * GH + GH;        // W_l [128][64] + c_l
+    static constexpr size_t WOUT = L0 + GNL * LSZ, COUT = WOUT + GH;
+    static constexpr size_t N = COUT + 1;
+};
+
+// One graph (cell graph of a pressure matrix) with inputs and target.
+struct GnnGraph {
+    int n = 0, K = 0;
+    const int *col = nullptr;   // [K][n] device
+    const float *w = nullptr;   // [K][n] device, message weights
+    const float *feat = nullptr;  // [n][8] device
+    const float *t = nullptr;   // [n] device target (may be null)
+};
+
+// One captured (system, solution) pair: buf = [w: K*n][feat: n*8][t: n]
+struct GnnSample {
+    int graph = 0;  // index of the structure (col) it uses
+    DBuf<float> buf;
+};
+
+class Gnn {
+  public:
+    explicit Gnn(uint64_t seed);
+    std::vector<float> params;  // host master copy
+    hipStream_t s = nullptr;
+    // device parameters
+    DBuf<float> dparams, grad, adam_m, adam_v;
+    DBuf<uint16_t> wbf;  // BF16 weights: W0t [64][16], W_l [64][128] x3 (transposed)
+    int step = 0;
+    // activations for one graph
+    int cap_n = 0;
+    DBuf<uint16_t> fbf, hbf[GNL + 1], mbf[GNL], dz;
+    DBuf<float> y, dh, dx, part, scal;
+    // data set
+    std::vector<DBuf<int>> cols;
+    std::vector<int> col_n, col_K;
+    std::vector<GnnSample> samples;
+    GnnGraph sample_graph(const GnnSample &S) const;
+
+    void ensure(int n);
+    // run on another stream (the caller's) for the duration of one call
+    struct OnStream {
+        Gnn &g;
+        hipStream_t old;
+        OnStream(Gnn &g_, hipStream_t s_) : g(g_), old(g_.s) { g.s = s_; }
+        ~OnStream() { g.s = old; }
+    };
+    void upload_params();
+    void forward(const GnnGraph &g);
+    double loss_grad(const GnnGraph &g);  // forward + backward, grad filled
+    void adam(float lr);
+};
+
+// Galerkin initial guess x = a y + c p minimising the A-norm error over
+// span{y, p} (either may be null) for symmetric definite or semi-definite A.
+// Ay, Ap: work vectors. The 2x2 system is solved on the device (no host
+// round trip); st: 2 doubles of device scratch receive (a, c).
+void galerkin_guess(Krylov &kry, const Ell &A, const double *b, const double *y, const double *p,
+                    double *x, double *Ay, double *Ap, double *st, Red red, hipStream_t s);
+// y64 = y32 with its mean removed
+void gnn_output_to_guess(int n, const float *y32, double *y64, Red red, hipStream_t s);
+
+// Inputs of one pressure system, on the device: features and message weights.
+// st: 4 doubles of device scratch for the statistics.
+void gnn_features(int n, int K, const double *diag, const double *val, const double *b,
+                  const double *pprev, const double *vol, const double *const cx[3],
+                  const double lo[3], const double hi[3], float *feat, float *w, double *st,
+                  Red red, hipStream_t s);
+// t = (x - mean x) / rms(x - mean x); st: 2 doubles of device scratch
+void gnn_target(int n, const double *x, float *t, double *st, Red red, hipStream_t s);
+
+}  // namespace cfd

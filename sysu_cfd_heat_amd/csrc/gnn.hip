@@ -1,0 +1,888 @@
+// Learned pressure initial guess: message-passing graph network with the
+// per-cell MLPs on BF16 MFMA (v_mfma_f32_16x16x16_bf16, FP32 accumulation),
+// gather-based message passing (and its transpose for the backward pass),
+// deterministic weight-gradient reduction, Adam, and the feature / target
+// kernels that turn a pressure system into network inputs.
+//
+// MFMA 16x16x16 BF16 operand layout (wave64, lane l, r = l & 15, q = l >> 4):
+//   A (16x16, rows x k):  lane holds A[r][4q .. 4q+3]
+//   B (16x16, k x cols):  lane holds B[4q .. 4q+3][r]
+//   C/D (16x16):          lane holds C[4q .. 4q+3][r]
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+
+#include "gnn.h"
+#include "navier.h"
+
+namespace cfd {
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+uint16_t f2bf_host(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+#define GLOOP(i, n) \
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+
+// ------------------------------------------------------------ forward --
+// Y[c][0:64] = relu(Xa[c][0:KA] Wa + Xb[c][0:KIN-KA] Wb + bias). wt is the
+// transposed BF16 weight [64][KIN]. 64 cells per workgroup, 16 per wave.
+template <int KA, int KIN>
+__global__ __launch_bounds__(256) void k_gnn_fwd(int n, const uint16_t *__restrict__ Xa,
+                                                 const uint16_t *__restrict__ Xb,
+                                                 const uint16_t *__restrict__ wt,
+                                                 const float *__restrict__ bias,
+                                                 uint16_t *__restrict__ Y) {
+    constexpr int SK = KIN + 4;  // padded LDS row: spreads banks
+    __shared__ __attribute__((aligned(16))) uint16_t sW[64 * SK];
+    for (int t = threadIdx.x; t < 64 * KIN; t += 256) sW[(t / KIN) * SK + t % KIN] = wt[t];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, q = lane >> 4;
+    const int row0 = blockIdx.x * 64 + wave * 16;
+    const int cell = row0 + r;
+    f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < KIN; k0 += 16) {
+        bf16x4 a = bf16x4{0, 0, 0, 0};
+        if (cell < n) {
+            const uint16_t *src = k0 < KA ? Xa + (size_t)cell * KA + k0 + 4 * q
+                                          : Xb + (size_t)cell * (KIN - KA) + (k0 - KA) + 4 * q;
+            a = *(const bf16x4 *)src;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            bf16x4 b = *(const bf16x4 *)&sW[(16 * t + r) * SK + k0 + 4 * q];
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int c = row0 + 4 * q + j, col = 16 * t + r;
+            if (c < n) {
+                float z = acc[t][j] + bias[col];
+                Y[(size_t)c * 64 + col] = f2bf(z > 0.f ? z : 0.f);
+            }
+        }
+}
+
+// y[c] = h[c] . w + c0 (FP32 output head)
+__global__ void k_gnn_out(int n, const uint16_t *__restrict__ h, const float *__restrict__ w,
+                          const float *__restrict__ c0, float *__restrict__ y) {
+    GLOOP(c, n) {
+        float s = 0.f;
+        for (int o = 0; o < 64; ++o) s += bf2f(h[(size_t)c * 64 + o]) * w[o];
+        y[c] = s + c0[0];
+    }
+}
+
+// m[i][o] = sum_k w[k][i] h[col[k][i]][o]; 8 threads per cell, 8 features each
+template <class TIN, class TOUT>
+__global__ void k_gnn_msg(int n, int K, const int *__restrict__ col, const float *__restrict__ w,
+                          const TIN *__restrict__ h, TOUT *__restrict__ m, int accumulate) {
+    GLOOP(t, n * 8) {
+        const int i = t >> 3, o0 = (t & 7) * 8;
+        float s[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s[j] = 0.f;
+        for (int k = 0; k < K; ++k) {
+            const size_t ok = (size_t)k * n + i;
+            const float wk = w[ok];
+            const int c = col[ok];
+            if (wk == 0.f || c == i) continue;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                float hv;
+                if constexpr (sizeof(TIN) == 2) hv = bf2f(h[(size_t)c * 64 + o0 + j]);
+                else hv = h[(size_t)c * 64 + o0 + j];
+                s[j] += wk * hv;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            size_t o = (size_t)i * 64 + o0 + j;
+            if constexpr (sizeof(TOUT) == 2) m[o] = f2bf(s[j]);
+            else m[o] = accumulate ? m[o] + s[j] : s[j];
+        }
+    }
+}
+
+// rw[k][j] = w[s][i] with i = col[k][j], col[s][i] = j (transposed weights)
+__global__ void k_gnn_rev(int n, int K, const int *__restrict__ col, const float *__restrict__ w,
+                          float *__restrict__ rw) {
+    GLOOP(j, n) {
+        for (int k = 0; k < K; ++k) {
+            const int i = col[(size_t)k * n + j];
+            float v = 0.f;
+            if (i != j)
+                for (int s = 0; s < K; ++s)
+                    if (col[(size_t)s * n + i] == j) {
+                        v = w[(size_t)s * n + i];
+                        break;
+                    }
+            rw[(size_t)k * n + j] = v;
+        }
+    }
+}
+
+// ----------------------------------------------------------- backward --
+// dz[c][o] = bf16( dh[c][o] * (h[c][o] > 0) )
+__global__ void k_gnn_mask(int n, const float *__restrict__ dh, const uint16_t *__restrict__ h,
+                           uint16_t *__restrict__ dz) {
+    GLOOP(t, n * 64) dz[t] = f2bf(bf2f(h[t]) > 0.f ? dh[t] : 0.f);
+}
+
+// dXa[c][0:KA] = dZ[c] Wa^T, dXb[c][0:KIN-KA] = dZ[c] Wb^T; wn = BF16 W [KIN][64]
+template <int KA, int KIN>
+__global__ __launch_bounds__(256) void k_gnn_dx(int n, const uint16_t *__restrict__ dZ,
+                                                const uint16_t *__restrict__ wn,
+                                                float *__restrict__ dXa, float *__restrict__ dXb) {
+    constexpr int SK = 64 + 4;
+    __shared__ __attribute__((aligned(16))) uint16_t sW[KIN * SK];
+    for (int t = threadIdx.x; t < KIN * 64; t += 256) sW[(t / 64) * SK + t % 64] = wn[t];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, q = lane >> 4;
+    const int row0 = blockIdx.x * 64 + wave * 16;
+    const int cell = row0 + r;
+    constexpr int NT = KIN / 16;
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < 64; k0 += 16) {
+        bf16x4 a = bf16x4{0, 0, 0, 0};
+        if (cell < n) a = *(const bf16x4 *)(dZ + (size_t)cell * 64 + k0 + 4 * q);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            bf16x4 b = *(const bf16x4 *)&sW[(16 * t + r) * SK + k0 + 4 * q];
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int c = row0 + 4 * q + j, f = 16 * t + r;
+            if (c >= n) continue;
+            if (f < KA) dXa[(size_t)c * KA + f] = acc[t][j];
+            else dXb[(size_t)c * (KIN - KA) + f - KA] = acc[t][j];
+        }
+}
+
+// per-workgroup partial of dW[KIN][64] = sum_c X[c]^T dZ[c] over 64 cells
+template <int KA, int KIN>
+__global__ __launch_bounds__(256) void k_gnn_dw(int n, const uint16_t *__restrict__ Xa,
+                                                const uint16_t *__restrict__ Xb,
+                                                const uint16_t *__restrict__ dZ,
+                                                float *__restrict__ part) {
+    constexpr int CH = 64, SC = CH + 4;
+    __shared__ __attribute__((aligned(16))) uint16_t sX[KIN * SC];
+    __shared__ __attribute__((aligned(16))) uint16_t sZ[64 * SC];
+    const int c0 = blockIdx.x * CH;
+    for (int t = threadIdx.x; t < CH * KIN; t += 256) {
+        int c = t / KIN, f = t % KIN, cell = c0 + c;
+        uint16_t v = 0;
+        if (cell < n)
+            v = f < KA ? Xa[(size_t)cell * KA + f] : Xb[(size_t)cell * (KIN - KA) + f - KA];
+        sX[f * SC + c] = v;
+    }
+    for (int t = threadIdx.x; t < CH * 64; t += 256) {
+        int c = t / 64, o = t % 64, cell = c0 + c;
+        sZ[o * SC + c] = cell < n ? dZ[(size_t)cell * 64 + o] : (uint16_t)0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, q = lane >> 4;
+    constexpr int MT = KIN / 16, TILES = MT * 4;
+    float *out = part + (size_t)blockIdx.x * KIN * 64;
+    for (int tile = wave; tile < TILES; tile += 4) {
+        const int mt = tile / 4, nt = tile % 4;
+        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k0 = 0; k0 < CH; k0 += 16) {
+            bf16x4 a = *(const bf16x4 *)&sX[(16 * mt + r) * SC + k0 + 4 * q];
+            bf16x4 b = *(const bf16x4 *)&sZ[(16 * nt + r) * SC + k0 + 4 * q];
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) out[(size_t)(16 * mt + 4 * q + j) * 64 + 16 * nt + r] = acc[j];
+    }
+}
+
+// out[k] = sum over nb partial blocks (stride apart) of part[b*stride + k],
+// k < sz, summed in block order (deterministic)
+__global__ void k_gnn_reduce(int nb, int sz, int stride, const float *__restrict__ part,
+                             float *__restrict__ out) {
+    GLOOP(k, sz) {
+        float s = 0.f;
+        for (int b = 0; b < nb; ++b) s += part[(size_t)b * stride + k];
+        out[k] = s;
+    }
+}
+
+// per-block column sums of dZ (bias gradient), 64 columns
+__global__ void k_gnn_colsum(int n, const uint16_t *__restrict__ dZ, float *__restrict__ part) {
+    __shared__ float sh[4][64];
+    const int o = threadIdx.x & 63, g = threadIdx.x >> 6;
+    float s = 0.f;
+    for (int c = blockIdx.x * 256 + g; c < min(n, (int)(blockIdx.x + 1) * 256); c += 4)
+        s += bf2f(dZ[(size_t)c * 64 + o]);
+    sh[g][o] = s;
+    __syncthreads();
+    if (g == 0) part[(size_t)blockIdx.x * 64 + o] = sh[0][o] + sh[1][o] + sh[2][o] + sh[3][o];
+}
+
+// output head backward: dh[c][o] = dy[c] w[o]; partials of dw[o] and dc
+__global__ void k_gnn_out_bwd(int n, const float *__restrict__ dy, const uint16_t *__restrict__ h,
+                              const float *__restrict__ w, float *__restrict__ dh,
+                              float *__restrict__ part) {
+    __shared__ float sh[4][65];
+    const int o = threadIdx.x & 63, g = threadIdx.x >> 6;
+    float s = 0.f, sc = 0.f;
+    for (int c = blockIdx.x * 256 + g; c < min(n, (int)(blockIdx.x + 1) * 256); c += 4) {
+        float d = dy[c];
+        dh[(size_t)c * 64 + o] = d * w[o];
+        s += d * bf2f(h[(size_t)c * 64 + o]);
+        if (o == 0) sc += d;
+    }
+    sh[g][o] = s;
+    if (o == 0) sh[g][64] = sc;
+    __syncthreads();
+    if (g == 0) {
+        part[(size_t)blockIdx.x * 65 + o] = sh[0][o] + sh[1][o] + sh[2][o] + sh[3][o];
+        if (o == 0) part[(size_t)blockIdx.x * 65 + 64] = sh[0][64] + sh[1][64] + sh[2][64] + sh[3][64];
+    }
+}
+
+// loss = mean(e^2), e = (y - mean y) - (t - mean t); since e has zero mean,
+// d loss / d y = 2 e / n exactly
+__global__ void k_gnn_mean(int n, const float *__restrict__ y, const float *__restrict__ t,
+                           float *scal, double *part, int fin) {
+    double acc[2] = {0., 0.};
+    RED_LOOP(c, n) {
+        acc[0] += y[c];
+        acc[1] += t[c];
+    }
+    if (grid_reduce<2>(acc, part, fin)) {
+        scal[0] = (float)(acc[0] / n);
+        scal[2] = (float)(acc[1] / n);
+    }
+}
+__global__ void k_gnn_loss(int n, const float *__restrict__ y, const float *__restrict__ t,
+                           float *__restrict__ dy, float *scal, double *part, int fin) {
+    const float ym = scal[0], tm = scal[2];
+    double acc[1] = {0.};
+    RED_LOOP(c, n) {
+        float e = (y[c] - ym) - (t[c] - tm);
+        dy[c] = 2.f * e / n;
+        acc[0] += (double)e * e;
+    }
+    if (grid_reduce<1>(acc, part, fin)) scal[1] = (float)(acc[0] / n);
+}
+
+__global__ void k_gnn_adam(int np, float *__restrict__ p, const float *__restrict__ g,
+                           float *__restrict__ m, float *__restrict__ v, float lr, float b1c,
+                           float b2c) {
+    GLOOP(k, np) {
+        float gk = g[k];
+        float mk = 0.9f * m[k] + 0.1f * gk, vk = 0.999f * v[k] + 0.001f * gk * gk;
+        m[k] = mk;
+        v[k] = vk;
+        p[k] -= lr * (mk / b1c) / (sqrtf(vk / b2c) + 1e-8f);
+    }
+}
+
+// BF16 weight copies: transposed [64][KIN] (forward) and [KIN][64] (backward)
+__global__ void k_gnn_wcopy(const float *__restrict__ p, uint16_t *__restrict__ wt,
+                            uint16_t *__restrict__ wn) {
+    // layer 0: W0 [8][64] padded to [16][64]
+    GLOOP(t, 64 * GIN) {
+        int o = t / GIN, k = t % GIN;
+        uint16_t v = k < CFD_GNN_IN ? f2bf(p[GnnLayout::W0 + k * 64 + o]) : (uint16_t)0;
+        wt[t] = v;                      // [o][k]
+        wn[k * 64 + o] = v;             // [k][o]
+    }
+    for (int l = 0; l < GNL; ++l) {
+        const float *W = p + GnnLayout::L0 + l * GnnLayout::LSZ;
+        uint16_t *wtl = wt + 64 * GIN + l * 128 * 64, *wnl = wn + 64 * GIN + l * 128 * 64;
+        GLOOP(t, 64 * 128) {
+            int o = t / 128, k = t % 128;
+            uint16_t v = f2bf(W[k * 64 + o]);
+            wtl[t] = v;
+            wnl[k * 64 + o] = v;
+        }
+    }
+}
+
+// features of one pressure system (see include/cfd_gnn.h)
+__global__ void k_feat_stats(int n, const double *__restrict__ diag, const double *__restrict__ b,
+                             const double *__restrict__ pp, const double *__restrict__ vol,
+                             double *out, double *part, int fin) {
+    double acc[4] = {0., 0., 0., 0.};
+    RED_LOOP(i, n) {
+        acc[0] += vol[i];
+        acc[1] += b[i] * b[i];
+        acc[2] += pp ? pp[i] * pp[i] : 0.;
+        acc[3] += diag[i] * diag[i];
+    }
+    if (grid_reduce<4>(acc, part, fin)) {
+        out[0] = acc[0] / n;
+        out[1] = sqrt(acc[1] / n);
+        out[2] = sqrt(acc[2] / n);
+        out[3] = sqrt(acc[3] / n);
+    }
+}
+struct Box {
+    double lo[3], hi[3];
+    const double *cx[3];
+};
+__global__ void k_features(int n, int K, const double *__restrict__ diag,
+                           const double *__restrict__ val, const double *__restrict__ b,
+                           const double *__restrict__ pp, const double *__restrict__ vol, Box bx,
+                           const double *st, float *__restrict__ f, float *__restrict__ w) {
+    const double mv = st[0], rb = st[1], rp = st[2], rd = st[3];
+    GLOOP(i, n) {
+        float *fi = f + (size_t)i * CFD_GNN_IN;
+        fi[0] = (float)(vol[i] / mv);
+        for (int a = 0; a < 3; ++a)
+            fi[1 + a] = (float)((bx.cx[a][i] - bx.lo[a]) / (bx.hi[a] - bx.lo[a]));
+        fi[4] = rb > 0 ? (float)(b[i] / rb) : 0.f;
+        fi[5] = (pp && rp > 0) ? (float)(pp[i] / rp) : 0.f;
+        fi[6] = rd > 0 ? (float)(diag[i] / rd) : 0.f;
+        fi[7] = 1.f;
+        const double ad = fabs(diag[i]);
+        for (int k = 0; k < K; ++k) {
+            size_t o = (size_t)k * n + i;
+            w[o] = ad > 0 ? (float)(val[o] / ad) : 0.f;
+        }
+    }
+}
+__global__ void k_tmean(int n, const double *__restrict__ x, double *out, double *part,
+                        int fin) {
+    double acc[1] = {0.};
+    RED_LOOP(i, n) acc[0] += x[i];
+    if (grid_reduce<1>(acc, part, fin)) out[0] = acc[0] / n;
+}
+__global__ void k_tvar(int n, const double *__restrict__ x, double *out, double *part,
+                       int fin) {
+    const double mu = out[0];
+    double acc[1] = {0.};
+    RED_LOOP(i, n) acc[0] += (x[i] - mu) * (x[i] - mu);
+    if (grid_reduce<1>(acc, part, fin)) out[1] = sqrt(acc[0] / n);
+}
+__global__ void k_tnorm(int n, const double *__restrict__ x, const double *st, float *__restrict__ t) {
+    const double mu = st[0], sd = st[1];
+    GLOOP(i, n) t[i] = sd > 0 ? (float)((x[i] - mu) / sd) : 0.f;
+}
+__global__ void k_feat_pad(int n, const float *__restrict__ f, uint16_t *__restrict__ fb) {
+    GLOOP(t, n * GIN) {
+        int i = t / GIN, k = t % GIN;
+        fb[t] = k < CFD_GNN_IN ? f2bf(f[(size_t)i * CFD_GNN_IN + k]) : (uint16_t)0;
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- host --
+Gnn::Gnn(uint64_t seed) {
+    params.assign(GnnLayout::N, 0.f);
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<float> nd(0.f, 1.f);
+    auto init = [&](size_t off, int fan_in, int count) {
+        float sc = std::sqrt(2.f / fan_in);  // He initialisation
+        for (int k = 0; k < count; ++k) params[off + k] = sc * nd(rng);
+    };
+    init(GnnLayout::W0, CFD_GNN_IN, CFD_GNN_IN * GH);
+    for (int l = 0; l < GNL; ++l) init(GnnLayout::L0 + l * GnnLayout::LSZ, 2 * GH, 2 * GH * GH);
+    init(GnnLayout::WOUT, GH, GH);
+    CFD_HIP(hipStreamCreate(&s));
+    dparams.alloc(GnnLayout::N);
+    grad.alloc(GnnLayout::N);
+    adam_m.alloc(GnnLayout::N);
+    adam_v.alloc(GnnLayout::N);
+    adam_m.zero(s);
+    adam_v.zero(s);
+    wbf.alloc(2 * (64 * GIN + GNL * 128 * 64));
+    part.alloc(12 * kRedGrid);
+    scal.alloc(16);
+    upload_params();
+}
+
+void Gnn::upload_params() {
+    dparams.upload(params.data(), GnnLayout::N, s);
+    const size_t half = 64 * GIN + GNL * 128 * 64;
+    hipLaunchKernelGGL(k_gnn_wcopy, dim3(16), dim3(256), 0, s, (const float *)dparams.p, wbf.p,
+                       wbf.p + half);
+    CFD_LAUNCH_CHECK();
+    CFD_HIP(hipStreamSynchronize(s));
+}
+
+void Gnn::ensure(int n) {
+    if (n <= cap_n) return;
+    cap_n = n;
+    fbf.alloc((size_t)n * GIN);
+    for (int l = 0; l <= GNL; ++l) hbf[l].alloc((size_t)n * 64);
+    for (int l = 0; l < GNL; ++l) mbf[l].alloc((size_t)n * 64);
+    dz.alloc((size_t)n * 64);
+    y.alloc(n);
+    dh.alloc((size_t)n * 64 * 3);  // dh, dXa, dXb
+    dx.alloc((size_t)n * 64 + (size_t)n * 8);  // rw [K<=8][n] + dy
+    size_t nb = (n + 63) / 64;
+    if (part.n < nb * 128 * 64) part.alloc(nb * 128 * 64);
+}
+
+void Gnn::forward(const GnnGraph &g) {
+    const int n = g.n;
+    ensure(n);
+    const size_t half = 64 * GIN + GNL * 128 * 64;
+    const uint16_t *wt = wbf.p;
+    const float *P = dparams.p;
+    const int gb = (n + 63) / 64;
+    hipLaunchKernelGGL(k_feat_pad, dim3(grid_for((long long)n * GIN)), dim3(256), 0, s, n, g.feat,
+                       fbf.p);
+    hipLaunchKernelGGL((k_gnn_fwd<GIN, GIN>), dim3(gb), dim3(256), 0, s, n, (const uint16_t *)fbf.p,
+                       (const uint16_t *)nullptr, wt, P + GnnLayout::C0, hbf[0].p);
+    for (int l = 0; l < GNL; ++l) {
+        hipLaunchKernelGGL((k_gnn_msg<uint16_t, uint16_t>), dim3(grid_for((long long)n * 8)),
+                           dim3(256), 0, s, n, g.K, g.col, g.w, (const uint16_t *)hbf[l].p,
+                           mbf[l].p, 0);
+        hipLaunchKernelGGL((k_gnn_fwd<64, 128>), dim3(gb), dim3(256), 0, s, n,
+                           (const uint16_t *)hbf[l].p, (const uint16_t *)mbf[l].p,
+                           wt + 64 * GIN + l * 128 * 64,
+                           P + GnnLayout::L0 + l * GnnLayout::LSZ + 2 * GH * GH, hbf[l + 1].p);
+    }
+    hipLaunchKernelGGL(k_gnn_out, dim3(grid_for(n)), dim3(256), 0, s, n,
+                       (const uint16_t *)hbf[GNL].p, P + GnnLayout::WOUT, P + GnnLayout::COUT, y.p);
+    CFD_LAUNCH_CHECK();
+    (void)half;
+}
+
+double Gnn::loss_grad(const GnnGraph &g) {
+    const int n = g.n, K = g.K;
+    forward(g);
+    const size_t half = 64 * GIN + GNL * 128 * 64;
+    const uint16_t *wn = wbf.p + half;
+    float *G = grad.p;
+    float *dhp = dh.p, *dXa = dh.p + (size_t)n * 64, *dXb = dh.p + 2 * (size_t)n * 64;
+    float *rw = dx.p, *dy = dx.p + (size_t)n * 8;
+    const int gb = (n + 63) / 64, cb = (n + 255) / 256;
+    double *dpart = (double *)part.p;  // reductions reuse the partial buffer head
+    CFD_REDUCE_LAUNCH(k_gnn_mean, red_grid(n), s, n, (const float *)y.p,
+                       g.t, scal.p, dpart, fin);
+    CFD_REDUCE_LAUNCH(k_gnn_loss, red_grid(n), s, n, (const float *)y.p,
+                       g.t, dy, scal.p, dpart, fin);
+    // head
+    hipLaunchKernelGGL(k_gnn_out_bwd, dim3(cb), dim3(256), 0, s, n, (const float *)dy,
+                       (const uint16_t *)hbf[GNL].p, dparams.p + GnnLayout::WOUT, dhp, part.p);
+    hipLaunchKernelGGL(k_gnn_reduce, dim3(1), dim3(128), 0, s, cb, 65, 65, (const float *)part.p,
+                       G + GnnLayout::WOUT);
+    hipLaunchKernelGGL(k_gnn_rev, dim3(grid_for(n)), dim3(256), 0, s, n, K, g.col, g.w, rw);
+    for (int l = GNL - 1; l >= 0; --l) {
+        float *Gl = G + GnnLayout::L0 + l * GnnLayout::LSZ;
+        hipLaunchKernelGGL(k_gnn_mask, dim3(grid_for((long long)n * 64)), dim3(256), 0, s, n,
+                           (const float *)dhp, (const uint16_t *)hbf[l + 1].p, dz.p);
+        hipLaunchKernelGGL((k_gnn_dw<64, 128>), dim3(gb), dim3(256), 0, s, n,
+                           (const uint16_t *)hbf[l].p, (const uint16_t *)mbf[l].p,
+                           (const uint16_t *)dz.p, part.p);
+        hipLaunchKernelGGL(k_gnn_reduce, dim3(grid_for(128 * 64)), dim3(256), 0, s, gb, 128 * 64,
+                           128 * 64, (const float *)part.p, Gl);
+        hipLaunchKernelGGL(k_gnn_colsum, dim3(cb), dim3(256), 0, s, n, (const uint16_t *)dz.p,
+                           part.p);
+        hipLaunchKernelGGL(k_gnn_reduce, dim3(1), dim3(64), 0, s, cb, 64, 64,
+                           (const float *)part.p, Gl + 2 * GH * GH);
+        hipLaunchKernelGGL((k_gnn_dx<64, 128>), dim3(gb), dim3(256), 0, s, n,
+                           (const uint16_t *)dz.p, wn + 64 * GIN + l * 128 * 64, dXa, dXb);
+        // dh_l = dXa + transpose-message of dXb
+        CFD_HIP(hipMemcpyAsync(dhp, dXa, sizeof(float) * n * 64, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL((k_gnn_msg<float, float>), dim3(grid_for((long long)n * 8)), dim3(256),
+                           0, s, n, K, g.col, (const float *)rw, (const float *)dXb, dhp, 1);
+    }
+    hipLaunchKernelGGL(k_gnn_mask, dim3(grid_for((long long)n * 64)), dim3(256), 0, s, n,
+                       (const float *)dhp, (const uint16_t *)hbf[0].p, dz.p);
+    // layer 0 weights: the padded input has GIN = 16 columns, rows 8..15 unused
+    hipLaunchKernelGGL((k_gnn_dw<GIN, GIN>), dim3(gb), dim3(256), 0, s, n, (const uint16_t *)fbf.p,
+                       (const uint16_t *)nullptr, (const uint16_t *)dz.p, part.p);
+    // each block's partial is [16][64]; rows 8..15 belong to the zero padding
+    hipLaunchKernelGGL(k_gnn_reduce, dim3(grid_for(CFD_GNN_IN * 64)), dim3(256), 0, s, gb,
+                       CFD_GNN_IN * 64, GIN * 64, (const float *)part.p, G + GnnLayout::W0);
+    hipLaunchKernelGGL(k_gnn_colsum, dim3(cb), dim3(256), 0, s, n, (const uint16_t *)dz.p,
+                       part.p);
+    hipLaunchKernelGGL(k_gnn_reduce, dim3(1), dim3(64), 0, s, cb, 64, 64, (const float *)part.p,
+                       G + GnnLayout::C0);
+    CFD_LAUNCH_CHECK();
+    float sc[2];
+    CFD_HIP(hipMemcpyAsync(sc, scal.p, sizeof(sc), hipMemcpyDeviceToHost, s));
+    CFD_HIP(hipStreamSynchronize(s));
+    return sc[1];
+}
+
+void Gnn::adam(float lr) {
+    ++step;
+    float b1c = 1.f - std::pow(0.9f, (float)step), b2c = 1.f - std::pow(0.999f, (float)step);
+    hipLaunchKernelGGL(k_gnn_adam, dim3(grid_for(GnnLayout::N)), dim3(256), 0, s,
+                       (int)GnnLayout::N, dparams.p, (const float *)grad.p, adam_m.p, adam_v.p, lr,
+                       b1c, b2c);
+    const size_t half = 64 * GIN + GNL * 128 * 64;
+    hipLaunchKernelGGL(k_gnn_wcopy, dim3(16), dim3(256), 0, s, (const float *)dparams.p, wbf.p,
+                       wbf.p + half);
+    CFD_LAUNCH_CHECK();
+}
+
+void gnn_features(int n, int K, const double *diag, const double *val, const double *b,
+                  const double *pprev, const double *vol, const double *const cx[3],
+                  const double lo[3], const double hi[3], float *feat, float *w, double *st,
+                  Red red, hipStream_t s) {
+    CFD_REDUCE_LAUNCH(k_feat_stats, red_grid(n), s, n, diag, b, pprev, vol,
+                       st, red.part, fin);
+    Box bx;
+    for (int a = 0; a < 3; ++a) {
+        bx.lo[a] = lo[a];
+        bx.hi[a] = hi[a] > lo[a] ? hi[a] : lo[a] + 1.;
+        bx.cx[a] = cx[a];
+    }
+    hipLaunchKernelGGL(k_features, dim3(grid_for(n)), dim3(256), 0, s, n, K, diag, val, b, pprev,
+                       vol, bx, (const double *)st, feat, w);
+    CFD_LAUNCH_CHECK();
+}
+
+void gnn_target(int n, const double *x, float *t, double *st, Red red, hipStream_t s) {
+    CFD_REDUCE_LAUNCH(k_tmean, red_grid(n), s, n, x, st, red.part, fin);
+    CFD_REDUCE_LAUNCH(k_tvar, red_grid(n), s, n, x, st, red.part, fin);
+    hipLaunchKernelGGL(k_tnorm, dim3(grid_for(n)), dim3(256), 0, s, n, x, (const double *)st, t);
+    CFD_LAUNCH_CHECK();
+}
+
+namespace {
+__global__ void k_galerkin_dots(int n, const double *__restrict__ b, const double *__restrict__ y,
+                                const double *__restrict__ Ay, const double *__restrict__ p,
+                                const double *__restrict__ Ap, double *st, double *part,
+                                int fin) {
+    double acc[5] = {0., 0., 0., 0., 0.};
+    RED_LOOP(i, n) {
+        const double yi = y ? y[i] : 0., ayi = y ? Ay[i] : 0.;
+        const double pi = p ? p[i] : 0., api = p ? Ap[i] : 0.;
+        acc[0] += yi * ayi;
+        acc[1] += yi * api;
+        acc[2] += pi * api;
+        acc[3] += yi * b[i];
+        acc[4] += pi * b[i];
+    }
+    if (grid_reduce<5>(acc, part, fin)) {
+        const double yy = acc[0], yp = acc[1], pp = acc[2], yb = acc[3], pb = acc[4];
+        const double det = yy * pp - yp * yp;
+        double a = 0., c = 0.;
+        if (yy != 0. && pp != 0. && fabs(det) > 1e-12 * fabs(yy * pp)) {
+            a = (yb * pp - pb * yp) / det;
+            c = (yy * pb - yp * yb) / det;
+        } else if (yy != 0.) {
+            a = yb / yy;
+        } else if (pp != 0.) {
+            c = pb / pp;
+        }
+        if (!isfinite(a) || !isfinite(c)) a = c = 0.;  // a useless guess falls back to zero
+        st[0] = a;
+        st[1] = c;
+    }
+}
+__global__ void k_galerkin_x(int n, const double *__restrict__ y, const double *__restrict__ p,
+                             const double *st, double *__restrict__ x) {
+    const double a = st[0], c = st[1];
+    GLOOP(i, n) x[i] = (y ? a * y[i] : 0.) + (p ? c * p[i] : 0.);
+}
+__global__ void k_f32_to_f64(int n, const float *__restrict__ a, double *__restrict__ b) {
+    GLOOP(i, n) b[i] = a[i];
+}
+}  // namespace
+
+void galerkin_guess(Krylov &kry, const Ell &A, const double *b, const double *y, const double *p,
+                    double *x, double *Ay, double *Ap, double *st, Red red, hipStream_t s) {
+    const int n = A.n;
+    if (y) kry.spmv(A, y, Ay);
+    if (p) kry.spmv(A, p, Ap);
+    CFD_REDUCE_LAUNCH(k_galerkin_dots, red_grid(n), s, n, b, y,
+                       (const double *)Ay, p, (const double *)Ap, st, red.part, fin);
+    hipLaunchKernelGGL(k_galerkin_x, dim3(grid_for(n)), dim3(256), 0, s, n, y, p,
+                       (const double *)st, x);
+    CFD_LAUNCH_CHECK();
+}
+
+void gnn_output_to_guess(int n, const float *y32, double *y64, Red red, hipStream_t s) {
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(grid_for(n)), dim3(256), 0, s, n, y32, y64);
+    launch_remove_mean(y64, n, red, s);
+    CFD_LAUNCH_CHECK();
+}
+
+GnnGraph Gnn::sample_graph(const GnnSample &S) const {
+    GnnGraph g;
+    g.n = col_n[S.graph];
+    g.K = col_K[S.graph];
+    g.col = cols[S.graph].p;
+    g.w = S.buf.p;
+    g.feat = S.buf.p + (size_t)g.K * g.n;
+    g.t = g.feat + (size_t)g.n * CFD_GNN_IN;
+    return g;
+}
+
+// ------------------------------------------- solver-side guess / capture --
+namespace {
+void mesh_box(const HostMesh &m, double lo[3], double hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = 1e300;
+        hi[a] = -1e300;
+    }
+    for (int i = 0; i < m.ncel; ++i)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], m.cell_x[3 * (size_t)i + a]);
+            hi[a] = std::max(hi[a], m.cell_x[3 * (size_t)i + a]);
+        }
+}
+}  // namespace
+
+// Features of the current pressure system into (feat, w).
+static void ns_features(NSSolver &ns, float *feat, float *w) {
+    double lo[3], hi[3];
+    mesh_box(ns.hm, lo, hi);
+    const DevMesh &m = ns.dm();
+    gnn_features(ns.hm.ncel, m.K, ns.Ap_diag, ns.Ap_val, ns.bp, ns.have_pprev ? ns.pprev.p : nullptr,
+                 m.c_vol, m.c_x, lo, hi, feat, w, ns.gst.p, ns.red(), ns.s);
+}
+
+void NSSolver::pressure_guess() {
+    const int n = hm.ncel;
+    if (guess_mode == 0 && !cap_net) return;
+    pprev.ensure(n);
+    gst.ensure(32);
+    if (guess_mode == 0) return;
+    const double *y = nullptr, *p = have_pprev ? pprev.p : nullptr;
+    if (guess_mode == 2) {
+        if (!guess_net) throw HipError("guess mode 2 needs a network");
+        if (dm().K > 8) throw HipError("the network takes at most 8 neighbour slots");
+        gfeat.ensure((size_t)n * CFD_GNN_IN);
+        gw.ensure((size_t)dm().K * n);
+        gy.ensure(n);
+        ns_features(*this, gfeat, gw);
+        GnnGraph g;
+        g.n = n;
+        g.K = dm().K;
+        g.col = dm().ell_col;
+        g.w = gw;
+        g.feat = gfeat;
+        {
+            Gnn::OnStream on(*guess_net, s);
+            guess_net->forward(g);
+        }
+        gnn_output_to_guess(n, guess_net->y, gy, red(), s);
+        y = gy;
+    }
+    if (!y && !p) return;  // x0 stays zero
+    gAy.ensure(n);
+    gAp.ensure(n);
+    galerkin_guess(*kry, Ap(), bp, y, p, xsol, gAy, gAp, gst.p + 8, red(), s);
+}
+
+void NSSolver::pressure_capture() {
+    const int n = hm.ncel;
+    if (cap_net && (int)cap_net->samples.size() < cap_max) {
+        if (dm().K > 8) throw HipError("the network takes at most 8 neighbour slots");
+        Gnn &g = *cap_net;
+        const int K = dm().K;
+        if (cap_graph < 0) {
+            DBuf<int> c((size_t)K * n);
+            CFD_HIP(hipMemcpyAsync(c.p, dm().ell_col, sizeof(int) * K * n, hipMemcpyDeviceToDevice,
+                                   s));
+            g.cols.push_back(std::move(c));
+            g.col_n.push_back(n);
+            g.col_K.push_back(K);
+            cap_graph = (int)g.cols.size() - 1;
+        }
+        GnnSample S;
+        S.graph = cap_graph;
+        S.buf.alloc((size_t)K * n + (size_t)n * (CFD_GNN_IN + 1));
+        float *w = S.buf.p, *feat = w + (size_t)K * n, *t = feat + (size_t)n * CFD_GNN_IN;
+        ns_features(*this, feat, w);
+        gnn_target(n, xsol, t, gst.p + 4, red(), s);
+        CFD_HIP(hipStreamSynchronize(s));
+        g.samples.push_back(std::move(S));
+    }
+    if (guess_mode != 0 || cap_net) {
+        CFD_HIP(hipMemcpyAsync(pprev.p, xsol.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        have_pprev = true;
+    }
+}
+
+}  // namespace cfd
+
+// ============================================================ C-ABI ==
+using namespace cfd;
+struct cfd_gnn_s {
+    Gnn g;
+    explicit cfd_gnn_s(uint64_t seed) : g(seed) {}
+};
+
+namespace {
+template <class F>
+int gguard(F f) {
+    try {
+        return f();
+    } catch (const std::exception &e) {
+        last_error() = e.what();
+        return -1;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+cfd_gnn *cfd_gnn_create(uint64_t seed) {
+    try {
+        return new cfd_gnn(seed);
+    } catch (const std::exception &e) {
+        last_error() = e.what();
+        return nullptr;
+    }
+}
+void cfd_gnn_free(cfd_gnn *g) { delete g; }
+size_t cfd_gnn_num_params(void) { return GnnLayout::N; }
+int cfd_gnn_get_params(const cfd_gnn *g, float *out) {
+    return gguard([&] {
+        std::vector<float> tmp(GnnLayout::N);
+        g->g.dparams.download(tmp.data(), GnnLayout::N, g->g.s);
+        std::memcpy(out, tmp.data(), sizeof(float) * GnnLayout::N);
+        return 0;
+    });
+}
+int cfd_gnn_set_params(cfd_gnn *g, const float *in) {
+    return gguard([&] {
+        std::memcpy(g->g.params.data(), in, sizeof(float) * GnnLayout::N);
+        g->g.upload_params();
+        return 0;
+    });
+}
+
+static int host_graph(cfd_gnn *g, int n, int K, const int *col, const float *w, const float *feat,
+                      const float *t, DBuf<int> &dc, DBuf<float> &dw, DBuf<float> &df,
+                      DBuf<float> &dt, GnnGraph &gr) {
+    if (K > 8) throw HipError("GNN graphs take at most 8 neighbour slots");
+    hipStream_t s = g->g.s;
+    dc.alloc((size_t)K * n);
+    dw.alloc((size_t)K * n);
+    df.alloc((size_t)n * CFD_GNN_IN);
+    dc.upload(col, (size_t)K * n, s);
+    dw.upload(w, (size_t)K * n, s);
+    df.upload(feat, (size_t)n * CFD_GNN_IN, s);
+    if (t) {
+        dt.alloc(n);
+        dt.upload(t, n, s);
+    }
+    gr.n = n;
+    gr.K = K;
+    gr.col = dc.p;
+    gr.w = dw.p;
+    gr.feat = df.p;
+    gr.t = t ? dt.p : nullptr;
+    return 0;
+}
+
+int cfd_gnn_forward_host(cfd_gnn *g, int n, int K, const int *col, const float *w,
+                         const float *feat, float *y) {
+    return gguard([&] {
+        DBuf<int> dc;
+        DBuf<float> dw, df, dt;
+        GnnGraph gr;
+        host_graph(g, n, K, col, w, feat, nullptr, dc, dw, df, dt, gr);
+        g->g.forward(gr);
+        g->g.y.download(y, n, g->g.s);
+        return 0;
+    });
+}
+
+int cfd_gnn_loss_grad_host(cfd_gnn *g, int n, int K, const int *col, const float *w,
+                           const float *feat, const float *t, float *loss, float *grad) {
+    return gguard([&] {
+        DBuf<int> dc;
+        DBuf<float> dw, df, dt;
+        GnnGraph gr;
+        host_graph(g, n, K, col, w, feat, t, dc, dw, df, dt, gr);
+        *loss = (float)g->g.loss_grad(gr);
+        g->g.grad.download(grad, GnnLayout::N, g->g.s);
+        return 0;
+    });
+}
+
+int cfd_gnn_num_samples(const cfd_gnn *g) { return (int)g->g.samples.size(); }
+
+int cfd_gnn_train(cfd_gnn *gh, int epochs, float lr, uint64_t seed, double out[2]) {
+    return gguard([&] {
+        Gnn &g = gh->g;
+        if (g.samples.empty()) throw HipError("no captured samples");
+        std::mt19937_64 rng(seed);
+        std::vector<int> order(g.samples.size());
+        for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+        for (int e = 0; e < epochs; ++e) {
+            std::shuffle(order.begin(), order.end(), rng);
+            double tot = 0.;
+            for (int k : order) {
+                tot += g.loss_grad(g.sample_graph(g.samples[k]));
+                g.adam(lr);
+            }
+            tot /= order.size();
+            if (e == 0) out[0] = tot;
+            out[1] = tot;
+        }
+        CFD_HIP(hipStreamSynchronize(g.s));
+        std::vector<float> p(GnnLayout::N);
+        g.dparams.download(p.data(), GnnLayout::N, g.s);
+        g.params = p;
+        return 0;
+    });
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int cfd_ns_capture(cfd_ns *h, cfd_gnn *g, int max_samples) {
+    return gguard([&] {
+        NSSolver &n = h->ns;
+        Gnn *net = g ? &g->g : nullptr;
+        if (net != n.cap_net) n.cap_graph = -1;
+        n.cap_net = net;
+        n.cap_max = max_samples;
+        return 0;
+    });
+}
+
+int cfd_ns_set_guess(cfd_ns *h, cfd_gnn *g, int mode) {
+    return gguard([&] {
+        if (mode < 0 || mode > 2) throw HipError("guess mode must be 0, 1 or 2");
+        if (mode == 2 && !g) throw HipError("guess mode 2 needs a network");
+        h->ns.guess_mode = mode;
+        h->ns.guess_net = g ? &g->g : nullptr;
+        return 0;
+    });
+}
+
+}  // extern "C"

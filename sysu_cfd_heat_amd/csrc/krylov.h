@@ -95,7 +95,6 @@ class Krylov {
     KWork<double> wd_;
     KWork<float> wf_;
     DBuf<double> rt_, v_, s_, t_, ph_, sh_, w0_, w1_, V_, part_;
-    DBuf<unsigned> ticket_;
     DBuf<KState> state_;
     KState *hstate_ = nullptr;
     std::vector<hipEvent_t> evs_;

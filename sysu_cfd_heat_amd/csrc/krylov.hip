@@ -51,10 +51,10 @@ __global__ void k_spmv(EllT<T> A, const T *__restrict__ x, T *__restrict__ y) {
 template <class T>
 __global__ void k_residual(EllT<T> A, const T *__restrict__ b, const T *__restrict__ x,
                            T *__restrict__ r, T *__restrict__ zero1, T *__restrict__ zero2,
-                           KState *st, double *part, unsigned *tk, int mode, double eps,
+                           KState *st, double *part, int fin, int mode, double eps,
                            int maxit) {
     double acc[3] = {0., 0., 0.};
-    GRID_LOOP(i, A.n) {
+    RED_LOOP(i, A.n) {
         T ri = b[i] - row_dot(A, i, x);
         r[i] = ri;
         if (zero1) zero1[i] = T(0);
@@ -63,7 +63,7 @@ __global__ void k_residual(EllT<T> A, const T *__restrict__ b, const T *__restri
         acc[1] += (double)b[i] * b[i];
         acc[2] += (double)ri * (ri / A.diag[i]);
     }
-    if (grid_sum<3>(acc, part, tk)) {
+    if (grid_reduce<3>(acc, part, fin)) {
         // mode 0: plain residual (no state change); 1: CG start;
         // 3: BiCGSTAB start; 2: GMRES (re)start
         if (mode == 0) {
@@ -143,12 +143,12 @@ __global__ void k_add_f(int n, double *__restrict__ x, const float *__restrict__
 template <bool JAC, class T>
 __global__ void k_cg_pq(EllT<T> A, const T *__restrict__ r, const T *__restrict__ z,
                         const T *__restrict__ pold, T *__restrict__ pnew, T *__restrict__ q,
-                        KState *st, double *part, unsigned *tk) {
+                        KState *st, double *part, int fin) {
     if (st->done) return;
     const T beta = (T)st->beta;
     const int n = A.n;
     double acc[1] = {0.};
-    GRID_LOOP(i, n) {
+    RED_LOOP(i, n) {
         T zi = JAC ? r[i] / A.diag[i] : z[i];
         T pi = zi + beta * pold[i];
         T s = A.diag[i] * pi;
@@ -162,7 +162,7 @@ __global__ void k_cg_pq(EllT<T> A, const T *__restrict__ r, const T *__restrict_
         q[i] = s;
         acc[0] += (double)pi * s;
     }
-    if (grid_sum<1>(acc, part, tk)) {
+    if (grid_reduce<1>(acc, part, fin)) {
         st->pq = acc[0];
         st->alpha = st->rho / acc[0];
     }
@@ -172,18 +172,18 @@ __global__ void k_cg_pq(EllT<T> A, const T *__restrict__ r, const T *__restrict_
 template <bool JAC, class T>
 __global__ void k_cg_update(int n, T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
                             const T *__restrict__ q, const T *__restrict__ d, KState *st,
-                            double *part, unsigned *tk) {
+                            double *part, int fin) {
     if (st->done) return;
     const T alpha = (T)st->alpha;
     double acc[2] = {0., 0.};
-    GRID_LOOP(i, n) {
+    RED_LOOP(i, n) {
         x[i] += alpha * p[i];
         T ri = r[i] - alpha * q[i];
         r[i] = ri;
         acc[0] += (double)ri * ri;
         if (JAC) acc[1] += (double)ri * (ri / d[i]);
     }
-    if (grid_sum<2>(acc, part, tk)) {
+    if (grid_reduce<2>(acc, part, fin)) {
         st->iter += 1;
         st->rho_old = st->rho;
         st->rr = acc[0];
@@ -199,11 +199,11 @@ __global__ void k_cg_update(int n, T *__restrict__ x, T *__restrict__ r, const T
 // rho = r.z ; beta = rho / rho_old (0 on the first iteration)
 template <class T>
 __global__ void k_cg_rho(int n, const T *__restrict__ r, const T *__restrict__ z, KState *st,
-                         double *part, unsigned *tk) {
+                         double *part, int fin) {
     if (st->done) return;
     double acc[1] = {0.};
-    GRID_LOOP(i, n) acc[0] += (double)r[i] * z[i];
-    if (grid_sum<1>(acc, part, tk)) {
+    RED_LOOP(i, n) acc[0] += (double)r[i] * z[i];
+    if (grid_reduce<1>(acc, part, fin)) {
         st->rho = acc[0];
         st->beta = st->iter == 0 ? 0. : acc[0] / st->rho_old;
     }
@@ -229,16 +229,16 @@ __global__ void k_bi_p(int n, const double *__restrict__ r, double *__restrict__
 template <int MODE>
 __global__ void k_bi_spmv(Ell A, const double *__restrict__ x, double *__restrict__ y,
                           const double *__restrict__ other, KState *st, double *part,
-                          unsigned *tk) {
+                          int fin) {
     if (st->done || (MODE == 1 && st->half)) return;
     double acc[2] = {0., 0.};
-    GRID_LOOP(i, A.n) {
+    RED_LOOP(i, A.n) {
         double s = row_dot(A, i, x);
         y[i] = s;
         acc[0] += MODE == 0 ? other[i] * s : s * other[i];
         if (MODE == 1) acc[1] += s * s;
     }
-    if (grid_sum<2>(acc, part, tk)) {
+    if (grid_reduce<2>(acc, part, fin)) {
         if (MODE == 0) st->alpha = st->rho / acc[0];
         else st->omega = acc[0] / acc[1];
     }
@@ -248,28 +248,28 @@ __global__ void k_bi_spmv(Ell A, const double *__restrict__ x, double *__restric
 template <bool JAC>
 __global__ void k_bi_s(int n, const double *__restrict__ r, const double *__restrict__ v,
                        double *__restrict__ s, double *__restrict__ sh,
-                       const double *__restrict__ d, KState *st, double *part, unsigned *tk) {
+                       const double *__restrict__ d, KState *st, double *part, int fin) {
     if (st->done) return;
     const double alpha = st->alpha;
     double acc[1] = {0.};
-    GRID_LOOP(i, n) {
+    RED_LOOP(i, n) {
         double si = r[i] - alpha * v[i];
         s[i] = si;
         if (JAC) sh[i] = si / d[i];
         acc[0] += si * si;
     }
-    if (grid_sum<1>(acc, part, tk)) st->half = rtc(sqrt(acc[0]), st->bnorm, st->eps);
+    if (grid_reduce<1>(acc, part, fin)) st->half = rtc(sqrt(acc[0]), st->bnorm, st->eps);
 }
 
 __global__ void k_bi_x(int n, double *__restrict__ x, double *__restrict__ r,
                        const double *__restrict__ ph, const double *__restrict__ sh,
                        const double *__restrict__ s, const double *__restrict__ t,
-                       const double *__restrict__ rt, KState *st, double *part, unsigned *tk) {
+                       const double *__restrict__ rt, KState *st, double *part, int fin) {
     if (st->done) return;
     const double alpha = st->alpha, om = st->omega;
     const bool half = st->half;
     double acc[2] = {0., 0.};
-    GRID_LOOP(i, n) {
+    RED_LOOP(i, n) {
         double ri;
         if (half) {
             x[i] += alpha * ph[i];
@@ -282,7 +282,7 @@ __global__ void k_bi_x(int n, double *__restrict__ x, double *__restrict__ r,
         acc[0] += ri * ri;
         acc[1] += rt[i] * ri;
     }
-    if (grid_sum<2>(acc, part, tk)) {
+    if (grid_reduce<2>(acc, part, fin)) {
         st->iter += 1;
         st->rr = acc[0];
         st->rnorm = sqrt(acc[0]);
@@ -329,19 +329,19 @@ __global__ void k_gm_spmv(Ell A, const double *__restrict__ z, double *__restric
 
 // h[t] = w . V_t, t = 0..j
 __global__ void k_gm_dot(int n, const double *__restrict__ w, const double *__restrict__ V,
-                         KState *st, double *part, unsigned *tk) {
+                         KState *st, double *part, int fin) {
     if (st->done || st->cycle_done) return;
     const int j = st->j;
     double acc[12];
 #pragma unroll
     for (int t = 0; t < 12; ++t) acc[t] = 0.;
-    GRID_LOOP(i, n) {
+    RED_LOOP(i, n) {
         double wi = w[i];
 #pragma unroll
         for (int t = 0; t < 12; ++t)
             if (t <= j) acc[t] += wi * V[(size_t)t * n + i];
     }
-    if (grid_sum<12>(acc, part, tk))
+    if (grid_reduce<12>(acc, part, fin))
         for (int t = 0; t < 12; ++t) st->h[t] = t <= j ? acc[t] : 0.;
 }
 
@@ -349,7 +349,7 @@ __global__ void k_gm_dot(int n, const double *__restrict__ w, const double *__re
 // Givens update of the Hessenberg column
 template <int PASS>
 __global__ void k_gm_sub(int n, double *__restrict__ w, const double *__restrict__ V,
-                         KState *st, double *part, unsigned *tk) {
+                         KState *st, double *part, int fin) {
     if (st->done || st->cycle_done) return;
     const int j = st->j;
     double c[12];
@@ -358,7 +358,7 @@ __global__ void k_gm_sub(int n, double *__restrict__ w, const double *__restrict
     double acc[12];
 #pragma unroll
     for (int t = 0; t < 12; ++t) acc[t] = 0.;
-    GRID_LOOP(i, n) {
+    RED_LOOP(i, n) {
         double wi = w[i];
 #pragma unroll
         for (int t = 0; t < 12; ++t)
@@ -372,7 +372,7 @@ __global__ void k_gm_sub(int n, double *__restrict__ w, const double *__restrict
             acc[0] += wi * wi;
         }
     }
-    if (!grid_sum<12>(acc, part, tk)) return;
+    if (!grid_reduce<12>(acc, part, fin)) return;
     if (PASS == 1) {
         for (int t = 0; t < 12; ++t) st->h2[t] = t <= j ? acc[t] : 0.;
         return;
@@ -638,8 +638,6 @@ template struct KWork<double>;
 template struct KWork<float>;
 
 Krylov::Krylov(hipStream_t s) : st_(s) {
-    ticket_.alloc(1);
-    ticket_.zero(st_);
     state_.alloc(1);
     part_.alloc(12 * kRedGrid);
     CFD_HIP(hipHostMalloc((void **)&hstate_, sizeof(KState), hipHostMallocDefault));
@@ -701,9 +699,9 @@ void Krylov::spmv(const Ell &A, const double *x, double *y) {
 
 double Krylov::relres(const Ell &A, const double *x, const double *b) {
     ensure(A.n, A.K);
-    hipLaunchKernelGGL(k_residual<double>, dim3(red_grid(A.n)), dim3(kBlock), 0, st_, A, b, x,
+    CFD_REDUCE_LAUNCH(k_residual<double>, red_grid(A.n), st_, A, b, x,
                        wd_.tmp.p, (double *)nullptr, (double *)nullptr, state_.p, part_.p,
-                       ticket_.p, 0, 0., 0);
+                       fin, 0, 0., 0);
     CFD_LAUNCH_CHECK();
     read_state();
     double bn = std::sqrt(hstate_->bb), rn = std::sqrt(hstate_->rr);
@@ -863,9 +861,9 @@ SolveStats Krylov::solve_mixed(const Ell &A0, const double *b0, double *x0,
     double rel = 0.;
     for (int outer = 0; outer < 100; ++outer) {
         // FP64 residual r = b - A x
-        hipLaunchKernelGGL(k_residual<double>, dim3(red_grid(n)), dim3(kBlock), 0, st_, A, b,
+        CFD_REDUCE_LAUNCH(k_residual<double>, red_grid(n), st_, A, b,
                            (const double *)x, wd_.r.p, (double *)nullptr, (double *)nullptr,
-                           state_.p, part_.p, ticket_.p, 0, 0., 0);
+                           state_.p, part_.p, fin, 0, 0., 0);
         CFD_LAUNCH_CHECK();
         read_state();
         double bn = std::sqrt(hstate_->bb), rn = std::sqrt(hstate_->rr);
@@ -902,12 +900,12 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
                       Colouring *col, int pc, KWork<T> &w) {
     const int n = A.n, G = red_grid(n);
     const bool jac = pc == CFD_PC_JACOBI;
-    hipLaunchKernelGGL(k_residual<T>, dim3(G), dim3(kBlock), 0, st_, A, b, (const T *)x, w.r.p,
-                       w.p0.p, w.p1.p, state_.p, part_.p, ticket_.p, 1, cfg.tol, cfg.maxit);
+    CFD_REDUCE_LAUNCH(k_residual<T>, G, st_, A, b, (const T *)x, w.r.p,
+                       w.p0.p, w.p1.p, state_.p, part_.p, fin, 1, cfg.tol, cfg.maxit);
     if (!jac) {
         apply_pc(A, pc, cfg.omega, (const T *)w.r.p, w.z.p, col, w);
-        hipLaunchKernelGGL(k_cg_rho<T>, dim3(G), dim3(kBlock), 0, st_, n, (const T *)w.r.p,
-                           (const T *)w.z.p, state_.p, part_.p, ticket_.p);
+        CFD_REDUCE_LAUNCH(k_cg_rho<T>, G, st_, n, (const T *)w.r.p,
+                           (const T *)w.z.p, state_.p, part_.p, fin);
     }
     CFD_LAUNCH_CHECK();
     int batch = check_first, issued = 0, prev_it = 0;
@@ -918,29 +916,29 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
         for (int k = 0; k < batch; ++k) {
             if (prof) CFD_HIP(hipEventRecord(ev(2 * nev), st_));
             if (jac)
-                hipLaunchKernelGGL((k_cg_pq<true, T>), dim3(G), dim3(kBlock), 0, st_, A,
+                CFD_REDUCE_LAUNCH((k_cg_pq<true, T>), G, st_, A,
                                    (const T *)w.r.p, (const T *)nullptr, (const T *)pa, pb, w.q.p,
-                                   state_.p, part_.p, ticket_.p);
+                                   state_.p, part_.p, fin);
             else
-                hipLaunchKernelGGL((k_cg_pq<false, T>), dim3(G), dim3(kBlock), 0, st_, A,
+                CFD_REDUCE_LAUNCH((k_cg_pq<false, T>), G, st_, A,
                                    (const T *)w.r.p, (const T *)w.z.p, (const T *)pa, pb, w.q.p,
-                                   state_.p, part_.p, ticket_.p);
+                                   state_.p, part_.p, fin);
             if (prof) {
                 CFD_HIP(hipEventRecord(ev(2 * nev + 1), st_));
                 ++nev;
             }
             if (jac) {
-                hipLaunchKernelGGL((k_cg_update<true, T>), dim3(G), dim3(kBlock), 0, st_, n, x,
+                CFD_REDUCE_LAUNCH((k_cg_update<true, T>), G, st_, n, x,
                                    w.r.p, (const T *)pb, (const T *)w.q.p, (const T *)A.diag,
-                                   state_.p, part_.p, ticket_.p);
+                                   state_.p, part_.p, fin);
             } else {
-                hipLaunchKernelGGL((k_cg_update<false, T>), dim3(G), dim3(kBlock), 0, st_, n, x,
+                CFD_REDUCE_LAUNCH((k_cg_update<false, T>), G, st_, n, x,
                                    w.r.p, (const T *)pb, (const T *)w.q.p, (const T *)A.diag,
-                                   state_.p, part_.p, ticket_.p);
+                                   state_.p, part_.p, fin);
                 apply_pc(A, pc, cfg.omega, (const T *)w.r.p, w.z.p, col, w);
-                hipLaunchKernelGGL(k_cg_rho<T>, dim3(G), dim3(kBlock), 0, st_, n,
+                CFD_REDUCE_LAUNCH(k_cg_rho<T>, G, st_, n,
                                    (const T *)w.r.p, (const T *)w.z.p, state_.p, part_.p,
-                                   ticket_.p);
+                                   fin);
             }
             std::swap(pa, pb);
         }
@@ -981,8 +979,8 @@ SolveStats Krylov::bicgstab(const Ell &A, const double *b, double *x, const cfd_
                             Colouring *col, int pc) {
     const int n = A.n, G = red_grid(n), g = grid_for(n);
     const bool jac = pc == CFD_PC_JACOBI;
-    hipLaunchKernelGGL(k_residual<double>, dim3(G), dim3(kBlock), 0, st_, A, b, (const double *)x, wd_.r.p,
-                       wd_.p0.p, v_.p, state_.p, part_.p, ticket_.p, 3, cfg.tol, cfg.maxit);
+    CFD_REDUCE_LAUNCH(k_residual<double>, G, st_, A, b, (const double *)x, wd_.r.p,
+                       wd_.p0.p, v_.p, state_.p, part_.p, fin, 3, cfg.tol, cfg.maxit);
     hipLaunchKernelGGL(k_copy, dim3(g), dim3(kBlock), 0, st_, n, (const double *)wd_.r.p, rt_.p);
     CFD_LAUNCH_CHECK();
     int batch = check_first, issued = 0, prev_it = 0;
@@ -997,20 +995,20 @@ SolveStats Krylov::bicgstab(const Ell &A, const double *b, double *x, const cfd_
                                    v_.p, ph_.p, A.diag, state_.p);
                 apply_pc(A, pc, cfg.omega, (const double *)wd_.p0.p, ph_.p, col, wd_);
             }
-            hipLaunchKernelGGL(k_bi_spmv<0>, dim3(G), dim3(kBlock), 0, st_, A, ph_.p, v_.p, rt_.p,
-                               state_.p, part_.p, ticket_.p);
+            CFD_REDUCE_LAUNCH(k_bi_spmv<0>, G, st_, A, ph_.p, v_.p, rt_.p,
+                               state_.p, part_.p, fin);
             if (jac) {
-                hipLaunchKernelGGL(k_bi_s<true>, dim3(G), dim3(kBlock), 0, st_, n, wd_.r.p, v_.p,
-                                   s_.p, sh_.p, A.diag, state_.p, part_.p, ticket_.p);
+                CFD_REDUCE_LAUNCH(k_bi_s<true>, G, st_, n, wd_.r.p, v_.p,
+                                   s_.p, sh_.p, A.diag, state_.p, part_.p, fin);
             } else {
-                hipLaunchKernelGGL(k_bi_s<false>, dim3(G), dim3(kBlock), 0, st_, n, wd_.r.p, v_.p,
-                                   s_.p, sh_.p, A.diag, state_.p, part_.p, ticket_.p);
+                CFD_REDUCE_LAUNCH(k_bi_s<false>, G, st_, n, wd_.r.p, v_.p,
+                                   s_.p, sh_.p, A.diag, state_.p, part_.p, fin);
                 apply_pc(A, pc, cfg.omega, (const double *)s_.p, sh_.p, col, wd_);
             }
-            hipLaunchKernelGGL(k_bi_spmv<1>, dim3(G), dim3(kBlock), 0, st_, A, sh_.p, t_.p, s_.p,
-                               state_.p, part_.p, ticket_.p);
-            hipLaunchKernelGGL(k_bi_x, dim3(G), dim3(kBlock), 0, st_, n, x, wd_.r.p, ph_.p, sh_.p,
-                               s_.p, t_.p, rt_.p, state_.p, part_.p, ticket_.p);
+            CFD_REDUCE_LAUNCH(k_bi_spmv<1>, G, st_, A, sh_.p, t_.p, s_.p,
+                               state_.p, part_.p, fin);
+            CFD_REDUCE_LAUNCH(k_bi_x, G, st_, n, x, wd_.r.p, ph_.p, sh_.p,
+                               s_.p, t_.p, rt_.p, state_.p, part_.p, fin);
         }
         CFD_LAUNCH_CHECK();
         issued += batch;
@@ -1038,8 +1036,8 @@ SolveStats Krylov::gmres(const Ell &A, const double *b, double *x, const cfd_sol
         CFD_HIP(hipStreamSynchronize(st_));
     }
     double *W = w0_.p, *Wn = w1_.p;
-    hipLaunchKernelGGL(k_residual<double>, dim3(G), dim3(kBlock), 0, st_, A, b, (const double *)x, W,
-                       (double *)nullptr, (double *)nullptr, state_.p, part_.p, ticket_.p, 2,
+    CFD_REDUCE_LAUNCH(k_residual<double>, G, st_, A, b, (const double *)x, W,
+                       (double *)nullptr, (double *)nullptr, state_.p, part_.p, fin, 2,
                        cfg.tol, cfg.maxit);
     CFD_LAUNCH_CHECK();
     read_state();
@@ -1056,12 +1054,12 @@ SolveStats Krylov::gmres(const Ell &A, const double *b, double *x, const cfd_sol
                 hipLaunchKernelGGL(k_gm_spmv, dim3(g), dim3(kBlock), 0, st_, A, wd_.z.p, Wn,
                                    state_.p);
             }
-            hipLaunchKernelGGL(k_gm_dot, dim3(G), dim3(kBlock), 0, st_, n, Wn, V_.p, state_.p,
-                               part_.p, ticket_.p);
-            hipLaunchKernelGGL(k_gm_sub<1>, dim3(G), dim3(kBlock), 0, st_, n, Wn, V_.p,
-                               state_.p, part_.p, ticket_.p);
-            hipLaunchKernelGGL(k_gm_sub<2>, dim3(G), dim3(kBlock), 0, st_, n, Wn, V_.p,
-                               state_.p, part_.p, ticket_.p);
+            CFD_REDUCE_LAUNCH(k_gm_dot, G, st_, n, Wn, V_.p, state_.p,
+                               part_.p, fin);
+            CFD_REDUCE_LAUNCH(k_gm_sub<1>, G, st_, n, Wn, V_.p,
+                               state_.p, part_.p, fin);
+            CFD_REDUCE_LAUNCH(k_gm_sub<2>, G, st_, n, Wn, V_.p,
+                               state_.p, part_.p, fin);
             std::swap(W, Wn);
         }
         hipLaunchKernelGGL(k_gm_y, dim3(1), dim3(64), 0, st_, state_.p);
@@ -1076,8 +1074,8 @@ SolveStats Krylov::gmres(const Ell &A, const double *b, double *x, const cfd_sol
         }
         W = w0_.p;
         Wn = w1_.p;
-        hipLaunchKernelGGL(k_residual<double>, dim3(G), dim3(kBlock), 0, st_, A, b, (const double *)x, W,
-                           (double *)nullptr, (double *)nullptr, state_.p, part_.p, ticket_.p, 2,
+        CFD_REDUCE_LAUNCH(k_residual<double>, G, st_, A, b, (const double *)x, W,
+                           (double *)nullptr, (double *)nullptr, state_.p, part_.p, fin, 2,
                            cfg.tol, -1);
         CFD_LAUNCH_CHECK();
         read_state();

@@ -29,12 +29,12 @@ struct MVec {  // per-system vectors
 #define MLOOP(i, n) \
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 
-__global__ void k_m_init(Ell A, MVec V, int nb, KState *st, double *part, unsigned *tk,
+__global__ void k_m_init(Ell A, MVec V, int nb, KState *st, double *part, int fin,
                          double eps, int maxit) {
     double acc[2 * NB];
     for (int k = 0; k < 2 * NB; ++k) acc[k] = 0.;
     const int n = A.n;
-    MLOOP(i, n) {
+    RED_LOOP(i, n) {
         double s[NB];
         const double d = A.diag[i];
         for (int q = 0; q < NB; ++q) if (q < nb) s[q] = d * V.x[q][i];
@@ -54,7 +54,7 @@ __global__ void k_m_init(Ell A, MVec V, int nb, KState *st, double *part, unsign
             acc[NB + q] += V.b[q][i] * V.b[q][i];
         }
     }
-    if (grid_sum<2 * NB>(acc, part, tk))
+    if (grid_reduce<2 * NB>(acc, part, fin))
         for (int q = 0; q < NB; ++q) if (q < nb) {
             KState &S = st[q];
             S.eps = eps;
@@ -99,13 +99,13 @@ __global__ void k_m_p(int n, MVec V, int nb, const double *__restrict__ d, const
 
 // MODE 0: v = A ph, rt.v -> alpha ; MODE 1: t = A sh, (t.s, t.t) -> omega
 template <int MODE>
-__global__ void k_m_spmv(Ell A, MVec V, int nb, KState *st, double *part, unsigned *tk) {
+__global__ void k_m_spmv(Ell A, MVec V, int nb, KState *st, double *part, int fin) {
     bool act[NB];
     for (int q = 0; q < NB; ++q) if (q < nb) act[q] = !st[q].done && (MODE == 0 || !st[q].half);
     double acc[2 * NB];
     for (int k = 0; k < 2 * NB; ++k) acc[k] = 0.;
     const int n = A.n;
-    MLOOP(i, n) {
+    RED_LOOP(i, n) {
         double s[NB];
         const double d = A.diag[i];
         for (int q = 0; q < NB; ++q) if (q < nb) s[q] = act[q] ? d * (MODE == 0 ? V.ph[q][i] : V.sh[q][i]) : 0.;
@@ -128,7 +128,7 @@ __global__ void k_m_spmv(Ell A, MVec V, int nb, KState *st, double *part, unsign
             }
         }
     }
-    if (grid_sum<2 * NB>(acc, part, tk))
+    if (grid_reduce<2 * NB>(acc, part, fin))
         for (int q = 0; q < NB; ++q) if (q < nb) {
             if (!act[q]) continue;
             if (MODE == 0) st[q].alpha = st[q].rho / acc[q];
@@ -137,7 +137,7 @@ __global__ void k_m_spmv(Ell A, MVec V, int nb, KState *st, double *part, unsign
 }
 
 __global__ void k_m_s(int n, MVec V, int nb, const double *__restrict__ d, KState *st,
-                      double *part, unsigned *tk) {
+                      double *part, int fin) {
     bool act[NB];
     double al[NB];
     for (int q = 0; q < NB; ++q) if (q < nb) {
@@ -146,7 +146,7 @@ __global__ void k_m_s(int n, MVec V, int nb, const double *__restrict__ d, KStat
     }
     double acc[NB];
     for (int q = 0; q < NB; ++q) acc[q] = 0.;
-    MLOOP(i, n) {
+    RED_LOOP(i, n) {
         const double di = d[i];
         for (int q = 0; q < NB; ++q) if (q < nb) {
             if (!act[q]) continue;
@@ -156,12 +156,12 @@ __global__ void k_m_s(int n, MVec V, int nb, const double *__restrict__ d, KStat
             acc[q] += si * si;
         }
     }
-    if (grid_sum<NB>(acc, part, tk))
+    if (grid_reduce<NB>(acc, part, fin))
         for (int q = 0; q < NB; ++q) if (q < nb)
             if (act[q]) st[q].half = rtc_m(sqrt(acc[q]), st[q].bnorm, st[q].eps);
 }
 
-__global__ void k_m_x(int n, MVec V, int nb, KState *st, double *part, unsigned *tk) {
+__global__ void k_m_x(int n, MVec V, int nb, KState *st, double *part, int fin) {
     bool act[NB], half[NB];
     double al[NB], om[NB];
     for (int q = 0; q < NB; ++q) if (q < nb) {
@@ -172,7 +172,7 @@ __global__ void k_m_x(int n, MVec V, int nb, KState *st, double *part, unsigned 
     }
     double acc[2 * NB];
     for (int k = 0; k < 2 * NB; ++k) acc[k] = 0.;
-    MLOOP(i, n) {
+    RED_LOOP(i, n) {
         for (int q = 0; q < NB; ++q) if (q < nb) {
             if (!act[q]) continue;
             double ri;
@@ -188,7 +188,7 @@ __global__ void k_m_x(int n, MVec V, int nb, KState *st, double *part, unsigned 
             acc[NB + q] += V.rt[q][i] * ri;
         }
     }
-    if (grid_sum<2 * NB>(acc, part, tk))
+    if (grid_reduce<2 * NB>(acc, part, fin))
         for (int q = 0; q < NB; ++q) if (q < nb) {
             if (!act[q]) continue;
             KState &S = st[q];
@@ -229,22 +229,22 @@ void Krylov::solve_multi(const Ell &A, int nb, const double *const *b, double *c
         V.x[q] = q < nb ? x[q] : nullptr;
         V.b[q] = q < nb ? b[q] : nullptr;
     }
-    hipLaunchKernelGGL(k_m_init, dim3(G), dim3(kBlock), 0, st_, A, V, nb, mstate_.p, part_.p,
-                       ticket_.p, cfg.tol, cfg.maxit);
+    CFD_REDUCE_LAUNCH(k_m_init, G, st_, A, V, nb, mstate_.p, part_.p,
+                       fin, cfg.tol, cfg.maxit);
     CFD_LAUNCH_CHECK();
     int batch = check_first, issued = 0;
     for (;;) {
         for (int k = 0; k < batch; ++k) {
             hipLaunchKernelGGL(k_m_p, dim3(g), dim3(kBlock), 0, st_, n, V, nb,
                                (const double *)A.diag, (const KState *)mstate_.p);
-            hipLaunchKernelGGL(k_m_spmv<0>, dim3(G), dim3(kBlock), 0, st_, A, V, nb, mstate_.p,
-                               part_.p, ticket_.p);
-            hipLaunchKernelGGL(k_m_s, dim3(G), dim3(kBlock), 0, st_, n, V, nb,
-                               (const double *)A.diag, mstate_.p, part_.p, ticket_.p);
-            hipLaunchKernelGGL(k_m_spmv<1>, dim3(G), dim3(kBlock), 0, st_, A, V, nb, mstate_.p,
-                               part_.p, ticket_.p);
-            hipLaunchKernelGGL(k_m_x, dim3(G), dim3(kBlock), 0, st_, n, V, nb, mstate_.p,
-                               part_.p, ticket_.p);
+            CFD_REDUCE_LAUNCH(k_m_spmv<0>, G, st_, A, V, nb, mstate_.p,
+                               part_.p, fin);
+            CFD_REDUCE_LAUNCH(k_m_s, G, st_, n, V, nb,
+                               (const double *)A.diag, mstate_.p, part_.p, fin);
+            CFD_REDUCE_LAUNCH(k_m_spmv<1>, G, st_, A, V, nb, mstate_.p,
+                               part_.p, fin);
+            CFD_REDUCE_LAUNCH(k_m_x, G, st_, n, V, nb, mstate_.p,
+                               part_.p, fin);
         }
         CFD_LAUNCH_CHECK();
         issued += batch;

@@ -24,6 +24,7 @@ struct DevMeshOwner {
 };
 
 class NSSolver;
+class Gnn;
 
 // HeatConductionSolver (src/HeatConductionSolver.h:5-64), steady
 class HTSolver {
@@ -68,19 +69,30 @@ class NSSolver {
     DBuf<double> As_diag, As_val, Ap_diag, Ap_val;
     DBuf<double> Prod, TES, EDS, ApTE, ApED, VisTE, VisED, zeros;
     DBuf<double> res_dev, gtmp, red_part;
-    DBuf<unsigned> red_tk;
     DBuf<BcScal> bcs;
     DBuf<int> errflag;
     double Residual[10] = {0};
     SolveStats last_u, last_v, last_w, last_p, last_k, last_e, last_t;
     std::string last_as;  // which system As holds ("velocity", "k", "eps", "T")
     double stats[8] = {0};
+
+    // pressure-correction initial guess (no counterpart in the reference,
+    // which starts from x0 = 0, src/navier_pressure.cpp:21)
+    Gnn *guess_net = nullptr;  // borrowed; must outlive its use here
+    int guess_mode = 0;        // 0: zero, 1: previous p', 2: network + previous p'
+    Gnn *cap_net = nullptr;    // data set receiving (system, p') pairs
+    int cap_max = 0, cap_graph = -1;
+    DBuf<double> pprev, gy, gAy, gAp, gst;
+    DBuf<float> gfeat, gw;
+    bool have_pprev = false;
+    void pressure_guess();
+    void pressure_capture();
     bool prof_pressure = false;  // live timing of the pressure CG SpMV kernel
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     ~NSSolver();
     int setup(const cfd_raw_mesh &raw, const cfd_params &p, std::string &err);
-    Red red() { return Red{red_part.p, red_tk.p}; }
+    Red red() { return Red{red_part.p}; }
     const DevMesh &dm() const { return dmo.dm; }
     Ell As() { return Ell{hm.ncel, dmo.dm.K, As_diag.p, As_val.p, dmo.dm.ell_col}; }
     Ell Ap() { return Ell{hm.ncel, dmo.dm.K, Ap_diag.p, Ap_val.p, dmo.dm.ell_col}; }
@@ -110,3 +122,7 @@ class NSSolver {
 };
 
 }  // namespace cfd
+
+struct cfd_ns_s {
+    cfd::NSSolver ns;
+};

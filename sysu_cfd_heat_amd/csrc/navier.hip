@@ -197,8 +197,6 @@ int NSSolver::setup(const cfd_raw_mesh &raw, const cfd_params &p, std::string &e
     res_dev.alloc(10);
     gtmp.alloc(std::max<size_t>(3 * C, 2 * (size_t)hm.nvrt) + 1);
     red_part.alloc(12 * kRedGrid);
-    red_tk.alloc(1);
-    red_tk.zero(s);
     bcs.alloc(1);
     errflag.alloc(1);
     errflag.zero(s);
@@ -358,6 +356,7 @@ int NSSolver::CalculatePressure() {
     launch_fill(xsol, hm.ncel, 0., s);
     BuildPressureMatrix();
     if (P.pres_project) launch_remove_mean(bp, hm.ncel, red(), s);
+    pressure_guess();
     kry->prof = prof_pressure;
     last_p = solve(Ap(), bp, xsol, P.pres, colouring_for(P.pres), P.pres_precision);
     kry->prof = false;
@@ -366,6 +365,7 @@ int NSSolver::CalculatePressure() {
         last_error() = "pressure iteration cannot converge";
         return -1;
     }
+    pressure_capture();
     SetBCDeltaP();
     const double *phi[1] = {xsol};
     const double *bphi[1] = {BP};
@@ -575,9 +575,6 @@ int HTSolver::UpdateEnergy() {
 // ============================================================ C-ABI ==
 using namespace cfd;
 
-struct cfd_ns_s {
-    NSSolver ns;
-};
 
 namespace {
 

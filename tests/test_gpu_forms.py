@@ -1,0 +1,57 @@
+"""The forms beside the plain FP64 drop-in, each against the FP64 path.
+
+Multi-RHS momentum: the three systems share every matrix read but keep
+their own recurrences and reductions, so u, v, w are bit-identical to three
+single solves. Mixed-precision pressure: FP32 inner CG, FP64 refinement; every
+solve ends at the same true FP64 relative residual as the FP64 solve, and the
+fields agree within that tolerance.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(raw, p, iters):
+    import sysu_cfd_heat_amd as P
+    s = P.NSSolver(raw, p)
+    res = []
+    for _ in range(iters):
+        s.call("Iterate")
+        res.append((s.last_solve("u"), s.last_solve("p")))
+    out = {nm: s.get(nm).copy() for nm in ("Un", "Vn", "Wn", "Pn")}
+    st = s.stats().copy()
+    s.close()
+    return out, res, st
+
+
+@pytest.mark.parametrize("tets", [False, True])
+def test_multi_rhs_bit_identical(tets):
+    from sysu_cfd_heat_amd import cases
+    print(f"[forms] multi-RHS tets={tets}", flush=True)
+    raw, p = cases.cavity(n=8, tets=tets, jitter=0.1 if tets else 0.0, seed=3, max_step=6)
+    p.multi_rhs = 0
+    a, ra, sa = _run(raw, p, 3)
+    p.multi_rhs = 1
+    b, rb, sb = _run(raw, p, 3)
+    for nm in a:
+        assert np.array_equal(a[nm], b[nm]), nm
+    assert ra == rb
+    assert sa[2] == sb[2]  # momentum iterations
+
+
+def test_mixed_precision_pressure():
+    from sysu_cfd_heat_amd import cases
+    print("[forms] mixed precision", flush=True)
+    raw, p = cases.channel(nx=12, ny=6, nz=6, max_step=6)
+    p.pres_precision = 0
+    a, ra, _ = _run(raw, p, 3)
+    p.pres_precision = 1
+    b, rb, _ = _run(raw, p, 3)
+    for (_, (it, rr)) in rb:
+        assert rr <= p.pres.tol, rr  # true FP64 residual, recomputed after refinement
+    for nm in a:
+        e = rel_l2(b[nm], a[nm])
+        assert e <= 1e-6, (nm, e)

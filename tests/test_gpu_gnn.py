@@ -1,0 +1,107 @@
+"""Learned pressure-correction guess on the device.
+
+Forward pass and loss gradient of the BF16-MFMA network against the oracle
+(oracle/gnn_ref.py, which rounds to bfloat16 where the device does and is
+itself pinned by finite differences). Tolerances: rel-L2 1e-2 on the output
+and 3e-2 per parameter block on the gradient (the device accumulates FP32
+sums in another order, so an activation can round to the neighbouring
+bfloat16 value). Then the whole loop: capture (system, p') pairs from the
+solver, train on the device, and solve held-out cases with the guess.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from oracle import gnn_ref as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _graph(nx, ny, nz, seed):
+    import sysu_cfd_heat_amd as P
+    raw = P.box_mesh(nx, ny, nz, tets=True, jitter=0.1, seed=seed)
+    cc = P.Mesh(raw).get("cell_cell")
+    n = cc.shape[0]
+    nb = [[int(c) for c in row if c >= 0] for row in cc]
+    K = max(len(r) for r in nb)
+    col = np.tile(np.arange(n, dtype=np.int32), (K, 1))
+    rng = np.random.default_rng(seed)
+    w = np.zeros((K, n), np.float32)
+    for i, r in enumerate(nb):
+        for k, c in enumerate(r):
+            col[k, i] = c
+            w[k, i] = -rng.uniform(0.05, 0.4)
+    feat = rng.standard_normal((n, G.IN)).astype(np.float32)
+    feat[:, 7] = 1.0
+    t = rng.standard_normal(n).astype(np.float32)
+    return col, w, feat, t
+
+
+@pytest.mark.parametrize("shape", [(5, 4, 3), (8, 6, 6)])
+def test_forward_and_gradient_match_oracle(shape):
+    import sysu_cfd_heat_amd as P
+    print(f"[gnn] forward/gradient shape={shape}", flush=True)
+    col, w, feat, t = _graph(*shape, seed=sum(shape))
+    params = (G.init_params(11) * 0.8).astype(np.float32)
+    params[G.OFF["c0"]:G.OFF["c0"] + G.H] = 0.02
+    net = P.Gnn(seed=1)
+    net.set_params(params)
+    assert np.array_equal(net.params(), params)
+    y = net.forward(col, w, feat)
+    yr, _ = G.forward(params, col, w, feat, bf16_mode=True)
+    assert rel_l2(y, yr) <= 1e-2, rel_l2(y, yr)
+    loss, g = net.loss_grad(col, w, feat, t)
+    lr, gr = G.loss_grad(params, col, w, feat, t, bf16_mode=True)
+    assert abs(loss - lr) <= 1e-2 * lr, (loss, lr)
+    for key, size in (("W0", G.IN * G.H), ("c0", G.H), ("W1", 2 * G.H * G.H), ("c1", G.H),
+                      ("W2", 2 * G.H * G.H), ("c2", G.H), ("W3", 2 * G.H * G.H),
+                      ("c3", G.H), ("wout", G.H)):
+        o = G.OFF[key]
+        e = rel_l2(g[o:o + size], gr[o:o + size])
+        print(f"[gnn] grad {key}: rel-L2 {e:.2e}", flush=True)
+        assert e <= 3e-2, (key, e)
+    # the loss is invariant to a constant shift of y: d loss / d c_out = 0
+    # exactly, so both sides hold only rounding noise
+    o = G.OFF["cout"]
+    scale = np.linalg.norm(gr[G.OFF["wout"]:o])
+    assert abs(g[o]) <= 1e-5 * scale and abs(gr[o]) <= 1e-5 * scale, (g[o], gr[o], scale)
+    # and the FP64 (unrounded) oracle says what bfloat16 costs
+    y64, _ = G.forward(params.astype(np.float64), col, w, feat, bf16_mode=False)
+    assert rel_l2(y, y64) <= 5e-2
+
+
+def test_capture_train_and_guess():
+    import sysu_cfd_heat_amd as P
+    from sysu_cfd_heat_amd import cases
+    nx, ny, nz, iters = 16, 8, 8, 8
+    net = P.Gnn(seed=3)
+    for u in (0.5, 1.0, 1.5, 2.0):
+        print(f"[gnn] capture u_in={u}", flush=True)
+        raw, p = cases.channel(nx=nx, ny=ny, nz=nz, max_step=iters + 2, u_in=u)
+        s = P.NSSolver(raw, p)
+        s.capture(net, 1000)
+        for _ in range(iters):
+            s.call("Iterate")
+        s.close()
+    assert net.num_samples() == 4 * iters
+    print("[gnn] train", flush=True)
+    first, last = net.train(epochs=40, lr=2e-3, seed=0)
+    assert np.isfinite(last) and last < 0.5 * first, (first, last)
+    res = {}
+    for mode in (0, 1, 2):
+        print(f"[gnn] held-out u_in=1.25 mode={mode}", flush=True)
+        raw, p = cases.channel(nx=nx, ny=ny, nz=nz, max_step=iters + 2, u_in=1.25)
+        s = P.NSSolver(raw, p)
+        s.set_guess(mode, net if mode == 2 else None)
+        for _ in range(iters):
+            s.call("Iterate")
+            it, rr = s.last_solve("p")
+            assert rr <= p.pres.tol, (mode, rr)  # the plain CG always finishes the job
+        res[mode] = (s.stats()[3], s.get("Un").copy(), s.get("Pn").copy())
+        s.close()
+    print("[gnn] pressure iterations: zero %d, previous p' %d, learned %d"
+          % (res[0][0], res[1][0], res[2][0]), flush=True)
+    assert res[2][0] <= res[0][0]
+    # every mode converges each solve to the same tolerance: the fields agree
+    for mode in (1, 2):
+        assert rel_l2(res[mode][1], res[0][1]) <= 1e-4

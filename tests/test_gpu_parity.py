@@ -43,7 +43,7 @@ def _cases():
     return {
         "cavity_hex": cases.cavity(n=6, max_step=4),
         "cavity_tet": cases.cavity(n=4, tets=True, jitter=0.1, seed=5, max_step=4),
-        "channel_cht": cases.channel(nx=8, ny=4, nz=4, tets=True, jitter=0.1, seed=2,
+        "channel_cht": cases.channel(nx=10, ny=4, nz=4, tets=True, jitter=0.1, seed=2,
                                      max_step=4),
     }
 

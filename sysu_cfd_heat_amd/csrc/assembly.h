@@ -44,7 +44,7 @@ struct BcScal {
     int mode;  // 1: outlet rescale, 2: additive correction
 };
 
-// All launches go to stream s. part: reduction partials (12*kRedGrid
+// All launches go to stream s. part: reduction partials (kPartSize
 // doubles; a reducing kernel uses NV * grid of them).
 struct Red {
     double *part;

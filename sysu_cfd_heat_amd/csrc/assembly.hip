@@ -988,7 +988,7 @@ void launch_solid_bc_temperature(const DevMesh &m, const double *T, const double
 void launch_remove_mean(double *b, int n, Red red, hipStream_t s) {
     // the mean lands in the last reduction slot (part[12*kRedGrid-1] is never
     // a partial: reductions use at most 12*gridDim.x < 12*kRedGrid slots)
-    double *mean = red.part + 12 * kRedGrid - 1;
+    double *mean = red.part + kPartSize - 1;
     CFD_REDUCE_LAUNCH(k_sum, std::min(red_grid(n), kRedGrid - 1), s, n,
                        (const double *)b, mean, red.part, fin);
     hipLaunchKernelGGL(k_sub_scalar, dim3(grid_for(n)), dim3(kBlock), 0, s, n,

@@ -153,6 +153,38 @@ __device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *part, int f
     for (int i = fin ? (int)(n) : (int)(blockIdx.x * blockDim.x + threadIdx.x); i < (n); \
          i += gridDim.x * blockDim.x)
 
+// XCD-aware workgroup order. The dispatcher deals workgroups round-robin to
+// the 8 XCDs (workgroup b runs on XCD b % 8), each with its own L2. Mapping b
+// to the logical block (b % 8) * (G / 8) + b / 8 gives every XCD one
+// contiguous range of rows, so the neighbour gathers of a sparse row mostly
+// hit rows its own L2 has just loaded. Identity when G is not a multiple of 8.
+__device__ __forceinline__ int xcd_block() {
+    const int b = blockIdx.x, G = gridDim.x;
+    return (G & 7) ? b : (b & 7) * (G >> 3) + (b >> 3);
+}
+// grid of one-row-per-thread launches over m rows, rounded up to 8 workgroups
+inline int grid8(long long m) {
+    long long g = (m + kBlock - 1) / kBlock;
+    g = (g + 7) / 8 * 8;
+    return (int)(g < 8 ? 8 : g);
+}
+// fixed grid of the reducing sparse kernels (grid-stride): more workgroups
+// than red_grid keeps more gathers in flight
+constexpr int kSpmvGrid = 4096;
+inline int spmv_grid(long long n) {
+    long long g = (n + kBlock - 1) / kBlock;
+    if (g >= kSpmvGrid) return kSpmvGrid;
+    return (int)(g < 1 ? 1 : g);
+}
+// size of every reduction-partials buffer (doubles): NV * grid for the
+// largest grid and NV in use, and the per-colour partials of a sweep
+constexpr int kPartSize = 12 * 8192;
+
+// Grid-stride loop of a reducing kernel in XCD-aware workgroup order.
+#define RED_LOOP_X(i, n)                                                               \
+    for (int i = fin ? (int)(n) : (int)(xcd_block() * blockDim.x + threadIdx.x); i < (n); \
+         i += gridDim.x * blockDim.x)
+
 // Launch a reducing kernel: producer on G workgroups, then the finishing
 // workgroup. The argument list names `fin` where the kernel takes it.
 #define CFD_REDUCE_LAUNCH(kern, G, s, ...)                                     \

@@ -84,7 +84,8 @@ class Krylov {
     int check_max = 64;   // batch cap
 
     // Live timing of the CG SpMV kernel (k_cg_pq): HIP events around each
-    // launch on this solver's stream; only launches that did work count.
+    // producer launch on this solver's stream; only launches that did work
+    // count.
     bool prof = false;
     double prof_ms = 0.;
     long long prof_launches = 0;
@@ -107,8 +108,10 @@ class Krylov {
     template <class T>
     void setup_pc(const EllT<T> &A, int pc, Colouring *col, KWork<T> &w);
     template <class T>
+    // rz_blocks: ILU only; also write the workgroup partials of c.y to the
+    // head of the partials buffer and return their count
     void apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, Colouring *col,
-                  KWork<T> &w);
+                  KWork<T> &w, int *rz_blocks = nullptr);
     template <class T>
     SolveStats cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &cfg, Colouring *col,
                   int pc, KWork<T> &w);

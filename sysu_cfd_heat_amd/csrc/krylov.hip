@@ -41,24 +41,29 @@ __device__ __forceinline__ T row_dot(const EllT<T> &A, int i, const T *__restric
 
 #define GRID_LOOP(i, n) \
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+// the same in XCD-aware workgroup order (sparse products: see xcd_block)
+#define GRID_LOOP_X(i, n) \
+    for (int i = xcd_block() * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 
 template <class T>
 __global__ void k_spmv(EllT<T> A, const T *__restrict__ x, T *__restrict__ y) {
-    GRID_LOOP(i, A.n) y[i] = row_dot(A, i, x);
+    GRID_LOOP_X(i, A.n) y[i] = row_dot(A, i, x);
 }
 
-// r = b - A x ; partials ||r||^2, ||b||^2, r.(r/d) (Jacobi rho)
+// r = b - A x ; partials ||r||^2, ||b||^2, r.(r/d) (Jacobi rho) ; optional
+// zeroed vectors and the Jacobi z = r / d
 template <class T>
 __global__ void k_residual(EllT<T> A, const T *__restrict__ b, const T *__restrict__ x,
                            T *__restrict__ r, T *__restrict__ zero1, T *__restrict__ zero2,
-                           KState *st, double *part, int fin, int mode, double eps,
-                           int maxit) {
+                           T *__restrict__ zj, KState *st, double *part, int fin, int mode,
+                           double eps, int maxit) {
     double acc[3] = {0., 0., 0.};
-    RED_LOOP(i, A.n) {
+    RED_LOOP_X(i, A.n) {
         T ri = b[i] - row_dot(A, i, x);
         r[i] = ri;
         if (zero1) zero1[i] = T(0);
         if (zero2) zero2[i] = T(0);
+        if (zj) zj[i] = ri / A.diag[i];
         acc[0] += (double)ri * ri;
         acc[1] += (double)b[i] * b[i];
         acc[2] += (double)ri * (ri / A.diag[i]);
@@ -138,27 +143,31 @@ __global__ void k_add_f(int n, double *__restrict__ x, const float *__restrict__
 }
 
 // ------------------------------------------------------------------- CG --
-// p_new = z + beta p_old ; q = A p_new ; partial p.q ; alpha = rho / p.q.
-// Jacobi: z = r / d is formed on the fly for the row and its neighbours.
-template <bool JAC, class T>
-__global__ void k_cg_pq(EllT<T> A, const T *__restrict__ r, const T *__restrict__ z,
-                        const T *__restrict__ pold, T *__restrict__ pnew, T *__restrict__ q,
+// p = z + beta p ; q = A z + beta q ; partial p.q ; alpha = rho / p.q.
+// q = A p in exact arithmetic (A p = A z + beta A p_old), so the sparse
+// product gathers one vector (z) instead of two; p and q are updated in
+// place (each row reads and writes only its own entries). The drift of the
+// recurrence is bounded by the iteration count times the rounding of one
+// product, orders below the stopping tolerance; the returned accuracy is the
+// true residual b - A x recomputed at the end. beta = 0 on the first
+// iteration, where p and q start at zero.
+template <class T>
+__global__ void k_cg_pq(EllT<T> A, const T *__restrict__ z, T *__restrict__ p, T *__restrict__ q,
                         KState *st, double *part, int fin) {
     if (st->done) return;
     const T beta = (T)st->beta;
     const int n = A.n;
     double acc[1] = {0.};
-    RED_LOOP(i, n) {
-        T zi = JAC ? r[i] / A.diag[i] : z[i];
-        T pi = zi + beta * pold[i];
-        T s = A.diag[i] * pi;
+    RED_LOOP_X(i, n) {
+        const T zi = z[i];
+        const T pi = zi + beta * p[i];
+        T s = A.diag[i] * zi;
         for (int k = 0; k < A.K; ++k) {
             size_t o = (size_t)k * n + i;
-            int c = A.col[o];
-            T zc = JAC ? r[c] / A.diag[c] : z[c];
-            s += A.val[o] * (zc + beta * pold[c]);
+            s += A.val[o] * z[A.col[o]];
         }
-        pnew[i] = pi;
+        s += beta * q[i];
+        p[i] = pi;
         q[i] = s;
         acc[0] += (double)pi * s;
     }
@@ -168,11 +177,12 @@ __global__ void k_cg_pq(EllT<T> A, const T *__restrict__ r, const T *__restrict_
     }
 }
 
-// x += alpha p ; r -= alpha q ; partial ||r||^2 (and Jacobi rho = r.(r/d))
+// x += alpha p ; r -= alpha q ; partial ||r||^2. Jacobi: z = r / d and the
+// partial rho = r.z, then beta = rho / rho_old.
 template <bool JAC, class T>
 __global__ void k_cg_update(int n, T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
-                            const T *__restrict__ q, const T *__restrict__ d, KState *st,
-                            double *part, int fin) {
+                            const T *__restrict__ q, const T *__restrict__ d, T *__restrict__ z,
+                            KState *st, double *part, int fin) {
     if (st->done) return;
     const T alpha = (T)st->alpha;
     double acc[2] = {0., 0.};
@@ -181,7 +191,11 @@ __global__ void k_cg_update(int n, T *__restrict__ x, T *__restrict__ r, const T
         T ri = r[i] - alpha * q[i];
         r[i] = ri;
         acc[0] += (double)ri * ri;
-        if (JAC) acc[1] += (double)ri * (ri / d[i]);
+        if (JAC) {
+            T zi = ri / d[i];
+            z[i] = zi;
+            acc[1] += (double)ri * zi;
+        }
     }
     if (grid_reduce<2>(acc, part, fin)) {
         st->iter += 1;
@@ -232,7 +246,7 @@ __global__ void k_bi_spmv(Ell A, const double *__restrict__ x, double *__restric
                           int fin) {
     if (st->done || (MODE == 1 && st->half)) return;
     double acc[2] = {0., 0.};
-    RED_LOOP(i, A.n) {
+    RED_LOOP_X(i, A.n) {
         double s = row_dot(A, i, x);
         y[i] = s;
         acc[0] += MODE == 0 ? other[i] * s : s * other[i];
@@ -306,7 +320,7 @@ __global__ void k_gm_v(Ell A, const double *__restrict__ W, double *__restrict__
     if (st->done || st->cycle_done) return;
     const double h = st->hscale;
     const int n = A.n;
-    GRID_LOOP(i, n) {
+    GRID_LOOP_X(i, n) {
         double vi = W[i] / h;
         Vj[i] = vi;
         if (JAC) {
@@ -324,7 +338,7 @@ __global__ void k_gm_v(Ell A, const double *__restrict__ W, double *__restrict__
 __global__ void k_gm_spmv(Ell A, const double *__restrict__ z, double *__restrict__ Wn,
                           const KState *st) {
     if (st->done || st->cycle_done) return;
-    GRID_LOOP(i, A.n) Wn[i] = row_dot(A, i, z);
+    GRID_LOOP_X(i, A.n) Wn[i] = row_dot(A, i, z);
 }
 
 // h[t] = w . V_t, t = 0..j
@@ -517,32 +531,62 @@ __global__ void k_ilu_factor(int n, int K, const int *__restrict__ col, int r0, 
         }
     }
 }
+// ILU(0) sweeps over colour blocks [r0, r1) of the permuted matrix, one row
+// per thread in XCD-aware workgroup order. Colour 0 has no earlier
+// neighbours, so its forward result is c itself: it is never launched and
+// rows below c0end are read from c. The last colour has no later neighbours:
+// its backward step y = t / u_pp runs in its forward launch (fuse). With
+// rpart, every workgroup also writes its partial of c.y (CG's rho = r.z) to
+// rpart[blockIdx.x]; the launches of one application use consecutive slices.
 template <class T>
 __global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col, const T *__restrict__ lu,
-                          int r0, int r1, const T *__restrict__ c, T *__restrict__ t) {
-    int p = r0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= r1) return;
-    T acc = c[p];
-    for (int k = 0; k < K; ++k) {
-        size_t o = (size_t)k * n + p;
-        int cl = col[o];
-        if (cl < p) acc -= lu[o] * t[cl];
+                          const T *__restrict__ ud, int r0, int r1, int c0end,
+                          const T *__restrict__ c, T *__restrict__ t, T *__restrict__ y, int fuse,
+                          double *rpart) {
+    const int p = r0 + xcd_block() * blockDim.x + threadIdx.x;
+    double rz[1] = {0.};
+    if (p < r1) {
+        T acc = c[p];
+        for (int k = 0; k < K; ++k) {
+            size_t o = (size_t)k * n + p;
+            int cl = col[o];
+            if (cl < p) acc -= lu[o] * (cl < c0end ? c[cl] : t[cl]);
+        }
+        if (fuse) {
+            T yv = acc / ud[p];
+            y[p] = yv;
+            rz[0] = (double)c[p] * yv;
+        } else {
+            t[p] = acc;
+        }
     }
-    t[p] = acc;
+    if (rpart) {
+        block_sum<1>(rz);
+        if (threadIdx.x == 0) rpart[blockIdx.x] = rz[0];
+    }
 }
 template <class T>
 __global__ void k_ilu_bwd(int n, int K, const int *__restrict__ col, const T *__restrict__ lu,
-                          const T *__restrict__ ud, int r0, int r1, const T *__restrict__ t,
-                          T *__restrict__ y) {
-    int p = r0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= r1) return;
-    T acc = t[p];
-    for (int k = 0; k < K; ++k) {
-        size_t o = (size_t)k * n + p;
-        int cl = col[o];
-        if (cl > p) acc -= lu[o] * y[cl];
+                          const T *__restrict__ ud, int r0, int r1, int c0end,
+                          const T *__restrict__ c, const T *__restrict__ t, T *__restrict__ y,
+                          double *rpart) {
+    const int p = r0 + xcd_block() * blockDim.x + threadIdx.x;
+    double rz[1] = {0.};
+    if (p < r1) {
+        T acc = p < c0end ? c[p] : t[p];
+        for (int k = 0; k < K; ++k) {
+            size_t o = (size_t)k * n + p;
+            int cl = col[o];
+            if (cl > p) acc -= lu[o] * y[cl];
+        }
+        T yv = acc / ud[p];
+        y[p] = yv;
+        rz[0] = (double)c[p] * yv;
     }
-    y[p] = acc / ud[p];
+    if (rpart) {
+        block_sum<1>(rz);
+        if (threadIdx.x == 0) rpart[blockIdx.x] = rz[0];
+    }
 }
 
 // permuted copies: A'[p] = A[perm[p]] (columns via the cached pcol)
@@ -639,7 +683,7 @@ template struct KWork<float>;
 
 Krylov::Krylov(hipStream_t s) : st_(s) {
     state_.alloc(1);
-    part_.alloc(12 * kRedGrid);
+    part_.alloc(kPartSize);
     CFD_HIP(hipHostMalloc((void **)&hstate_, sizeof(KState), hipHostMallocDefault));
     std::memset(hstate_, 0, sizeof(KState));
 }
@@ -693,14 +737,14 @@ void Krylov::read_state() {
 }
 
 void Krylov::spmv(const Ell &A, const double *x, double *y) {
-    hipLaunchKernelGGL(k_spmv<double>, dim3(grid_for(A.n)), dim3(kBlock), 0, st_, A, x, y);
+    hipLaunchKernelGGL(k_spmv<double>, dim3(grid8(A.n)), dim3(kBlock), 0, st_, A, x, y);
     CFD_LAUNCH_CHECK();
 }
 
 double Krylov::relres(const Ell &A, const double *x, const double *b) {
     ensure(A.n, A.K);
     CFD_REDUCE_LAUNCH(k_residual<double>, red_grid(A.n), st_, A, b, x,
-                       wd_.tmp.p, (double *)nullptr, (double *)nullptr, state_.p, part_.p,
+                       wd_.tmp.p, (double *)nullptr, (double *)nullptr, (double *)nullptr, state_.p, part_.p,
                        fin, 0, 0., 0);
     CFD_LAUNCH_CHECK();
     read_state();
@@ -728,7 +772,8 @@ void Krylov::setup_pc(const EllT<T> &A, int pc, Colouring *col, KWork<T> &w) {
 // A is the permuted matrix for SSOR / ILU
 template <class T>
 void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, Colouring *col,
-                      KWork<T> &w) {
+                      KWork<T> &w, int *rz_blocks) {
+    if (rz_blocks) *rz_blocks = 0;
     const int n = A.n;
     if (pc == CFD_PC_NONE) {
         CFD_HIP(hipMemcpyAsync(y, c, sizeof(T) * n, hipMemcpyDeviceToDevice, st_));
@@ -751,19 +796,30 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
         hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
                            omega * (2. - omega), y);
     } else {
-        for (int k = 0; k < col->ncolour; ++k) {
-            int r0 = col->off[k], r1 = col->off[k + 1];
-            if (r1 > r0)
-                hipLaunchKernelGGL(k_ilu_fwd<T>, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_, n,
-                                   A.K, A.col, (const T *)w.lu.p, r0, r1, c, w.tmp.p);
+        const int nc = col->ncolour;
+        const int c0end = nc > 1 ? col->off[1] : n;
+        double *rp = rz_blocks ? part_.p : nullptr;
+        int used = 0;
+        for (int k = nc > 1 ? 1 : 0; k < nc; ++k) {  // colour 0 only when it is the last
+            const int r0 = col->off[k], r1 = col->off[k + 1];
+            if (r1 <= r0) continue;
+            const bool last = k == nc - 1;
+            const int g = grid8(r1 - r0);
+            hipLaunchKernelGGL(k_ilu_fwd<T>, dim3(g), dim3(kBlock), 0, st_, n, A.K, A.col,
+                               (const T *)w.lu.p, (const T *)w.ud.p, r0, r1, c0end, c, w.tmp.p,
+                               y, (int)last, (last && rp) ? rp + used : (double *)nullptr);
+            if (last && rp) used += g;
         }
-        for (int k = col->ncolour - 1; k >= 0; --k) {
-            int r0 = col->off[k], r1 = col->off[k + 1];
-            if (r1 > r0)
-                hipLaunchKernelGGL(k_ilu_bwd<T>, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_, n,
-                                   A.K, A.col, (const T *)w.lu.p, (const T *)w.ud.p, r0, r1,
-                                   (const T *)w.tmp.p, y);
+        for (int k = nc - 2; k >= 0; --k) {
+            const int r0 = col->off[k], r1 = col->off[k + 1];
+            if (r1 <= r0) continue;
+            const int g = grid8(r1 - r0);
+            hipLaunchKernelGGL(k_ilu_bwd<T>, dim3(g), dim3(kBlock), 0, st_, n, A.K, A.col,
+                               (const T *)w.lu.p, (const T *)w.ud.p, r0, r1, c0end, c,
+                               (const T *)w.tmp.p, y, rp ? rp + used : (double *)nullptr);
+            if (rp) used += g;
         }
+        if (rz_blocks) *rz_blocks = used;
     }
     CFD_LAUNCH_CHECK();
 }
@@ -863,7 +919,7 @@ SolveStats Krylov::solve_mixed(const Ell &A0, const double *b0, double *x0,
         // FP64 residual r = b - A x
         CFD_REDUCE_LAUNCH(k_residual<double>, red_grid(n), st_, A, b,
                            (const double *)x, wd_.r.p, (double *)nullptr, (double *)nullptr,
-                           state_.p, part_.p, fin, 0, 0., 0);
+                           (double *)nullptr, state_.p, part_.p, fin, 0, 0., 0);
         CFD_LAUNCH_CHECK();
         read_state();
         double bn = std::sqrt(hstate_->bb), rn = std::sqrt(hstate_->rr);
@@ -898,49 +954,50 @@ SolveStats Krylov::solve_mixed(const Ell &A0, const double *b0, double *x0,
 template <class T>
 SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &cfg,
                       Colouring *col, int pc, KWork<T> &w) {
-    const int n = A.n, G = red_grid(n);
+    const int n = A.n, G = red_grid(n), Gs = spmv_grid(n);
     const bool jac = pc == CFD_PC_JACOBI;
-    CFD_REDUCE_LAUNCH(k_residual<T>, G, st_, A, b, (const T *)x, w.r.p,
-                       w.p0.p, w.p1.p, state_.p, part_.p, fin, 1, cfg.tol, cfg.maxit);
-    if (!jac) {
-        apply_pc(A, pc, cfg.omega, (const T *)w.r.p, w.z.p, col, w);
-        CFD_REDUCE_LAUNCH(k_cg_rho<T>, G, st_, n, (const T *)w.r.p,
-                           (const T *)w.z.p, state_.p, part_.p, fin);
-    }
+    // z = M^-1 r and rho = r.z (ILU folds the partials of r.z into its sweeps)
+    auto precond_rho = [&]() {
+        if (pc == CFD_PC_ILU) {
+            int nbk = 0;
+            apply_pc(A, pc, cfg.omega, (const T *)w.r.p, w.z.p, col, w, &nbk);
+            hipLaunchKernelGGL(k_cg_rho<T>, dim3(1), dim3(kBlock), 0, st_, n, (const T *)w.r.p,
+                               (const T *)w.z.p, state_.p, part_.p, nbk);
+        } else {
+            apply_pc(A, pc, cfg.omega, (const T *)w.r.p, w.z.p, col, w);
+            CFD_REDUCE_LAUNCH(k_cg_rho<T>, G, st_, n, (const T *)w.r.p, (const T *)w.z.p,
+                              state_.p, part_.p, fin);
+        }
+    };
+    CFD_REDUCE_LAUNCH(k_residual<T>, Gs, st_, A, b, (const T *)x, w.r.p, w.p0.p, w.q.p,
+                      jac ? w.z.p : (T *)nullptr, state_.p, part_.p, fin, 1, cfg.tol,
+                      cfg.maxit);
+    if (!jac) precond_rho();
     CFD_LAUNCH_CHECK();
     int batch = check_first, issued = 0, prev_it = 0;
     double prev_log = 1e300;
-    T *pa = w.p0.p, *pb = w.p1.p;
     size_t nev = 0;
     for (;;) {
         for (int k = 0; k < batch; ++k) {
             if (prof) CFD_HIP(hipEventRecord(ev(2 * nev), st_));
-            if (jac)
-                CFD_REDUCE_LAUNCH((k_cg_pq<true, T>), G, st_, A,
-                                   (const T *)w.r.p, (const T *)nullptr, (const T *)pa, pb, w.q.p,
-                                   state_.p, part_.p, fin);
-            else
-                CFD_REDUCE_LAUNCH((k_cg_pq<false, T>), G, st_, A,
-                                   (const T *)w.r.p, (const T *)w.z.p, (const T *)pa, pb, w.q.p,
-                                   state_.p, part_.p, fin);
+            hipLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, A, (const T *)w.z.p,
+                               w.p0.p, w.q.p, state_.p, part_.p, 0);
             if (prof) {
                 CFD_HIP(hipEventRecord(ev(2 * nev + 1), st_));
                 ++nev;
             }
+            hipLaunchKernelGGL(k_cg_pq<T>, dim3(1), dim3(kBlock), 0, st_, A, (const T *)w.z.p,
+                               w.p0.p, w.q.p, state_.p, part_.p, Gs);
             if (jac) {
-                CFD_REDUCE_LAUNCH((k_cg_update<true, T>), G, st_, n, x,
-                                   w.r.p, (const T *)pb, (const T *)w.q.p, (const T *)A.diag,
-                                   state_.p, part_.p, fin);
+                CFD_REDUCE_LAUNCH((k_cg_update<true, T>), G, st_, n, x, w.r.p, (const T *)w.p0.p,
+                                  (const T *)w.q.p, (const T *)A.diag, w.z.p, state_.p,
+                                  part_.p, fin);
             } else {
-                CFD_REDUCE_LAUNCH((k_cg_update<false, T>), G, st_, n, x,
-                                   w.r.p, (const T *)pb, (const T *)w.q.p, (const T *)A.diag,
-                                   state_.p, part_.p, fin);
-                apply_pc(A, pc, cfg.omega, (const T *)w.r.p, w.z.p, col, w);
-                CFD_REDUCE_LAUNCH(k_cg_rho<T>, G, st_, n,
-                                   (const T *)w.r.p, (const T *)w.z.p, state_.p, part_.p,
-                                   fin);
+                CFD_REDUCE_LAUNCH((k_cg_update<false, T>), G, st_, n, x, w.r.p,
+                                  (const T *)w.p0.p, (const T *)w.q.p, (const T *)A.diag,
+                                  (T *)nullptr, state_.p, part_.p, fin);
+                precond_rho();
             }
-            std::swap(pa, pb);
         }
         CFD_LAUNCH_CHECK();
         issued += batch;
@@ -977,10 +1034,10 @@ template SolveStats Krylov::cg<float>(const EllF &, const float *, float *, cons
 
 SolveStats Krylov::bicgstab(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
                             Colouring *col, int pc) {
-    const int n = A.n, G = red_grid(n), g = grid_for(n);
+    const int n = A.n, G = red_grid(n), Gs = spmv_grid(n), g = grid_for(n);
     const bool jac = pc == CFD_PC_JACOBI;
     CFD_REDUCE_LAUNCH(k_residual<double>, G, st_, A, b, (const double *)x, wd_.r.p,
-                       wd_.p0.p, v_.p, state_.p, part_.p, fin, 3, cfg.tol, cfg.maxit);
+                       wd_.p0.p, v_.p, (double *)nullptr, state_.p, part_.p, fin, 3, cfg.tol, cfg.maxit);
     hipLaunchKernelGGL(k_copy, dim3(g), dim3(kBlock), 0, st_, n, (const double *)wd_.r.p, rt_.p);
     CFD_LAUNCH_CHECK();
     int batch = check_first, issued = 0, prev_it = 0;
@@ -995,7 +1052,7 @@ SolveStats Krylov::bicgstab(const Ell &A, const double *b, double *x, const cfd_
                                    v_.p, ph_.p, A.diag, state_.p);
                 apply_pc(A, pc, cfg.omega, (const double *)wd_.p0.p, ph_.p, col, wd_);
             }
-            CFD_REDUCE_LAUNCH(k_bi_spmv<0>, G, st_, A, ph_.p, v_.p, rt_.p,
+            CFD_REDUCE_LAUNCH(k_bi_spmv<0>, Gs, st_, A, ph_.p, v_.p, rt_.p,
                                state_.p, part_.p, fin);
             if (jac) {
                 CFD_REDUCE_LAUNCH(k_bi_s<true>, G, st_, n, wd_.r.p, v_.p,
@@ -1005,7 +1062,7 @@ SolveStats Krylov::bicgstab(const Ell &A, const double *b, double *x, const cfd_
                                    s_.p, sh_.p, A.diag, state_.p, part_.p, fin);
                 apply_pc(A, pc, cfg.omega, (const double *)s_.p, sh_.p, col, wd_);
             }
-            CFD_REDUCE_LAUNCH(k_bi_spmv<1>, G, st_, A, sh_.p, t_.p, s_.p,
+            CFD_REDUCE_LAUNCH(k_bi_spmv<1>, Gs, st_, A, sh_.p, t_.p, s_.p,
                                state_.p, part_.p, fin);
             CFD_REDUCE_LAUNCH(k_bi_x, G, st_, n, x, wd_.r.p, ph_.p, sh_.p,
                                s_.p, t_.p, rt_.p, state_.p, part_.p, fin);
@@ -1025,7 +1082,7 @@ SolveStats Krylov::bicgstab(const Ell &A, const double *b, double *x, const cfd_
 
 SolveStats Krylov::gmres(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
                          Colouring *col, int pc) {
-    const int n = A.n, G = red_grid(n), g = grid_for(n), m = 10;
+    const int n = A.n, G = red_grid(n), g = grid_for(n), g8 = grid8(n), m = 10;
     const bool jac = pc == CFD_PC_JACOBI;
     // set m before the first restart kernel reads it
     {
@@ -1037,7 +1094,7 @@ SolveStats Krylov::gmres(const Ell &A, const double *b, double *x, const cfd_sol
     }
     double *W = w0_.p, *Wn = w1_.p;
     CFD_REDUCE_LAUNCH(k_residual<double>, G, st_, A, b, (const double *)x, W,
-                       (double *)nullptr, (double *)nullptr, state_.p, part_.p, fin, 2,
+                       (double *)nullptr, (double *)nullptr, (double *)nullptr, state_.p, part_.p, fin, 2,
                        cfg.tol, cfg.maxit);
     CFD_LAUNCH_CHECK();
     read_state();
@@ -1045,13 +1102,13 @@ SolveStats Krylov::gmres(const Ell &A, const double *b, double *x, const cfd_sol
         for (int j = 0; j < m; ++j) {
             double *Vj = V_.p + (size_t)j * n;
             if (jac) {
-                hipLaunchKernelGGL(k_gm_v<true>, dim3(g), dim3(kBlock), 0, st_, A, W, Vj, Wn,
+                hipLaunchKernelGGL(k_gm_v<true>, dim3(g8), dim3(kBlock), 0, st_, A, W, Vj, Wn,
                                    state_.p);
             } else {
-                hipLaunchKernelGGL(k_gm_v<false>, dim3(g), dim3(kBlock), 0, st_, A, W, Vj, Wn,
+                hipLaunchKernelGGL(k_gm_v<false>, dim3(g8), dim3(kBlock), 0, st_, A, W, Vj, Wn,
                                    state_.p);
                 apply_pc(A, pc, cfg.omega, (const double *)Vj, wd_.z.p, col, wd_);
-                hipLaunchKernelGGL(k_gm_spmv, dim3(g), dim3(kBlock), 0, st_, A, wd_.z.p, Wn,
+                hipLaunchKernelGGL(k_gm_spmv, dim3(g8), dim3(kBlock), 0, st_, A, wd_.z.p, Wn,
                                    state_.p);
             }
             CFD_REDUCE_LAUNCH(k_gm_dot, G, st_, n, Wn, V_.p, state_.p,
@@ -1075,7 +1132,7 @@ SolveStats Krylov::gmres(const Ell &A, const double *b, double *x, const cfd_sol
         W = w0_.p;
         Wn = w1_.p;
         CFD_REDUCE_LAUNCH(k_residual<double>, G, st_, A, b, (const double *)x, W,
-                           (double *)nullptr, (double *)nullptr, state_.p, part_.p, fin, 2,
+                           (double *)nullptr, (double *)nullptr, (double *)nullptr, state_.p, part_.p, fin, 2,
                            cfg.tol, -1);
         CFD_LAUNCH_CHECK();
         read_state();

@@ -34,7 +34,7 @@ __global__ void k_m_init(Ell A, MVec V, int nb, KState *st, double *part, int fi
     double acc[2 * NB];
     for (int k = 0; k < 2 * NB; ++k) acc[k] = 0.;
     const int n = A.n;
-    RED_LOOP(i, n) {
+    RED_LOOP_X(i, n) {
         double s[NB];
         const double d = A.diag[i];
         for (int q = 0; q < NB; ++q) if (q < nb) s[q] = d * V.x[q][i];
@@ -105,7 +105,7 @@ __global__ void k_m_spmv(Ell A, MVec V, int nb, KState *st, double *part, int fi
     double acc[2 * NB];
     for (int k = 0; k < 2 * NB; ++k) acc[k] = 0.;
     const int n = A.n;
-    RED_LOOP(i, n) {
+    RED_LOOP_X(i, n) {
         double s[NB];
         const double d = A.diag[i];
         for (int q = 0; q < NB; ++q) if (q < nb) s[q] = act[q] ? d * (MODE == 0 ? V.ph[q][i] : V.sh[q][i]) : 0.;
@@ -214,7 +214,7 @@ void Krylov::solve_multi(const Ell &A, int nb, const double *const *b, double *c
     if (nb < 1 || nb > NB) throw HipError("multi-RHS solve takes 1..3 systems");
     if (cfg.method != CFD_BICGSTAB || cfg.precond != CFD_PC_JACOBI)
         throw HipError("multi-RHS solve is BiCGSTAB + Jacobi");
-    const int n = A.n, G = red_grid(n), g = grid_for(n);
+    const int n = A.n, G = red_grid(n), Gs = spmv_grid(n), g = grid_for(n);
     ensure(n, A.K);
     if (mrhs_.n < (size_t)8 * NB * n) mrhs_.alloc((size_t)8 * NB * n);
     if (!mstate_.p) {
@@ -237,11 +237,11 @@ void Krylov::solve_multi(const Ell &A, int nb, const double *const *b, double *c
         for (int k = 0; k < batch; ++k) {
             hipLaunchKernelGGL(k_m_p, dim3(g), dim3(kBlock), 0, st_, n, V, nb,
                                (const double *)A.diag, (const KState *)mstate_.p);
-            CFD_REDUCE_LAUNCH(k_m_spmv<0>, G, st_, A, V, nb, mstate_.p,
+            CFD_REDUCE_LAUNCH(k_m_spmv<0>, Gs, st_, A, V, nb, mstate_.p,
                                part_.p, fin);
             CFD_REDUCE_LAUNCH(k_m_s, G, st_, n, V, nb,
                                (const double *)A.diag, mstate_.p, part_.p, fin);
-            CFD_REDUCE_LAUNCH(k_m_spmv<1>, G, st_, A, V, nb, mstate_.p,
+            CFD_REDUCE_LAUNCH(k_m_spmv<1>, Gs, st_, A, V, nb, mstate_.p,
                                part_.p, fin);
             CFD_REDUCE_LAUNCH(k_m_x, G, st_, n, V, nb, mstate_.p,
                                part_.p, fin);

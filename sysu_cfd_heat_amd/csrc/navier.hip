@@ -196,7 +196,7 @@ int NSSolver::setup(const cfd_raw_mesh &raw, const cfd_params &p, std::string &e
     RUFace.alloc(std::max<size_t>(F, 1));
     res_dev.alloc(10);
     gtmp.alloc(std::max<size_t>(3 * C, 2 * (size_t)hm.nvrt) + 1);
-    red_part.alloc(12 * kRedGrid);
+    red_part.alloc(kPartSize);
     bcs.alloc(1);
     errflag.alloc(1);
     errflag.zero(s);
@@ -811,17 +811,17 @@ int cfd_ns_profile_read(const cfd_ns *h, double out[8]) {
     long long nnz = 0;
     for (size_t k = 0; k < n.dmo.ell_col_host.size(); ++k)
         if (n.dmo.ell_col_host[k] != (int)(k % n.hm.ncel)) ++nnz;
-    // k_cg_pq per launch: reads r|z, diag, p_old (3 words per row), writes
-    // p_new, q (2 words per row); one value word + a 4 B column per stored
-    // off-diagonal; neighbour gathers of the vectors hit cache and are not
-    // counted (each vector is read once). Words are 8 B (FP64) or 4 B (FP32).
+    // k_cg_pq per launch: reads z, diag, p, q (4 words per row), writes p, q
+    // (2 words per row); one value word + a 4 B column per stored
+    // off-diagonal; neighbour gathers of z are not counted (each vector is
+    // read once). Words are 8 B (FP64) or 4 B (FP32).
     out[0] = (double)n.kry->prof_launches;
     out[1] = n.kry->prof_ms;
-    out[2] = 40.0 * n.hm.ncel + 12.0 * (double)nnz;
+    out[2] = 48.0 * n.hm.ncel + 12.0 * (double)nnz;
     out[3] = (double)nnz;
     out[4] = (double)n.kry->prof_launches_f32();
     out[5] = n.kry->prof_ms_f32();
-    out[6] = 20.0 * n.hm.ncel + 8.0 * (double)nnz;
+    out[6] = 24.0 * n.hm.ncel + 8.0 * (double)nnz;
     out[7] = 0.;
     return 0;
 }

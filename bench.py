@@ -35,13 +35,103 @@ def parse():
     ap.add_argument("--nz", type=int, default=64)
     ap.add_argument("--cpu-sample", type=int, default=1,
                     help="oracle outer iterations timed for cpu_baseline (0: skip)")
-    ap.add_argument("--cpu-nx", type=int, default=0,
-                    help="oracle sample mesh (0: same mesh as the GPU run)")
+    ap.add_argument("--cpu-nx", type=int, default=-1,
+                    help="oracle sample mesh nx (ny, nz scaled alike; 0: the GPU run's mesh; "
+                         "default nx/2, i.e. 1/8 of the cells)")
+    ap.add_argument("--learned", type=int, default=1,
+                    help="p-solve iterations: learned guess (+ mixed precision) vs plain FP64")
+    ap.add_argument("--learn-nx", type=int, default=32,
+                    help="training mesh nx (ny, nz = nx/2) for the learned guess")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy this many MiB device-to-device (5 times) and exit: the "
+                         "known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE "
+                         "counters")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8}))
+
+
+def measured_traffic(a):
+    """HBM bytes per k_cg_pq launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated); only for the workload it measured."""
+    if (a.nx, a.ny, a.nz) != (128, 64, 64):
+        return None
+    f = os.path.join(ROOT, "profiles", "pmc_k_cg_pq_channel_3M.md")
+    try:
+        for line in open(f):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def learned_vs_plain(a):
+    """Pressure-correction Krylov iterations on a held-out inlet speed at the
+    bench mesh: plain FP64 (x0 = 0, the reference) against the learned guess,
+    in FP64 and with the mixed-precision solve. The network is trained on the
+    device on (system, p') pairs captured from channel runs on a smaller mesh
+    with inlet speeds drawn from U(0.5, 2.0)."""
+    import numpy as np
+    import sysu_cfd_heat_amd as P
+    from sysu_cfd_heat_amd import cases
+    t0 = time.time()
+    lnx = a.learn_nx
+    net = P.Gnn(seed=3)
+    speeds = np.random.default_rng(0).uniform(0.5, 2.0, 8)
+    cap_iters = 8
+    for u in speeds:
+        raw, p = cases.channel(nx=lnx, ny=lnx // 2, nz=lnx // 2, max_step=cap_iters + 2,
+                               u_in=float(u))
+        s = P.NSSolver(raw, p)
+        s.capture(net, 100000)
+        for _ in range(cap_iters):
+            s.call("Iterate")
+        s.close()
+    first, _ = net.train(epochs=60, lr=2e-3, seed=1)
+    _, last = net.train(epochs=30, lr=5e-4, seed=2)
+    train_s = time.time() - t0
+    print(f"[bench] learned guess: {net.num_samples()} samples, loss {first:.3f} -> {last:.3f} "
+          f"in {train_s:.1f}s", file=sys.stderr, flush=True)
+    iters, res = 3, {}
+    for name, mode, prec in (("plain_fp64", 0, 0), ("learned_fp64", 2, 0),
+                             ("learned_mixed", 2, 1), ("prev_p_fp64", 1, 0)):
+        raw, p = cases.channel(nx=a.nx, ny=a.ny, nz=a.nz, max_step=iters + 2, u_in=1.25)
+        p.pres_precision = prec
+        s = P.NSSolver(raw, p)
+        s.set_guess(mode, net if mode == 2 else None)
+        worst = 0.0
+        for _ in range(iters):
+            s.call("Iterate")
+            worst = max(worst, s.last_solve("p")[1])
+        res[name] = {"p_iters": int(s.stats()[3]), "worst_true_relres": worst}
+        s.close()
+        print(f"[bench] {name}: {res[name]}", file=sys.stderr, flush=True)
+    plain = res["plain_fp64"]["p_iters"]
+    return {"eval": f"inlet speed 1.25 (held out), first {iters} outer iterations, same mesh as "
+                    f"the timed run, tolerance 1e-8 on the true FP64 residual",
+            "train": f"{net.num_samples()} (system, p') pairs from {len(speeds)} channel runs on "
+                     f"{lnx}x{lnx // 2}x{lnx // 2}x6 tets, 90 epochs Adam on the GPU, "
+                     f"loss {first:.3f} -> {last:.3f}",
+            **res,
+            "plain_over_learned_mixed": plain / max(res["learned_mixed"]["p_iters"], 1),
+            "plain_over_learned_fp64": plain / max(res["learned_fp64"]["p_iters"], 1)}
 
 
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(multi-GPU sharding is a design, see the sharding section of DESIGN.md)",
@@ -85,24 +175,31 @@ def main():
               "ke_iters_per_step": st[4] / a.steps, "T_iters_per_step": st[5] / a.steps,
               "solid_iters_per_step": st[6] / a.steps}
     gpu.close()
+    roof["traffic"] = measured_traffic(a)
+
+    learned = learned_vs_plain(a) if a.learned else None
 
     cpu = None
     if a.cpu_sample > 0:
         from oracle import oracle as O
-        nx = a.cpu_nx or a.nx
-        ny = a.ny if not a.cpu_nx else max(1, a.ny * a.cpu_nx // a.nx)
-        nz = a.nz if not a.cpu_nx else max(1, a.nz * a.cpu_nx // a.nx)
-        craw, cp = cases.channel(nx=nx, ny=ny, nz=nz, tets=True, jitter=0.1, seed=1,
+        cnx = a.nx // 2 if a.cpu_nx < 0 else (a.cpu_nx or a.nx)
+        ny = max(1, a.ny * cnx // a.nx)
+        nz = max(1, a.nz * cnx // a.nx)
+        craw, cp = cases.channel(nx=cnx, ny=ny, nz=nz, tets=True, jitter=0.1, seed=1,
                                  max_step=a.cpu_sample + 2)
         t2 = time.time()
         ref = O.NSSolver(craw, cp, mc=1)
         print(f"[bench] oracle setup {time.time() - t2:.1f}s", file=sys.stderr, flush=True)
         secs, _ = ref.time_iterations(a.cpu_sample)
         ref.close()
-        cpu = {"value": a.cpu_sample / secs, "unit": "SIMPLE iters/s", "cores": 1,
+        ratio = craw.counts[1] / raw.counts[1]
+        cpu = {"value": a.cpu_sample / secs * ratio, "unit": "SIMPLE iters/s", "cores": 1,
                "kind": "port",
-               "sample": f"first {a.cpu_sample} outer iteration(s) of the same case "
-                         f"({nx}x{ny}x{nz}x6 tets), single thread, oracle",
+               "sample": f"oracle, single thread: first {a.cpu_sample} outer iteration(s) of "
+                         f"the same case on {cnx}x{ny}x{nz}x6 tets ({craw.counts[1]} cells) in "
+                         f"{secs:.1f} s, rate scaled by the cell ratio {ratio:.4f} to "
+                         f"{raw.counts[1]} cells (linear in cells: favours the CPU, whose "
+                         f"Krylov iteration counts grow with the mesh)",
                "seconds": secs}
         print(f"[bench] oracle {secs:.1f}s for {a.cpu_sample} iteration(s)", file=sys.stderr,
               flush=True)
@@ -116,7 +213,7 @@ def main():
                                f"({raw.counts[1]} cells), k-eps Re=1e4, energy + solid",
                    "cells": raw.counts[1], "solvers": "BiCGSTAB+Jacobi / CG+ILU(0), 1e-8",
                    "parallelism": "single GPU"},
-        "roofline": roof, "cpu_baseline": cpu, "krylov": krylov,
+        "roofline": roof, "cpu_baseline": cpu, "krylov": krylov, "p_iters_learned": learned,
         "device_s_per_step": st[1] / max(st[0], 1),
     }
     print(json.dumps(line))

@@ -134,12 +134,26 @@ __device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *part, int f
             for (int k = 0; k < NV; ++k) part[k * gridDim.x + blockIdx.x] = v[k];
         return false;
     }
+    // four interleaved sums per thread keep four loads in flight; they are
+    // combined in a fixed order
+    double a4[4][NV];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) a4[j][k] = 0.;
+    const int B = blockDim.x;
+    int b = threadIdx.x;
+    for (; b + 3 * B < fin; b += 4 * B)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) a4[j][k] += part[k * fin + b + j * B];
+    for (; b < fin; b += B)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) a4[0][k] += part[k * fin + b];
     double acc[NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = 0.;
-    for (int b = threadIdx.x; b < fin; b += blockDim.x)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) acc[k] += part[k * fin + b];
+    for (int k = 0; k < NV; ++k) acc[k] = (a4[0][k] + a4[1][k]) + (a4[2][k] + a4[3][k]);
     block_sum<NV>(acc);
     if (threadIdx.x == 0)
 #pragma unroll

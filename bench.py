@@ -65,7 +65,7 @@ def measured_traffic(a):
     WRITE_SIZE summary (copy-calibrated); only for the workload it measured."""
     if (a.nx, a.ny, a.nz) != (128, 64, 64):
         return None
-    f = os.path.join(ROOT, "profiles", "pmc_k_cg_pq_channel_3M.md")
+    f = os.path.join(ROOT, "profiles", "pmc_k_cg_pq_channel_3M.txt")
     try:
         for line in open(f):
             if line.startswith("traffic_bytes_per_launch:"):

@@ -167,6 +167,23 @@ typedef struct cfd_ns_s cfd_ns;
 
 cfd_ns *cfd_ns_create(const cfd_raw_mesh *m, const cfd_params *p);
 void cfd_ns_free(cfd_ns *h);
+
+/* Batched cases (no counterpart in the reference, which runs one case per
+ * process): ncase independent problems meshes[k] / params[k] packed into one
+ * union mesh, so that every kernel launch and every Krylov solve covers all
+ * of them. The cases share every solver setting (schemes, relaxation, Krylov
+ * methods and tolerances, turbulence and energy switches) and have no
+ * conjugate heat transfer; region tables (boundary values, viscosity, hence
+ * Re) are per case. SetBCVelocity's mass balance, the pressure reference
+ * cell, the mean projection of the pressure right-hand side and every Krylov
+ * stopping test (||r_k|| < tol ||b_k|| for each case k) act per case, so each
+ * case converges as it would alone. The handle is an ordinary cfd_ns over
+ * the union: fields are the cases' fields one after the other. GMRES is not
+ * available for batched cases. */
+cfd_ns *cfd_ns_create_batch(int ncase, const cfd_raw_mesh *meshes, const cfd_params *params);
+/* ranges of case k in the union: cells [r[0], r[1]), faces [r[2], r[3]),
+ * boundary faces [r[4], r[5]); returns the number of cases, -1 on error */
+int cfd_ns_case_range(const cfd_ns *h, int k, int r[6]);
 /* counts of fluid (part 0) or solid (part 1) mesh: Nvrt,Ncel,Nfac,Nbnd,NCoupledBnd */
 int cfd_ns_counts(const cfd_ns *h, int part, int counts[5]);
 

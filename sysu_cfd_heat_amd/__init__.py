@@ -85,6 +85,9 @@ def lib() -> C.CDLL:
     L.cfd_mesh_get_dbl.argtypes = [vp, C.c_char_p, dp]
     L.cfd_ns_create.restype = vp
     L.cfd_ns_create.argtypes = [C.POINTER(RawMeshC), C.POINTER(ParamsC)]
+    L.cfd_ns_create_batch.restype = vp
+    L.cfd_ns_create_batch.argtypes = [C.c_int, C.POINTER(RawMeshC), C.POINTER(ParamsC)]
+    L.cfd_ns_case_range.argtypes = [vp, C.c_int, ip]
     L.cfd_ns_free.argtypes = [vp]
     L.cfd_ns_counts.argtypes = [vp, C.c_int, ip]
     L.cfd_ns_call.argtypes = [vp, C.c_char_p]
@@ -299,6 +302,34 @@ class NSSolver:
         if not self.h:
             raise RuntimeError(self._error())
         self.has_solid = bool(params.cht)
+
+    @classmethod
+    def batch(cls, cases) -> "NSSolver":
+        """Batched cases: [(RawMesh, ParamsC), ...] packed into one union mesh
+        (every launch covers all cases; per-case stopping tests)."""
+        self = cls.__new__(cls)
+        self._L = lib()
+        self._raw = [r for r, _ in cases]
+        meshes = (RawMeshC * len(cases))(*[r.c() for r, _ in cases])
+        params = (ParamsC * len(cases))(*[p for _, p in cases])
+        self.h = self._L.cfd_ns_create_batch(len(cases), meshes, params)
+        if not self.h:
+            raise RuntimeError(_err(self._L))
+        self.has_solid = False
+        return self
+
+    def case_ranges(self):
+        """Per case: (cells, faces, boundary faces) slices of the union."""
+        out, k = [], 0
+        while True:
+            r = (C.c_int * 6)()
+            n = self._L.cfd_ns_case_range(self.h, k, r)
+            if n < 0:
+                raise RuntimeError(_err(self._L))
+            out.append((slice(r[0], r[1]), slice(r[2], r[3]), slice(r[4], r[5])))
+            k += 1
+            if k >= n:
+                return out
 
     # hooks so the oracle wrapper can reuse this class
     def _load(self):

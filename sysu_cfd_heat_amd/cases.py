@@ -8,6 +8,8 @@ Region ids equal their boundary type for inlet (2), outlet (3) and symmetry
 """
 from __future__ import annotations
 
+import numpy as np
+
 from . import (CFD_BICGSTAB, CFD_CG, CFD_GMRES, CFD_PC_ILU, CFD_PC_JACOBI, CFD_PC_SSOR,
                add_region, box_mesh, default_params, solve_cfg)
 
@@ -36,17 +38,23 @@ def config_solvers(p, tol=1e-8):
     p.pres_project = 1
 
 
-def cavity(n=32, re=100.0, tets=False, jitter=0.0, seed=1, max_step=201, solvers="config"):
+def cavity(n=32, re=100.0, tets=False, jitter=0.0, seed=1, max_step=201, solvers="config",
+           turbulence=False):
     """Lid-driven cavity in the unit cube: n^3 hexahedra, lid y=1 moving with
     u=1 (an inlet with zero normal velocity: the reference has no moving
-    wall, src/navier_bc.cpp:20-25), other walls no-slip adiabatic."""
+    wall, src/navier_bc.cpp:20-25), other walls no-slip adiabatic. With
+    turbulence, k-epsilon starts from 5% intensity and a length scale of 1%
+    of the cavity on the lid and in the fluid."""
     raw = box_mesh(n, n, n, 1.0, 1.0, 1.0, tets=tets, jitter=jitter, seed=seed,
                    side_rid=(WALL, WALL, WALL, INLET, WALL, WALL), fluid_rid=5)
     p = default_params()
     mu = 1.0 / re
+    k0 = 1.5 * 0.05 ** 2 if turbulence else 0.0
+    e0 = 0.09 ** 0.75 * k0 ** 1.5 / 0.01 if turbulence else 0.0
     add_region(p, WALL, 1, 1, 0.0)                                # Hwall, q = 0
-    add_region(p, INLET, 2, 0, 0.0, (1.0, 0.0, 0.0, 0.0, 1.0, 0.0))  # u,v,w,p,r,t
-    add_region(p, 5, 5, 0, 0.0, (0.0, 0.0, 0.0, 1.0, 0.0, mu))    # u,v,w,ro,t,mu
+    add_region(p, INLET, 2, 0, 0.0, (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, k0, e0))  # u,v,w,p,r,t,k,e
+    add_region(p, 5, 5, 0, 0.0, (0.0, 0.0, 0.0, 1.0, 0.0, mu, k0, e0))    # u,v,w,ro,t,mu,k,e
+    p.turb_model = 1 if turbulence else 0
     p.max_step = max_step
     p.residual_steady = 1e-30
     (config_solvers if solvers == "config" else reference_solvers)(p)
@@ -88,3 +96,31 @@ def channel(nx=128, ny=64, nz=64, tets=True, jitter=0.1, seed=1, re=1e4, turbule
     p.residual_steady = 1e-30
     (config_solvers if solvers == "config" else reference_solvers)(p)
     return raw, p
+
+
+def batch(ncase=64, seed=1, nmin=8000, nmax=500000, re_lo=100.0, re_hi=2000.0,
+          turbulence=True, max_step=12):
+    """Batched uneven cases: ncase independent problems, alternately a
+    lid-driven cavity (hexahedra, n = round(N^(1/3))) and a channel without
+    a solid (jittered tets, 2m x m x m hexahedra x 6, m = round((N/12)^(1/3))),
+    cell counts N log-uniform in [nmin, nmax], Re uniform in [re_lo, re_hi]
+    (mu = 1/Re), all from one seed. Every case uses the same solver settings:
+    BiCGSTAB+Jacobi transport solved u/v/w together, CG+ILU(0) pressure with
+    the mixed-precision (FP32 inner, FP64 refinement) solve, tolerance 1e-8."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(ncase):
+        N = float(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        re = float(rng.uniform(re_lo, re_hi))
+        if k % 2 == 0:
+            n = max(2, int(round(N ** (1.0 / 3.0))))
+            raw, p = cavity(n=n, re=re, max_step=max_step, turbulence=turbulence)
+        else:
+            m = max(1, int(round((N / 12.0) ** (1.0 / 3.0))))
+            raw, p = channel(nx=2 * m, ny=m, nz=m, tets=True, jitter=0.1, seed=seed + k, re=re,
+                             turbulence=turbulence, energy=False, cht=False, heat=0.0,
+                             max_step=max_step)
+        p.multi_rhs = 1
+        p.pres_precision = 1
+        out.append((raw, p))
+    return out

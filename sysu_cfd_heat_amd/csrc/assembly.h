@@ -5,7 +5,7 @@
 
 namespace cfd {
 
-constexpr int kMaxRegions = 16;
+constexpr int kMaxRegions = 1024;  // batched cases carry one table each
 
 // Region table (BdRegion, src/BasicType.h:69-95) indexed by region slot.
 struct RegTab {
@@ -34,8 +34,17 @@ struct DevMesh {
     const double *v_x[3];
     const int *v2c_ptr, *v2c;  // vertex -> cells (MLP limiter)
     const int *b_face, *b_cell, *b_reg;
+    const int *b_qface;    // [B] face SetBCVelocity's outlet-area sum reads (Face[i], i the
+                           // boundary index within its mesh: src/navier_bc.cpp:72)
     const double *b_dist;
     const RegTab *reg;     // device copy
+    // batched cases (a disjoint union of independent meshes): ncase > 1
+    int ncase = 1;
+    const int *c_case = nullptr;     // [C] case of each cell
+    const int *b_case = nullptr;     // [B] case of each boundary face
+    const int *case_cell = nullptr;  // [ncase+1] cell offsets
+    const int *case_bnd = nullptr;   // [ncase+1] boundary offsets
+    const int *case_ref = nullptr;   // [ncase] pressure reference cell of each case
 };
 
 // Scalars produced on the device by SetBCVelocity's global mass balance.
@@ -117,6 +126,8 @@ void launch_solid_bc_temperature(const DevMesh &m, const double *T, const double
 void launch_fill(double *p, size_t n, double v, hipStream_t s);
 // b -= mean(b): consistency projection of a pure-Neumann right-hand side
 void launch_remove_mean(double *b, int n, Red red, hipStream_t s);
+// batched cases: the mean of b is removed from each case's cells separately
+void launch_remove_case_means(const DevMesh &m, double *b, Red red, hipStream_t s);
 void launch_scale_source(const DevMesh &m, double q, double *src, hipStream_t s);
 
 }  // namespace cfd

@@ -227,7 +227,7 @@ __global__ void k_bc_velocity(DevMesh m, const double *__restrict__ R,
         if (t1 == 2) acc[0] += br * (bu * n0 + bv * n1 + bw * n2);
         else if (t1 == 3) {
             acc[1] += br * (bu * n0 + bv * n1 + bw * n2);
-            acc[2] += br * m.f_area[b];  // Face[i] with the boundary index, as the reference
+            acc[2] += br * m.f_area[m.b_qface[b]];  // Face[i], i the boundary index, as the reference
         }
     }
     if (grid_reduce<3>(acc, part, fin)) {
@@ -244,13 +244,47 @@ __global__ void k_bc_velocity(DevMesh m, const double *__restrict__ R,
     }
 }
 
+// batched cases: SetBCVelocity's mass balance per case, one workgroup per
+// case over its boundary faces (values already written by k_bc_velocity)
+__global__ void k_bc_case_sums(DevMesh m, const double *__restrict__ BR,
+                               const double *__restrict__ BU, const double *__restrict__ BV,
+                               const double *__restrict__ BW, BcScal *sc) {
+    const int k = blockIdx.x;
+    double acc[3] = {0., 0., 0.};
+    for (int b = m.case_bnd[k] + threadIdx.x; b < m.case_bnd[k + 1]; b += blockDim.x) {
+        const int t1 = m.reg->type1[m.b_reg[b]], f = m.b_face[b];
+        if (t1 != 2 && t1 != 3) continue;
+        const double q = BR[b] * (BU[b] * m.f_n[0][f] + BV[b] * m.f_n[1][f] + BW[b] * m.f_n[2][f]);
+        if (t1 == 2) acc[0] += q;
+        else {
+            acc[1] += q;
+            acc[2] += BR[b] * m.f_area[m.b_qface[b]];
+        }
+    }
+    block_sum<3>(acc);
+    if (threadIdx.x == 0) {
+        BcScal &S = sc[k];
+        S.massin = acc[0];
+        S.massout = acc[1];
+        S.areaout = acc[2];
+        if (fabs(acc[1]) > kSmall) {
+            S.rate = -acc[0] / acc[1];
+            S.mode = 1;
+        } else {
+            S.rate = (-acc[0] - acc[1]) / acc[2];
+            S.mode = 2;
+        }
+    }
+}
+
 // outlet correction (src/navier_bc.cpp:74-106)
 __global__ void k_bc_outlet(DevMesh m, const BcScal *sc, double *__restrict__ BU,
                             double *__restrict__ BV, double *__restrict__ BW) {
-    const double rate = sc->rate;
-    const int mode = sc->mode;
     CELL_LOOP(b, m.nbnd) {
         if (m.reg->type1[m.b_reg[b]] != 3) continue;
+        const BcScal &S = sc[m.b_case ? m.b_case[b] : 0];
+        const double rate = S.rate;
+        const int mode = S.mode;
         if (mode == 1) {
             BU[b] *= rate;
             BV[b] *= rate;
@@ -513,9 +547,9 @@ __global__ void k_pressure_update(DevMesh m, const double *__restrict__ x, int r
                                   double *__restrict__ V, double *__restrict__ W,
                                   double *res_slot, double *part, int fin) {
     const int C = m.ncel;
-    const double xr = x[ref];
     double acc[1] = {0.};
     RED_LOOP(i, C) {
+        const double xr = x[m.case_ref ? m.case_ref[m.c_case[i]] : ref];
         const double vol = m.c_vol[i], d = x[i] - xr;
         acc[0] += fabs(d) * vol;
         P[i] = P[i] + urf * d;
@@ -864,6 +898,10 @@ void launch_bc_velocity(const DevMesh &m, const double *R, const double *U, cons
     if (m.nbnd == 0) return;
     CFD_REDUCE_LAUNCH(k_bc_velocity, red_grid(m.nbnd), s, m, R, U, V, W,
                        BR, BU, BV, BW, sc, red.part, fin);
+    if (m.ncase > 1)
+        hipLaunchKernelGGL(k_bc_case_sums, dim3(m.ncase), dim3(kBlock), 0, s, m,
+                           (const double *)BR, (const double *)BU, (const double *)BV,
+                           (const double *)BW, sc);
     hipLaunchKernelGGL(k_bc_outlet, dim3(grid_for(m.nbnd)), dim3(kBlock), 0, s, m,
                        (const BcScal *)sc, BU, BV, BW);
     CFD_LAUNCH_CHECK();
@@ -992,6 +1030,31 @@ void launch_remove_mean(double *b, int n, Red red, hipStream_t s) {
     CFD_REDUCE_LAUNCH(k_sum, std::min(red_grid(n), kRedGrid - 1), s, n,
                        (const double *)b, mean, red.part, fin);
     hipLaunchKernelGGL(k_sub_scalar, dim3(grid_for(n)), dim3(kBlock), 0, s, n,
+                       (const double *)mean, b);
+    CFD_LAUNCH_CHECK();
+}
+
+namespace {
+// per-case means: one workgroup per case; then b[i] -= mean[case of i]
+__global__ void k_case_mean(DevMesh m, const double *__restrict__ b, double *mean) {
+    const int k = blockIdx.x;
+    const int c0 = m.case_cell[k], c1 = m.case_cell[k + 1];
+    double acc[1] = {0.};
+    for (int i = c0 + threadIdx.x; i < c1; i += blockDim.x) acc[0] += b[i];
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) mean[k] = acc[0] / (c1 - c0);
+}
+__global__ void k_sub_case(DevMesh m, const double *__restrict__ mean, double *__restrict__ b) {
+    CELL_LOOP(i, m.ncel) b[i] -= mean[m.c_case[i]];
+}
+}  // namespace
+
+void launch_remove_case_means(const DevMesh &m, double *b, Red red, hipStream_t s) {
+    // the means land at the tail of the partials buffer, past every partial
+    double *mean = red.part + kPartSize - 1 - m.ncase;
+    hipLaunchKernelGGL(k_case_mean, dim3(m.ncase), dim3(kBlock), 0, s, m, (const double *)b,
+                       mean);
+    hipLaunchKernelGGL(k_sub_case, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m,
                        (const double *)mean, b);
     CFD_LAUNCH_CHECK();
 }

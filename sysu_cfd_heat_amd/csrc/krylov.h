@@ -80,6 +80,14 @@ class Krylov {
     double relres(const Ell &A, const double *x, const double *b);
     void sync();
 
+    // Batched cases: rows [off[k], off[k+1]) of the natural order form case
+    // k of a block-diagonal system. The device-side stopping test is then
+    // switched off and a solve stops at the first host check where every case
+    // meets ||r_k|| < tol ||b_k||; the accuracy returned is the worst case's
+    // true relative residual.
+    void set_cases(const std::vector<int> &off);
+    bool batched() const { return case_off_.size() > 2; }
+
     int check_first = 4;  // iterations before the first host check
     int check_max = 64;   // batch cap
 
@@ -103,6 +111,21 @@ class Krylov {
     long long prof_launches_f32_ = 0;
     hipEvent_t ev(size_t k);
     int next_batch(int batch, int &prev_it, double &prev_log, int maxit);
+    int next_batch_log(int batch, int &prev_it, double &prev_log, int maxit, double lr, double lt);
+    // batched cases
+    std::vector<int> case_off_;
+    std::vector<int> seg_host_;   // (start, end, case) in the current solve order
+    DBuf<int> seg_;
+    DBuf<double> seg_out_;
+    std::vector<double> seg_h_;
+    const Colouring *seg_col_ = nullptr;
+    int seg_n_ = -1;
+    bool seg_built_ = false;
+    void build_segments(const Colouring *col, int n);
+    // worst case ||r_k|| / ||b_k|| over the cases; *all: every case below tol
+    double case_worst(const double *r, const double *b, double tol, bool *all);
+    // worst case true relative residual of x
+    double case_relres(const Ell &A, const double *x, const double *b);
     void ensure(int n, int K);
     void read_state();
     template <class T>
@@ -114,9 +137,9 @@ class Krylov {
                   KWork<T> &w, int *rz_blocks = nullptr);
     template <class T>
     SolveStats cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &cfg, Colouring *col,
-                  int pc, KWork<T> &w);
+                  int pc, KWork<T> &w, bool per_case = false);
     SolveStats bicgstab(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
-                        Colouring *col, int pc);
+                        Colouring *col, int pc, bool per_case = false);
     SolveStats gmres(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
                      Colouring *col, int pc);
     DBuf<double> mrhs_;   // multi-RHS BiCGSTAB work vectors

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include "krylov.h"
 
@@ -707,7 +708,14 @@ hipEvent_t Krylov::ev(size_t k) {
 int Krylov::next_batch(int batch, int &prev_it, double &prev_log, int maxit) {
     const KState &h = *hstate_;
     double target = h.bnorm > 0 ? h.eps * h.bnorm : h.eps;
-    double lr = std::log(std::max(h.rnorm, 1e-300)), lt = std::log(std::max(target, 1e-300));
+    return next_batch_log(batch, prev_it, prev_log, maxit, std::log(std::max(h.rnorm, 1e-300)),
+                          std::log(std::max(target, 1e-300)));
+}
+
+// lr: log of the residual measure now, lt: log of its target
+int Krylov::next_batch_log(int batch, int &prev_it, double &prev_log, int maxit, double lr,
+                           double lt) {
+    const KState &h = *hstate_;
     int nb = batch * 2;
     if (h.iter > prev_it && prev_log > lr) {
         double rate = (prev_log - lr) / (h.iter - prev_it);
@@ -719,6 +727,95 @@ int Krylov::next_batch(int batch, int &prev_it, double &prev_log, int maxit) {
     nb = std::min(std::max(nb, 1), check_max);
     nb = std::min(nb, std::max(1, maxit - h.iter));
     return nb;
+}
+
+// ------------------------------------------------------- batched cases --
+namespace {
+// sums of r^2 and b^2 over each (start, end, case) segment, one workgroup each
+__global__ void k_seg_norms(const int *__restrict__ seg, const double *__restrict__ r,
+                            const double *__restrict__ b, double *__restrict__ out) {
+    const int s0 = seg[3 * blockIdx.x], s1 = seg[3 * blockIdx.x + 1];
+    double acc[2] = {0., 0.};
+    for (int i = s0 + threadIdx.x; i < s1; i += blockDim.x) {
+        acc[0] += r[i] * r[i];
+        acc[1] += b[i] * b[i];
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = acc[0];
+        out[2 * blockIdx.x + 1] = acc[1];
+    }
+}
+}  // namespace
+
+void Krylov::set_cases(const std::vector<int> &off) {
+    case_off_ = off;
+    seg_built_ = false;
+}
+
+// Segments of the cases in the solve's row order. Natural order: one per
+// case. Multicolour order: rows of colour c keep their natural order, so the
+// rows of case k in colour c are one contiguous range inside colour block c.
+void Krylov::build_segments(const Colouring *col, int n) {
+    if (seg_built_ && seg_col_ == col && seg_n_ == n) return;
+    const int nc = (int)case_off_.size() - 1;
+    if (case_off_.back() != n) throw HipError("batched cases do not cover the system");
+    seg_host_.clear();
+    if (!col) {
+        for (int k = 0; k < nc; ++k)
+            if (case_off_[k + 1] > case_off_[k])
+                seg_host_.insert(seg_host_.end(), {case_off_[k], case_off_[k + 1], k});
+    } else {
+        std::vector<int> cnt((size_t)col->ncolour * nc, 0);
+        for (int k = 0; k < nc; ++k)
+            for (int i = case_off_[k]; i < case_off_[k + 1]; ++i)
+                cnt[(size_t)col->colour_host[i] * nc + k]++;
+        for (int c = 0; c < col->ncolour; ++c) {
+            int at = col->off[c];
+            for (int k = 0; k < nc; ++k) {
+                int m = cnt[(size_t)c * nc + k];
+                if (m) seg_host_.insert(seg_host_.end(), {at, at + m, k});
+                at += m;
+            }
+        }
+    }
+    seg_.alloc(seg_host_.size());
+    seg_.upload(seg_host_.data(), seg_host_.size(), st_);
+    seg_out_.alloc(2 * (seg_host_.size() / 3) + 1);
+    seg_col_ = col;
+    seg_n_ = n;
+    seg_built_ = true;
+}
+
+double Krylov::case_worst(const double *r, const double *b, double tol, bool *all) {
+    const int ns = (int)seg_host_.size() / 3, nc = (int)case_off_.size() - 1;
+    hipLaunchKernelGGL(k_seg_norms, dim3(ns), dim3(kBlock), 0, st_, (const int *)seg_.p, r, b,
+                       seg_out_.p);
+    CFD_LAUNCH_CHECK();
+    seg_h_.resize(2 * (size_t)ns);
+    seg_out_.download(seg_h_.data(), 2 * (size_t)ns, st_);
+    std::vector<double> rr(nc, 0.), bb(nc, 0.);
+    for (int t = 0; t < ns; ++t) {  // fixed order: segment order
+        rr[seg_host_[3 * t + 2]] += seg_h_[2 * t];
+        bb[seg_host_[3 * t + 2]] += seg_h_[2 * t + 1];
+    }
+    double worst = 0.;
+    bool ok = true;
+    for (int k = 0; k < nc; ++k) {
+        double rn = std::sqrt(rr[k]), bn = std::sqrt(bb[k]);
+        worst = std::max(worst, bn > 0 ? rn / bn : rn);
+        ok = ok && (bn > 0 ? rn < tol * bn : rn < tol);
+    }
+    if (all) *all = ok;
+    return worst;
+}
+
+double Krylov::case_relres(const Ell &A, const double *x, const double *b) {
+    ensure(A.n, A.K);
+    CFD_REDUCE_LAUNCH(k_residual<double>, red_grid(A.n), st_, A, b, x, wd_.tmp.p,
+                      (double *)nullptr, (double *)nullptr, (double *)nullptr, state_.p, part_.p,
+                      fin, 0, 0., 0);
+    return case_worst(wd_.tmp.p, b, 0., nullptr);
 }
 
 void Krylov::ensure(int n, int K) {
@@ -869,11 +966,16 @@ SolveStats Krylov::solve(const Ell &A0, const double *b0, double *x0, const cfd_
         x = wd_.px.p;
         setup_pc(A, pc, col, wd_);
     }
+    const bool per_case = batched();
+    if (per_case) {
+        if (cfg.method == CFD_GMRES) throw HipError("batched cases: GMRES is not supported");
+        build_segments(perm ? col : nullptr, n);
+    }
     SolveStats S;
-    if (cfg.method == CFD_CG) S = cg(A, b, x, cfg, col, pc, wd_);
-    else if (cfg.method == CFD_BICGSTAB) S = bicgstab(A, b, x, cfg, col, pc);
+    if (cfg.method == CFD_CG) S = cg(A, b, x, cfg, col, pc, wd_, per_case);
+    else if (cfg.method == CFD_BICGSTAB) S = bicgstab(A, b, x, cfg, col, pc, per_case);
     else S = gmres(A, b, x, cfg, col, pc);
-    S.relres = relres(A, x, b);
+    S.relres = per_case ? case_relres(A, x, b) : relres(A, x, b);
     if (perm) {
         hipLaunchKernelGGL(k_perm_scatter, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
                            col->perm.p, (const double *)wd_.px.p, x0);
@@ -906,6 +1008,7 @@ SolveStats Krylov::solve_mixed(const Ell &A0, const double *b0, double *x0,
         b = wd_.pb.p;
         x = wd_.px.p;
     }
+    if (batched()) build_segments(perm ? col : nullptr, n);
     // FP32 copy of the (permuted) matrix and its preconditioner
     hipLaunchKernelGGL(k_matrix_to_float, dim3(grid_for(n)), dim3(kBlock), 0, st_, n, A.K,
                        (const double *)A.diag, (const double *)A.val, wf_.pdiag.p, wf_.pval.p);
@@ -924,13 +1027,22 @@ SolveStats Krylov::solve_mixed(const Ell &A0, const double *b0, double *x0,
         read_state();
         double bn = std::sqrt(hstate_->bb), rn = std::sqrt(hstate_->rr);
         rel = bn > 0 ? rn / bn : rn;
-        if ((bn > 0 ? rn < cfg.tol * bn : rn < cfg.tol) || total >= cfg.maxit) break;
+        if (batched()) {  // every case to the tolerance on its own FP64 residual
+            bool all = false;
+            rel = case_worst(wd_.r.p, b, cfg.tol, &all);
+            if (all || total >= cfg.maxit) break;
+        } else if ((bn > 0 ? rn < cfg.tol * bn : rn < cfg.tol) || total >= cfg.maxit) {
+            break;
+        }
         // FP32 correction: A e = r to inner_tol relative to ||r||
         hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
                            (const double *)wd_.r.p, wf_.pb.p);
         CFD_HIP(hipMemsetAsync(wf_.px.p, 0, sizeof(float) * n, st_));
         cfd_solve_cfg ic = cfg;
-        ic.tol = std::max(inner_tol, 0.5 * cfg.tol * bn / rn);
+        // batched cases: the inner solve's test is global, so it gets the
+        // fixed relative reduction and the outer loop runs until every case
+        // meets the tolerance on its own FP64 residual
+        ic.tol = batched() ? inner_tol : std::max(inner_tol, 0.5 * cfg.tol * bn / rn);
         ic.maxit = cfg.maxit - total;
         SolveStats si = cg(Af, (const float *)wf_.pb.p, wf_.px.p, ic, col, pc, wf_);
         total += si.iters;
@@ -953,7 +1065,9 @@ SolveStats Krylov::solve_mixed(const Ell &A0, const double *b0, double *x0,
 
 template <class T>
 SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &cfg,
-                      Colouring *col, int pc, KWork<T> &w) {
+                      Colouring *col, int pc, KWork<T> &w, bool per_case) {
+    if (!std::is_same<T, double>::value) per_case = false;  // FP32 inner solves: one test
+    const double dev_tol = per_case ? 0. : cfg.tol;  // batched: the host decides
     const int n = A.n, G = red_grid(n), Gs = spmv_grid(n);
     const bool jac = pc == CFD_PC_JACOBI;
     // z = M^-1 r and rho = r.z (ILU folds the partials of r.z into its sweeps)
@@ -970,7 +1084,7 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
         }
     };
     CFD_REDUCE_LAUNCH(k_residual<T>, Gs, st_, A, b, (const T *)x, w.r.p, w.p0.p, w.q.p,
-                      jac ? w.z.p : (T *)nullptr, state_.p, part_.p, fin, 1, cfg.tol,
+                      jac ? w.z.p : (T *)nullptr, state_.p, part_.p, fin, 1, dev_tol,
                       cfg.maxit);
     if (!jac) precond_rho();
     CFD_LAUNCH_CHECK();
@@ -1002,6 +1116,15 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
         CFD_LAUNCH_CHECK();
         issued += batch;
         read_state();
+        if (per_case) {
+            bool all = false;
+            const double worst = case_worst((const double *)w.r.p, (const double *)b, cfg.tol,
+                                            &all);
+            if (all || hstate_->done || issued > cfg.maxit + 2) break;
+            batch = next_batch_log(batch, prev_it, prev_log, cfg.maxit,
+                                   std::log(std::max(worst, 1e-300)), std::log(cfg.tol));
+            continue;
+        }
         if (hstate_->done || issued > cfg.maxit + 2) break;
         batch = next_batch(batch, prev_it, prev_log, cfg.maxit);
     }
@@ -1028,16 +1151,19 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
     return S;
 }
 template SolveStats Krylov::cg<double>(const Ell &, const double *, double *,
-                                       const cfd_solve_cfg &, Colouring *, int, KWork<double> &);
+                                       const cfd_solve_cfg &, Colouring *, int, KWork<double> &,
+                                       bool);
 template SolveStats Krylov::cg<float>(const EllF &, const float *, float *, const cfd_solve_cfg &,
-                                      Colouring *, int, KWork<float> &);
+                                      Colouring *, int, KWork<float> &, bool);
 
 SolveStats Krylov::bicgstab(const Ell &A, const double *b, double *x, const cfd_solve_cfg &cfg,
-                            Colouring *col, int pc) {
+                            Colouring *col, int pc, bool per_case) {
+    const double dev_tol = per_case ? 0. : cfg.tol;  // batched: the host decides
     const int n = A.n, G = red_grid(n), Gs = spmv_grid(n), g = grid_for(n);
     const bool jac = pc == CFD_PC_JACOBI;
     CFD_REDUCE_LAUNCH(k_residual<double>, G, st_, A, b, (const double *)x, wd_.r.p,
-                       wd_.p0.p, v_.p, (double *)nullptr, state_.p, part_.p, fin, 3, cfg.tol, cfg.maxit);
+                      wd_.p0.p, v_.p, (double *)nullptr, state_.p, part_.p, fin, 3, dev_tol,
+                      cfg.maxit);
     hipLaunchKernelGGL(k_copy, dim3(g), dim3(kBlock), 0, st_, n, (const double *)wd_.r.p, rt_.p);
     CFD_LAUNCH_CHECK();
     int batch = check_first, issued = 0, prev_it = 0;
@@ -1070,6 +1196,14 @@ SolveStats Krylov::bicgstab(const Ell &A, const double *b, double *x, const cfd_
         CFD_LAUNCH_CHECK();
         issued += batch;
         read_state();
+        if (per_case) {
+            bool all = false;
+            const double worst = case_worst(wd_.r.p, b, cfg.tol, &all);
+            if (all || hstate_->done || issued > cfg.maxit + 2) break;
+            batch = next_batch_log(batch, prev_it, prev_log, cfg.maxit,
+                                   std::log(std::max(worst, 1e-300)), std::log(cfg.tol));
+            continue;
+        }
         if (hstate_->done || issued > cfg.maxit + 2) break;
         batch = next_batch(batch, prev_it, prev_log, cfg.maxit);
     }

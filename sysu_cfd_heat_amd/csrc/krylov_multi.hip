@@ -229,8 +229,10 @@ void Krylov::solve_multi(const Ell &A, int nb, const double *const *b, double *c
         V.x[q] = q < nb ? x[q] : nullptr;
         V.b[q] = q < nb ? b[q] : nullptr;
     }
+    const bool per_case = batched();  // batched cases: the host decides per case
+    if (per_case) build_segments(nullptr, n);
     CFD_REDUCE_LAUNCH(k_m_init, G, st_, A, V, nb, mstate_.p, part_.p,
-                       fin, cfg.tol, cfg.maxit);
+                       fin, per_case ? 0. : cfg.tol, cfg.maxit);
     CFD_LAUNCH_CHECK();
     int batch = check_first, issued = 0;
     for (;;) {
@@ -251,21 +253,20 @@ void Krylov::solve_multi(const Ell &A, int nb, const double *const *b, double *c
         CFD_HIP(hipMemcpyAsync(hmstate_, mstate_.p, nb * sizeof(KState), hipMemcpyDeviceToHost, st_));
         CFD_HIP(hipStreamSynchronize(st_));
         bool all = true;
-        int maxiter = 0;
         for (int q = 0; q < NB; ++q) if (q < nb) {
-            all = all && hmstate_[q].done;
-            maxiter = std::max(maxiter, hmstate_[q].iter);
+            bool ok = hmstate_[q].done;
+            if (per_case && !ok) case_worst(V.r[q], V.b[q], cfg.tol, &ok);
+            all = all && ok;
         }
         if (all || issued > cfg.maxit + 2) break;
         batch = std::min(std::max(2, batch * 2), check_max);
-        (void)maxiter;
     }
     for (int q = 0; q < NB; ++q) if (q < nb) {
         out[q].iters = hmstate_[q].iter;
         out[q].breakdown = hmstate_[q].brk;
         out[q].rtcres = hmstate_[q].bnorm > 0 ? hmstate_[q].rnorm / hmstate_[q].bnorm
                                               : hmstate_[q].rnorm;
-        out[q].relres = relres(A, x[q], b[q]);
+        out[q].relres = per_case ? case_relres(A, x[q], b[q]) : relres(A, x[q], b[q]);
     }
 }
 

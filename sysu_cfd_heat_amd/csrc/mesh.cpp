@@ -412,6 +412,62 @@ int build_parts(const cfd_raw_mesh &raw, const cfd_params &p, HostMesh &fluid, H
     return 0;
 }
 
+// Disjoint union of independent meshes (batched cases): vertices, cells,
+// faces and boundaries of part k follow those of part k-1 with every index
+// offset; negative sentinels stay as they are. Each part keeps its own face
+// order (boundary faces first), so off[k] tells where part k starts in each
+// kind of array: off = {vert, cell, face, bnd} x (nparts + 1).
+int concat_meshes(const std::vector<HostMesh> &parts, HostMesh &o, std::vector<int> off[4],
+                  std::string &err) {
+    o = HostMesh();
+    for (int t = 0; t < 4; ++t) off[t].assign(1, 0);
+    auto shift = [](std::vector<int> &dst, const std::vector<int> &src, int by) {
+        for (int v : src) dst.push_back(v >= 0 ? v + by : v);
+    };
+    for (const HostMesh &m : parts) {
+        const int v0 = o.nvrt, c0 = o.ncel, f0 = o.nfac, b0 = o.nbnd;
+        if ((long long)c0 + m.ncel > 2000000000LL || (long long)f0 + m.nfac > 2000000000LL) {
+            err = "batched meshes too large";
+            return -1;
+        }
+        o.vert.insert(o.vert.end(), m.vert.begin(), m.vert.end());
+        shift(o.cell_vert, m.cell_vert, v0);
+        o.cell_rid.insert(o.cell_rid.end(), m.cell_rid.begin(), m.cell_rid.end());
+        o.cell_nface.insert(o.cell_nface.end(), m.cell_nface.begin(), m.cell_nface.end());
+        shift(o.cell_face, m.cell_face, f0);
+        shift(o.cell_cell, m.cell_cell, c0);
+        shift(o.face_vert, m.face_vert, v0);
+        shift(o.face_c1, m.face_c1, c0);
+        shift(o.face_c2, m.face_c2, c0);
+        shift(o.face_bnd, m.face_bnd, b0);
+        shift(o.bnd_vert, m.bnd_vert, v0);
+        o.bnd_rid.insert(o.bnd_rid.end(), m.bnd_rid.begin(), m.bnd_rid.end());
+        shift(o.bnd_face, m.bnd_face, f0);
+        for (auto pr : {std::make_pair(&o.face_x, &m.face_x), std::make_pair(&o.face_n, &m.face_n),
+                        std::make_pair(&o.face_xpac, &m.face_xpac),
+                        std::make_pair(&o.face_xnac, &m.face_xnac),
+                        std::make_pair(&o.face_area, &m.face_area),
+                        std::make_pair(&o.face_lambda, &m.face_lambda),
+                        std::make_pair(&o.face_rlencos, &m.face_rlencos),
+                        std::make_pair(&o.cell_x, &m.cell_x),
+                        std::make_pair(&o.cell_vol, &m.cell_vol),
+                        std::make_pair(&o.bnd_dist, &m.bnd_dist)})
+            pr.first->insert(pr.first->end(), pr.second->begin(), pr.second->end());
+        o.colour.insert(o.colour.end(), m.colour.begin(), m.colour.end());
+        o.ncolour = std::max(o.ncolour, m.ncolour);
+        o.nvrt += m.nvrt;
+        o.ncel += m.ncel;
+        o.nfac += m.nfac;
+        o.nbnd += m.nbnd;
+        o.ncb += m.ncb;
+        off[0].push_back(o.nvrt);
+        off[1].push_back(o.ncel);
+        off[2].push_back(o.nfac);
+        off[3].push_back(o.nbnd);
+    }
+    return 0;
+}
+
 // Greedy first-fit colouring over the interior-face adjacency, natural
 // order: colour(i) = least colour not used by a neighbour j < i.
 void colour_cells(HostMesh &m) {

@@ -34,6 +34,8 @@ int build_geometry(HostMesh &m, std::string &err);
 int split_cells(const HostMesh &full, const std::vector<int> &cells,
                 int coupled_id, HostMesh &out, std::string &err);
 void colour_cells(HostMesh &m);
+int concat_meshes(const std::vector<HostMesh> &parts, HostMesh &out, std::vector<int> off[4],
+                  std::string &err);
 int build_parts(const cfd_raw_mesh &raw, const cfd_params &p, HostMesh &fluid, HostMesh &solid,
                 bool &has_solid, int &coupled_id, std::string &err);
 

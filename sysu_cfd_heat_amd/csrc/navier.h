@@ -13,6 +13,13 @@
 
 namespace cfd {
 
+// Batched cases: where each case starts in the union mesh (cell, face,
+// boundary offsets, ncase + 1 entries each) and its pressure reference cell.
+struct CaseInfo {
+    std::vector<int> cell_off, face_off, bnd_off, ref;
+    int ncase() const { return (int)ref.size(); }
+};
+
 // Device copy of one HostMesh (owns the buffers behind DevMesh's pointers).
 struct DevMeshOwner {
     DevMesh dm;
@@ -20,7 +27,8 @@ struct DevMeshOwner {
     std::vector<DBuf<int>> ib;
     std::vector<DBuf<double>> db;
     DBuf<RegTab> reg;
-    void build(const HostMesh &h, const RegTab &rt, const std::vector<int> &breg, hipStream_t s);
+    void build(const HostMesh &h, const RegTab &rt, const std::vector<int> &breg, hipStream_t s,
+               const CaseInfo *cases = nullptr);
 };
 
 class NSSolver;
@@ -92,6 +100,14 @@ class NSSolver {
 
     ~NSSolver();
     int setup(const cfd_raw_mesh &raw, const cfd_params &p, std::string &err);
+    // Batched cases: ncase independent problems (no conjugate heat transfer)
+    // sharing solver settings, packed into one union mesh so that every
+    // kernel launch covers all of them; the global operations (mass balance,
+    // pressure reference, mean projection, Krylov stopping tests) act per case.
+    CaseInfo cases;
+    int setup_batch(int ncase, const cfd_raw_mesh *raws, const cfd_params *ps,
+                    std::string &err);
+    int finish_setup(std::string &err);  // device mirror, buffers, initial fields
     Red red() { return Red{red_part.p}; }
     const DevMesh &dm() const { return dmo.dm; }
     Ell As() { return Ell{hm.ncel, dmo.dm.K, As_diag.p, As_val.p, dmo.dm.ell_col}; }

@@ -9,7 +9,7 @@ namespace cfd {
 
 // ---------------------------------------------------------- device mesh --
 void DevMeshOwner::build(const HostMesh &h, const RegTab &rt, const std::vector<int> &breg,
-                         hipStream_t s) {
+                         hipStream_t s, const CaseInfo *ci) {
     ib.clear();
     db.clear();
     const int C = h.ncel, F = h.nfac, B = h.nbnd, V = h.nvrt;
@@ -105,6 +105,26 @@ void DevMeshOwner::build(const HostMesh &h, const RegTab &rt, const std::vector<
     m.b_cell = upi(bcell);
     m.b_reg = upi(breg);
     m.b_dist = upd(h.bnd_dist);
+    std::vector<int> qf(B);
+    for (int b = 0; b < B; ++b) qf[b] = b;
+    if (ci && ci->ncase() > 1) {
+        const int nc = ci->ncase();
+        std::vector<int> cc(C), bc(B);
+        for (int k = 0; k < nc; ++k) {
+            for (int i = ci->cell_off[k]; i < ci->cell_off[k + 1]; ++i) cc[i] = k;
+            for (int b = ci->bnd_off[k]; b < ci->bnd_off[k + 1]; ++b) {
+                bc[b] = k;
+                qf[b] = ci->face_off[k] + (b - ci->bnd_off[k]);
+            }
+        }
+        m.ncase = nc;
+        m.c_case = upi(cc);
+        m.b_case = upi(bc);
+        m.case_cell = upi(ci->cell_off);
+        m.case_bnd = upi(ci->bnd_off);
+        m.case_ref = upi(ci->ref);
+    }
+    m.b_qface = upi(qf);
     reg.alloc(1);
     reg.upload(&rt, 1, s);
     m.reg = reg.p;
@@ -180,11 +200,97 @@ int NSSolver::setup(const cfd_raw_mesh &raw, const cfd_params &p, std::string &e
         if (map_bnd(ht->hm, ht->breg, true)) return -1;
         if (ht->hm.nbnd - ht->hm.ncb != hm.nbnd - hm.ncb) { err = "coupled face counts differ"; return -1; }
     }
+    return finish_setup(err);
+}
+
+int NSSolver::setup_batch(int ncase, const cfd_raw_mesh *raws, const cfd_params *ps,
+                          std::string &err) {
+    if (ncase < 1) { err = "no cases"; return -1; }
+    P = ps[0];
+    auto same_cfg = [](const cfd_solve_cfg &a, const cfd_solve_cfg &b) {
+        return a.method == b.method && a.precond == b.precond && a.omega == b.omega &&
+               a.tol == b.tol && a.maxit == b.maxit;
+    };
+    for (int k = 0; k < ncase; ++k) {
+        const cfd_params &q = ps[k];
+        if (q.density_model != 0 || !q.steady) { err = "batched cases are steady, DensityModel 0"; return -1; }
+        if (q.cht) { err = "batched cases have no conjugate heat transfer"; return -1; }
+        bool same = q.turb_model == P.turb_model && q.solve_energy == P.solve_energy &&
+                    q.limiter == P.limiter && q.pres_precision == P.pres_precision &&
+                    q.multi_rhs == P.multi_rhs && q.pres_project == P.pres_project &&
+                    q.prl == P.prl && q.prte == P.prte && q.cp == P.cp &&
+                    same_cfg(q.mom, P.mom) && same_cfg(q.pres, P.pres) && same_cfg(q.ke, P.ke) &&
+                    same_cfg(q.energy, P.energy);
+        for (int t = 0; t < 8; ++t) same = same && q.urf[t] == P.urf[t];
+        if (!same) { err = "batched cases must share solver settings (case " + std::to_string(k) + ")"; return -1; }
+    }
+    // one region table for all cases: case k's region id r becomes k * 65536 + r
+    reg = RegTab();
+    std::vector<HostMesh> parts(ncase);
+    cases = CaseInfo();
+    for (int k = 0; k < ncase; ++k) {
+        const cfd_params &q = ps[k];
+        if (reg.n + q.nregion > kMaxRegions) { err = "too many regions in the batch"; return -1; }
+        for (int r = 0; r < q.nregion; ++r) {
+            const cfd_region &R0 = q.region[r];
+            if (R0.id < 0 || R0.id >= 65536) { err = "batched region ids must be in [0, 65536)"; return -1; }
+            const int id = k * 65536 + R0.id;
+            if (slot_of.count(id)) { err = "repeating region id"; return -1; }
+            int sl = reg.n++;
+            slot_of[id] = sl;
+            reg.id[sl] = R0.id;  // the case's own id: SetBCKEpsilon compares it with 2, 3, 4
+            reg.type1[sl] = R0.type1;
+            reg.type2[sl] = R0.type2;
+            reg.fixed[sl] = R0.fixed;
+            for (int t = 0; t < 10; ++t) reg.init[sl][t] = R0.init[t];
+        }
+        HostMesh solid;
+        bool has_solid = false;
+        int cid = -1;
+        if (build_parts(raws[k], q, parts[k], solid, has_solid, cid, err)) {
+            err = "case " + std::to_string(k) + ": " + err;
+            return -1;
+        }
+        for (int &r : parts[k].cell_rid) r = k * 65536 + r;
+        for (int &r : parts[k].bnd_rid) r = k * 65536 + r;
+        if (q.cell_pressure_ref < 0 || q.cell_pressure_ref >= parts[k].ncel) { err = "pressure reference cell out of range"; return -1; }
+    }
+    std::vector<int> off[4];
+    if (concat_meshes(parts, hm, off, err)) return -1;
+    parts.clear();
+    cases.cell_off = off[1];
+    cases.face_off = off[2];
+    cases.bnd_off = off[3];
+    for (int k = 0; k < ncase; ++k) cases.ref.push_back(off[1][k] + ps[k].cell_pressure_ref);
+    for (int f = 0; f < hm.nfac; ++f)
+        if (hm.face_c2[f] < 0 && hm.face_bnd[f] < 0) { err = "boundary face not listed in the mesh"; return -1; }
+    breg.resize(hm.nbnd);
+    for (int b = 0; b < hm.nbnd; ++b) {
+        auto it = slot_of.find(hm.bnd_rid[b]);
+        if (it == slot_of.end()) { err = "boundary region not defined"; return -1; }
+        int t1 = reg.type1[it->second];
+        if (t1 < 1 || t1 > 4) { err = "no such boundary type"; return -1; }
+        breg[b] = it->second;
+    }
+    for (int i = 0; i < hm.ncel; ++i) {
+        auto it = slot_of.find(hm.cell_rid[i]);
+        if (it == slot_of.end() || reg.type1[it->second] != 5 || reg.type2[it->second] != 0) {
+            err = "fluid cell region must be a fluid body";
+            return -1;
+        }
+    }
+    return finish_setup(err);
+}
+
+int NSSolver::finish_setup(std::string &err) {
+    (void)err;
     CFD_HIP(hipStreamCreate(&s));
     CFD_HIP(hipEventCreate(&ev0));
     CFD_HIP(hipEventCreate(&ev1));
     kry.reset(new Krylov(s));
-    dmo.build(hm, reg, breg, s);
+    const bool batched = cases.ncase() > 1;
+    dmo.build(hm, reg, breg, s, batched ? &cases : nullptr);
+    if (batched) kry->set_cases(cases.cell_off);
     const size_t C = hm.ncel, B = hm.nbnd, F = hm.nfac, K = dmo.dm.K;
     alloc_n({&R, &U, &V, &W, &P_, &T, &TE, &ED, &VisLam, &VisTur, &Apr, &xsol, &bu, &bv, &bw, &bp,
              &bs, &kcond, &As_diag, &Ap_diag, &Prod, &TES, &EDS, &ApTE, &ApED, &VisTE, &VisED,
@@ -197,7 +303,7 @@ int NSSolver::setup(const cfd_raw_mesh &raw, const cfd_params &p, std::string &e
     res_dev.alloc(10);
     gtmp.alloc(std::max<size_t>(3 * C, 2 * (size_t)hm.nvrt) + 1);
     red_part.alloc(kPartSize);
-    bcs.alloc(1);
+    bcs.alloc(std::max(1, cases.ncase()));
     errflag.alloc(1);
     errflag.zero(s);
     for (auto *b : {&R, &U, &V, &W, &P_, &T, &TE, &ED, &VisLam, &VisTur, &Apr, &xsol, &bu, &bv,
@@ -355,7 +461,10 @@ void NSSolver::CorrectRUFace2() {
 int NSSolver::CalculatePressure() {
     launch_fill(xsol, hm.ncel, 0., s);
     BuildPressureMatrix();
-    if (P.pres_project) launch_remove_mean(bp, hm.ncel, red(), s);
+    if (P.pres_project) {
+        if (cases.ncase() > 1) launch_remove_case_means(dm(), bp, red(), s);
+        else launch_remove_mean(bp, hm.ncel, red(), s);
+    }
     pressure_guess();
     kry->prof = prof_pressure;
     last_p = solve(Ap(), bp, xsol, P.pres, colouring_for(P.pres), P.pres_precision);
@@ -652,6 +761,33 @@ cfd_ns *cfd_ns_create(const cfd_raw_mesh *m, const cfd_params *p) {
         return nullptr;
     }
     return h;
+}
+
+cfd_ns *cfd_ns_create_batch(int ncase, const cfd_raw_mesh *meshes, const cfd_params *params) {
+    cfd_ns *h = new cfd_ns;
+    std::string err;
+    int rc = guarded([&] { return h->ns.setup_batch(ncase, meshes, params, err); });
+    if (rc) {
+        if (!err.empty()) last_error() = err;
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+int cfd_ns_case_range(const cfd_ns *h, int k, int r[6]) {
+    const CaseInfo &c = h->ns.cases;
+    const int nc = std::max(1, c.ncase());
+    if (k < 0 || k >= nc) { last_error() = "no such case"; return -1; }
+    if (c.ncase() < 1) {
+        const HostMesh &m = h->ns.hm;
+        r[0] = 0; r[1] = m.ncel; r[2] = 0; r[3] = m.nfac; r[4] = 0; r[5] = m.nbnd;
+        return 1;
+    }
+    r[0] = c.cell_off[k]; r[1] = c.cell_off[k + 1];
+    r[2] = c.face_off[k]; r[3] = c.face_off[k + 1];
+    r[4] = c.bnd_off[k]; r[5] = c.bnd_off[k + 1];
+    return nc;
 }
 
 void cfd_ns_free(cfd_ns *h) {

@@ -42,6 +42,10 @@ def parse():
                     help="p-solve iterations: learned guess (+ mixed precision) vs plain FP64")
     ap.add_argument("--learn-nx", type=int, default=32,
                     help="training mesh nx (ny, nz = nx/2) for the learned guess")
+    ap.add_argument("--workload", default="channel", choices=["channel", "cavity", "batch"],
+                    help="channel: the headline CHT channel (default); cavity: the 32^3 "
+                         "Re=100 lid-driven cavity; batch: 64 uneven cavity/channel cases "
+                         "packed per launch (extra lines, not the headline)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many MiB device-to-device (5 times) and exit: the "
                          "known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE "
@@ -127,6 +131,76 @@ def learned_vs_plain(a):
             "plain_over_learned_fp64": plain / max(res["learned_fp64"]["p_iters"], 1)}
 
 
+def _line(metric, value, unit, a, ms, workload, extra):
+    line = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": a.steps,
+            "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": workload, "parallelism": "single GPU"}}
+    line.update(extra)
+    print(json.dumps(line))
+
+
+def run_cavity(a):
+    """Laminar lid-driven cavity, 32^3 hexahedra, Re = 100, BiCGSTAB+Jacobi
+    momentum, CG+ILU(0) pressure, 1e-8: SIMPLE iterations per second."""
+    from sysu_cfd_heat_amd import NSSolver, cases
+    raw, p = cases.cavity(n=32, re=100.0, max_step=a.warmup + a.steps + 2)
+    gpu = NSSolver(raw, p)
+    for _ in range(a.warmup):
+        gpu.call("Iterate")
+    gpu._fn("reset_stats")(gpu.h)
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        gpu.call("Iterate")
+    el = time.perf_counter() - t1
+    st = gpu.stats()
+    gpu.close()
+    cpu = None
+    if a.cpu_sample > 0:
+        from oracle import oracle as O
+        ref = O.NSSolver(raw, p, mc=1)
+        ns = max(1, min(a.steps, 20))
+        secs, _ = ref.time_iterations(ns)
+        ref.close()
+        cpu = {"value": ns / secs, "unit": "SIMPLE iters/s", "cores": 1, "kind": "port",
+               "sample": f"oracle, single thread: first {ns} outer iterations of the same case"}
+    _line("SIMPLE iters/s, lid-driven cavity 32^3 Re=100", a.steps / el, "SIMPLE iters/s", a,
+          1000 * el / a.steps, "cavity 32^3 hexahedra (32768 cells), Re=100, laminar",
+          {"cpu_baseline": cpu, "krylov": {"mom_iters_per_step": st[2] / a.steps,
+                                           "p_iters_per_step": st[3] / a.steps}})
+
+
+def run_batch(a):
+    """64 uneven cases (cavities and channels, 8k..500k cells, Re 100..2000,
+    k-epsilon), packed into one union mesh: every launch covers all cases.
+    A step advances every case by one SIMPLE iteration."""
+    from sysu_cfd_heat_amd import NSSolver, cases
+    cs = cases.batch(64, seed=1, max_step=a.warmup + a.steps + 2)
+    cells = sum(r.counts[1] for r, _ in cs)
+    t0 = time.time()
+    gpu = NSSolver.batch(cs)
+    print(f"[bench] batch setup {time.time() - t0:.1f}s, {cells} cells in 64 cases",
+          file=sys.stderr, flush=True)
+    for _ in range(a.warmup):
+        gpu.call("Iterate")
+    gpu._fn("reset_stats")(gpu.h)
+    t1 = time.perf_counter()
+    for k in range(a.steps):
+        gpu.call("Iterate")
+        print(f"[bench] batch step {k + 1}/{a.steps}", file=sys.stderr, flush=True)
+    el = time.perf_counter() - t1
+    st = gpu.stats()
+    gpu.close()
+    _line("batched SIMPLE iters/s (all 64 cases advance per step)", a.steps / el,
+          "batch SIMPLE iters/s", a, 1000 * el / a.steps,
+          f"64 uneven cavity/channel cases, {cells} cells, k-eps, multi-RHS u/v/w, "
+          f"mixed-precision pressure", {"case_iters_per_s": 64 * a.steps / el,
+                                        "cell_iters_per_s": cells * a.steps / el,
+                                        "krylov": {"mom_iters_per_step": st[2] / a.steps,
+                                                   "p_iters_per_step": st[3] / a.steps,
+                                                   "ke_iters_per_step": st[4] / a.steps}})
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -137,6 +211,10 @@ def main():
               "(multi-GPU sharding is a design, see the sharding section of DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+    if a.workload == "cavity":
+        return run_cavity(a)
+    if a.workload == "batch":
+        return run_batch(a)
     import numpy as np
     from sysu_cfd_heat_amd import NSSolver, cases
 

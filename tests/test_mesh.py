@@ -206,3 +206,81 @@ def test_mesh_errors():
         P.Mesh(bad, p, 1)  # no solid part
     with pytest.raises(RuntimeError):
         P.box_mesh(0, 1, 1)
+
+
+# ---- gmsh element types (ReadGridFile, src/Preprocessor.cpp:13-146) -------
+_CUBE = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                  [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1], [0.5, 0.5, 0.5]], float)
+_FACES = [(0, 3, 2, 1), (4, 5, 6, 7), (0, 1, 5, 4), (2, 3, 7, 6), (1, 2, 6, 5), (0, 4, 7, 3)]
+
+
+def _orient(vs, apex):
+    """Order a polygon so that its right-hand normal points towards apex."""
+    v = [_CUBE[i] for i in vs]
+    n = np.cross(v[1] - v[0], v[2] - v[0])
+    return tuple(vs) if n @ (_CUBE[apex] - v[0]) > 0 else tuple(reversed(vs))
+
+
+def _write_msh(path, cells, bnds):
+    """cells: (gmsh type, vertices 0-based); bnds: (type, vertices)."""
+    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$Nodes", str(len(_CUBE))]
+    lines += [f"{i + 1} {x} {y} {z}" for i, (x, y, z) in enumerate(_CUBE)]
+    lines += ["$EndNodes", "$Elements", str(len(cells) + len(bnds))]
+    k = 1
+    for t, vs in bnds:
+        lines.append(f"{k} {t} 2 7 0 " + " ".join(str(v + 1) for v in vs)); k += 1
+    for t, vs in cells:
+        lines.append(f"{k} {t} 3 5 0 0 " + " ".join(str(v + 1) for v in vs)); k += 1
+    lines.append("$EndElements")
+    open(path, "w").write("\n".join(lines) + "\n")
+
+
+def _cube_split(kind):
+    if kind == "hex":
+        return [(5, list(range(8)))], [(3, list(f)) for f in _FACES], [1.0]
+    if kind == "pyramids":  # six pyramids, apex at the centre
+        cells = [(7, list(_orient(f, 8)) + [8]) for f in _FACES]
+        return cells, [(3, list(f)) for f in _FACES], [1.0 / 6] * 6
+    if kind == "prisms":  # two prisms over the triangles of the z = 0 face
+        cells = [(6, [0, 1, 2, 4, 5, 6]), (6, [0, 2, 3, 4, 6, 7])]
+        bnds = [(2, [0, 2, 1]), (2, [0, 3, 2]), (2, [4, 5, 6]), (2, [4, 6, 7])]
+        bnds += [(3, list(f)) for f in _FACES[2:]]
+        return cells, bnds, [0.5, 0.5]
+    # six Kuhn tetrahedra along the main diagonal 0-6
+    tets = [(0, 1, 2, 6), (0, 2, 3, 6), (0, 3, 7, 6), (0, 7, 4, 6), (0, 4, 5, 6), (0, 5, 1, 6)]
+    cells = [(4, list(_orient(t[:3], t[3])) + [t[3]]) for t in tets]
+    bnds = []
+    for f in _FACES:  # each cube face is cut into two triangles by the diagonal from 0 or 6
+        a, b, c, d = f
+        cut = (a, c) if (0 in (a, c) or 6 in (a, c)) else (b, d)
+        if cut == (a, c):
+            bnds += [(2, [a, b, c]), (2, [a, c, d])]
+        else:
+            bnds += [(2, [b, c, d]), (2, [b, d, a])]
+    return cells, bnds, [1.0 / 6] * 6
+
+
+@pytest.mark.parametrize("kind", ["hex", "pyramids", "prisms", "tets"])
+def test_gmsh_element_types_give_exact_volumes(kind, tmp_path, oracle):
+    cells, bnds, vols = _cube_split(kind)
+    path = str(tmp_path / f"{kind}.msh")
+    _write_msh(path, cells, bnds)
+    raw = P.RawMesh.read_gmsh(path)
+    assert raw.counts[1] == len(cells) and raw.counts[2] == len(bnds)
+    assert np.all(raw.cell_rid == 5) and np.all(raw.bnd_rid == 7)
+    m = P.Mesh(raw)
+    np.testing.assert_allclose(m.get("cell_vol"), vols, rtol=1e-12)
+    # every face of the split is found once: closed cells, interior faces shared
+    nf, nb = m.counts[2], m.counts[3]
+    per_cell = {"hex": 6, "pyramids": 5, "prisms": 5, "tets": 4}[kind]
+    assert 2 * (nf - nb) + nb == per_cell * len(cells)
+    area = m.get("face_n")
+    for i in range(len(cells)):  # closed surfaces: outward area vectors sum to zero
+        s = np.zeros(3)
+        for j in range(m.get("cell_nface")[i]):
+            f = m.get("cell_face")[i, j]
+            s += area[f] if m.get("face_cell1")[f] == i else -area[f]
+        assert np.linalg.norm(s) <= 1e-12
+    o = oracle.Mesh(raw)
+    for nm in ("face_cell1", "face_cell2", "cell_vol", "face_n"):
+        assert np.array_equal(o.get(nm), m.get(nm)), nm

@@ -146,6 +146,12 @@ typedef struct {
     int multi_rhs;         /* 1: solve u,v,w together (one launch per step) */
     int pres_project;      /* 1: remove the constant null-space component of
                               the pure-Neumann pressure system */
+    /* transient runs (steady = 0; the "transient,dt,total_time,scheme,
+     * MaxOuterStep" line, src/readparamfile.cpp:69-83) */
+    int time_scheme;       /* TimeScheme: 1 Euler, 2 "Dual" (2nd-order BDF) */
+    double dt;             /* time step */
+    double total_time;     /* stop once cur_time >= total_time */
+    int max_outer_step;    /* MaxOuterStep: outer iterations per time step + 1 */
 } cfd_params;
 
 /* Reference defaults: constructor values (src/navier.cpp:354-416), the
@@ -235,6 +241,15 @@ int cfd_ns_reset_stats(cfd_ns *h);
  * per launch, 0}. */
 int cfd_ns_profile(cfd_ns *h, int enable);
 int cfd_ns_profile_read(const cfd_ns *h, double out[8]);
+
+/* Field output in the reference's Tecplot layout (NavierStokesSolver::Output
+ * and Output2Tecplot, src/navier.cpp:264-340; HeatConductionSolver::
+ * Output2Tecplot, src/HeatConductionSolver.cpp:330-363): the header line, one
+ * FEBRICK zone of the fluid (vertex coordinates, then p u v w ro T
+ * |velocity| mu [te ed] per cell, 5 values a line, then the 8 vertices of
+ * each cell, 1-based) and, with conjugate heat transfer, one zone of the
+ * solid (zeros except T). Fields are copied from the device first. */
+int cfd_ns_write_tecplot(cfd_ns *h, const char *path);
 
 const char *cfd_last_error(void);
 

@@ -152,6 +152,7 @@ int oracle_ns_call(void *h, const char *method) {
     else if (m == "UpdateEnergy") rc = ns.UpdateEnergy();
     else if (m == "SolidSolve") rc = ns.ht ? ns.ht->UpdateEnergy() : -1;
     else if (m == "InitFlowField") rc = ns.InitFlowField();
+    else if (m == "SaveTransientOldData") ns.SaveTransientOldData();
     else if (m == "GradientU") rc = gradient(ns.M, ns.P.limiter, ns.Un.data(), ns.BU.data(), ns.dUdX.data());
     else if (m == "GradientV") rc = gradient(ns.M, ns.P.limiter, ns.Vn.data(), ns.BV.data(), ns.dVdX.data());
     else if (m == "GradientW") rc = gradient(ns.M, ns.P.limiter, ns.Wn.data(), ns.BW.data(), ns.dWdX.data());

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <cstring>
 
 namespace oref {
@@ -239,7 +240,10 @@ int RefNS::InitFlowField() {
         }
     }
     for (int i = 0; i < M.nfac; ++i) RUFace[i] = 0.;
-    if (ht) return ht->InitFlowField();
+    if (ht && ht->InitFlowField()) return -1;
+    // the history starts as the initial field (NSSolve saves it again at
+    // the start of every time step)
+    if (!P.steady) SaveTransientOldData();
     return 0;
 }
 
@@ -408,6 +412,21 @@ int RefNS::BuildVelocityMatrix() {
         double app = 0., apn[6] = {0, 0, 0, 0, 0, 0}, su = 0., sv = 0., sw = 0.;
         int ani[6], nj = 0;
         double fde[3], fdi[3], fcs[3] = {0., 0., 0.};
+        if (!P.steady) {  // unsteady terms (src/navier_velocity.cpp:103-118)
+            if (P.time_scheme == 1) {
+                double coef = Rn[i] / P.dt * C.vol;
+                app += coef;
+                su += coef * Unp[i];
+                sv += coef * Vnp[i];
+                sw += coef * Wnp[i];
+            } else if (P.time_scheme == 2) {
+                double coef = Rn[i] / P.dt * C.vol;
+                app += 1.5 * coef;
+                su += coef * (2 * Unp[i] - 0.5 * Unp2[i]);
+                sv += coef * (2 * Vnp[i] - 0.5 * Vnp2[i]);
+                sw += coef * (2 * Wnp[i] - 0.5 * Wnp2[i]);
+            }
+        }
         for (int j = 0; j < C.nface; ++j) {
             int iface = C.face[j];
             const Face &F = M.face[iface];
@@ -817,6 +836,10 @@ int RefNS::UpdateTurKEpsilon() {
         ApTE[i] = Dis / TE[i] * vol;
     }
     SetBCKEpsilon(TES.data(), EDS.data(), ApTE.data(), ApED.data(), Prod.data());
+    if (!P.steady) {  // src/navier_scalar.cpp:72-91; EDSource is overwritten after the k solve
+        transient_terms(ApTE.data(), TES.data(), TEp.data(), TEp2.data());
+        transient_terms(ApED.data(), EDS.data(), EDp.data(), EDp2.data());
+    }
     SolveCfg c = cfg_of(P.ke);
     if (BuildScalarMatrix(TE.data(), BTE.data(), VisTE.data(), TES.data(),
                           ApTE.data(), false))
@@ -860,6 +883,7 @@ int RefNS::UpdateEnergy() {
     }
     if (SetBCTemperature(BTem.data())) return -1;
     if (ht) ht->Solid2Fluid(BTem.data(), M.ncb, M.nbnd);
+    if (!P.steady) transient_terms(ApE.data(), ES.data(), Tnp.data(), Tnp2.data());  // :352-367
     if (BuildScalarMatrix(Tn.data(), BTem.data(), kcond.data(), ES.data(),
                           ApE.data(), true))
         return -1;
@@ -887,15 +911,62 @@ int RefNS::Iterate(double &resmax) {
     return 0;
 }
 
+// NSSolve (src/navier.cpp:13-97): steady runs MaxStep-1 outer iterations at
+// most; transient runs advance dt until total_time, each step iterating until
+// ResMax < 1e-4, a 1000-fold drop from the step's first iteration, or
+// MaxOuterStep-1 iterations
 int RefNS::NSSolve(int &iters) {
     iters = 0;
-    for (int it = 1; it < P.max_step; ++it) {
-        double r;
-        if (Iterate(r)) return -1;
-        iters = it;
-        if (r < P.residual_steady) break;
+    const int max_outer = P.steady ? P.max_step : P.max_outer_step;
+    const double dt = P.steady ? std::numeric_limits<double>::max() : P.dt;  // CYCASHUGE_D
+    cur_time = 0.;
+    cur_time += dt;
+    for (int step = 1; step <= P.max_step; ++step) {
+        if (!P.steady) SaveTransientOldData();
+        double r0 = 0.;
+        for (int it = 1; it < max_outer; ++it) {
+            double r;
+            if (Iterate(r)) return -1;
+            ++iters;
+            if (P.steady) {
+                if (r < P.residual_steady) break;
+            } else {
+                if (it == 1) r0 = r;
+                if (r < 1.e-4 || r0 / (r + 1.e-16) > 1000.) break;
+            }
+        }
+        if (cur_time >= P.total_time) break;
+        cur_time += dt;
     }
     return 0;
+}
+
+// SaveTransientOldData (src/navier.cpp:190-241). For the 2nd-order scheme
+// both history levels take the current value, as in the reference
+// (Unp2[i] = Unp[i] = Un[i]).
+void RefNS::SaveTransientOldData() {
+    if (P.time_scheme == 1) {
+        Rnp = Rn; Unp = Un; Vnp = Vn; Wnp = Wn; Tnp = Tn; TEp = TE; EDp = ED;
+    } else if (P.time_scheme == 2) {
+        Rnp2 = Rnp = Rn; Unp2 = Unp = Un; Vnp2 = Vnp = Vn; Wnp2 = Wnp = Wn;
+        Tnp2 = Tnp = Tn; TEp2 = TEp = TE; EDp2 = EDp = ED;
+    }
+    if (ht) ht->SaveTransientOldData();
+}
+
+// the unsteady contributions of src/navier_scalar.cpp:72-91 and :352-367
+void RefNS::transient_terms(double *Ap, double *src, const double *old1,
+                            const double *old2) const {
+    for (int i = 0; i < M.ncel; ++i) {
+        double coef = Rn[i] / P.dt * M.cell[i].vol;
+        if (P.time_scheme == 1) {
+            Ap[i] += coef;
+            src[i] += coef * old1[i];
+        } else if (P.time_scheme == 2) {
+            Ap[i] += 1.5 * coef;
+            src[i] += coef * (2 * old1[i] - 0.5 * old2[i]);
+        }
+    }
 }
 
 // ----------------------------------------------------- solid module --
@@ -947,12 +1018,12 @@ int RefHT::SetBCTemperature() {
 }
 
 // HeatConductionSolver::BuildScalarMatrix (src/HeatConductionSolver.cpp:110-223)
-int RefHT::BuildScalarMatrix(const double *src) {
+int RefHT::BuildScalarMatrix(const double *src, const double *App) {
     gradient(M, ns->P.limiter, Tn.data(), BTem.data(), dPhidX.data());
     As.constr(M.ncel, false);
     for (int i = 0; i < M.ncel; ++i) {
         const Cell &C = M.cell[i];
-        double app = 0., apn[6] = {0, 0, 0, 0, 0, 0}, sphi = src[i];
+        double app = App[i], apn[6] = {0, 0, 0, 0, 0, 0}, sphi = src[i];
         int ani[6], nj = 0;
         for (int j = 0; j < C.nface; ++j) {
             int iface = C.face[j];
@@ -1016,10 +1087,19 @@ int RefHT::BuildScalarMatrix(const double *src) {
 // HeatConductionSolver::UpdateEnergy (src/HeatConductionSolver.cpp:47-106)
 int RefHT::UpdateEnergy() {
     int n = M.ncel;
-    std::vector<double> ES(n, 0.);
+    std::vector<double> ES(n, 0.), ApE(n, 0.);
     for (int i = 0; i < n; ++i) ES[i] = ns->P.solid_heat_source * M.cell[i].vol;
     if (SetBCTemperature()) return -1;
-    if (BuildScalarMatrix(ES.data())) return -1;
+    // unsteady terms (src/HeatConductionSolver.cpp:68-84): the solid's
+    // TimeScheme is CycasSolver's default 1 (Euler), whatever the fluid uses,
+    // because NavierStokesSolver::TimeScheme shadows it (src/navier.h:29)
+    if (!ns->P.steady)
+        for (int i = 0; i < n; ++i) {
+            double coef = Rn[i] / ns->P.dt * M.cell[i].vol;
+            ApE[i] += coef;
+            ES[i] += coef * Tnp[i];
+        }
+    if (BuildScalarMatrix(ES.data(), ApE.data())) return -1;
     xsol = Tn;
     SolveCfg c;
     c.method = ns->P.solid.method;
@@ -1034,6 +1114,10 @@ int RefHT::UpdateEnergy() {
     Tn = xsol;
     return 0;
 }
+
+// HeatConductionSolver::SaveTransientOldData (src/HeatConductionSolver.cpp:
+// 306-326), TimeScheme 1
+void RefHT::SaveTransientOldData() { Tnp = Tn; }
 
 // coupledBoundCommunicationSolid2Fluid (src/HeatConductionSolver.cpp:400-427)
 void RefHT::Solid2Fluid(double *bt, int ncb, int nb) {

@@ -19,12 +19,14 @@ struct RefHT {
     RefNS *ns = nullptr;
     double URF = 1.0;  // src/Preprocessor.cpp:506
     std::vector<double> Tn, Rn, diffCoef, dPhidX, BTem, xsol, bs;
+    std::vector<double> Tnp, Tnp2;  // transient history
     std::vector<int> colour;
     QMat As;
     SolveResult last;
     int InitFlowField();
     int SetBCTemperature();
-    int BuildScalarMatrix(const double *src);
+    int BuildScalarMatrix(const double *src, const double *App);
+    void SaveTransientOldData();
     int UpdateEnergy();
     void Solid2Fluid(double *bt, int ncb, int nb);
     void Fluid2Solid(RefNS &f);
@@ -43,6 +45,10 @@ struct RefNS {
     std::vector<double> RUFace, dPdX, dUdX, dVdX, dWdX, dPhidX;
     std::vector<double> BRo, BU, BV, BW, BPre, BTem, BTE, BED, kcond;
     std::vector<double> xsol, bu, bv, bw, bp, bs;
+    // transient history (SaveTransientOldData, src/navier.cpp:190-241)
+    std::vector<double> Rnp, Unp, Vnp, Wnp, Tnp, TEp, EDp;
+    std::vector<double> Rnp2, Unp2, Vnp2, Wnp2, Tnp2, TEp2, EDp2;
+    double cur_time = 0.;
     double Residual[10] = {0};
     QMat As, Ap;
     SolveResult last_mom[3], last_pres, last_k, last_e, last_t;
@@ -75,6 +81,9 @@ struct RefNS {
     int UpdateEnergy();
     int Iterate(double &resmax);
     int NSSolve(int &iters);
+    void SaveTransientOldData();
+    // Euler / 2nd-order BDF terms on a diagonal and a source
+    void transient_terms(double *Ap, double *src, const double *old1, const double *old2) const;
     SolveCfg cfg_of(const cfd_solve_cfg &c) const;
 };
 

@@ -50,7 +50,9 @@ class ParamsC(C.Structure):
                 ("cp", C.c_double), ("mom", SolveCfgC), ("pres", SolveCfgC),
                 ("ke", SolveCfgC), ("energy", SolveCfgC), ("solid", SolveCfgC),
                 ("solid_heat_source", C.c_double), ("pres_precision", C.c_int),
-                ("multi_rhs", C.c_int), ("pres_project", C.c_int)]
+                ("multi_rhs", C.c_int), ("pres_project", C.c_int),
+                ("time_scheme", C.c_int), ("dt", C.c_double), ("total_time", C.c_double),
+                ("max_outer_step", C.c_int)]
 
 
 _lib: Optional[C.CDLL] = None
@@ -88,6 +90,7 @@ def lib() -> C.CDLL:
     L.cfd_ns_create_batch.restype = vp
     L.cfd_ns_create_batch.argtypes = [C.c_int, C.POINTER(RawMeshC), C.POINTER(ParamsC)]
     L.cfd_ns_case_range.argtypes = [vp, C.c_int, ip]
+    L.cfd_ns_write_tecplot.argtypes = [vp, C.c_char_p]
     L.cfd_ns_free.argtypes = [vp]
     L.cfd_ns_counts.argtypes = [vp, C.c_int, ip]
     L.cfd_ns_call.argtypes = [vp, C.c_char_p]
@@ -399,6 +402,11 @@ class NSSolver:
         out = np.zeros(8)
         self._L.cfd_ns_profile_read(self.h, _dp(out))
         return out
+
+    def write_tecplot(self, path: str) -> None:
+        """The reference's Tecplot output (fluid zone, solid zone with CHT)."""
+        if self._L.cfd_ns_write_tecplot(self.h, path.encode()):
+            raise RuntimeError(self._error())
 
     # -- pressure-correction initial guess (include/cfd_gnn.h) --
     def capture(self, net: Optional["Gnn"], max_samples: int) -> None:

@@ -73,8 +73,19 @@ struct VelArgs {
     const double *BU, *BV, *BW;
     double urf, grav[3];
     double *diag, *val, *bu, *bv, *bw, *Apr;
+    // transient runs: TimeScheme (0 steady, 1 Euler, 2 BDF), dt, history
+    int tscheme = 0;
+    double dt = 0.;
+    const double *Unp = nullptr, *Vnp = nullptr, *Wnp = nullptr;
+    const double *Unp2 = nullptr, *Vnp2 = nullptr, *Wnp2 = nullptr;
 };
 void launch_build_velocity(const DevMesh &m, const VelArgs &a, int *errflag, hipStream_t s);
+// unsteady terms of a scalar equation (src/navier_scalar.cpp:72-91,352-367,
+// src/HeatConductionSolver.cpp:68-84): coef = R/dt*vol; scheme 1: Ap += coef,
+// src += coef*old1; scheme 2: Ap += 1.5 coef, src += coef*(2 old1 - 0.5 old2)
+void launch_transient(int n, int scheme, double dt, const double *R, const double *vol,
+                      const double *old1, const double *old2, double *Ap, double *src,
+                      hipStream_t s);
 
 void launch_rhie_chow(const DevMesh &m, const double *R, const double *U, const double *V,
                       const double *W, const double *P, const double *gP, const double *Apr,

@@ -318,6 +318,19 @@ __global__ void k_build_velocity(DevMesh m, VelArgs a, int *errflag) {
     const int C = m.ncel, K = m.K;
     CELL_LOOP(i, C) {
         double app = 0., su = 0., sv = 0., sw = 0.;
+        if (a.tscheme == 1) {  // unsteady terms (src/navier_velocity.cpp:103-118)
+            const double coef = a.R[i] / a.dt * m.c_vol[i];
+            app += coef;
+            su += coef * a.Unp[i];
+            sv += coef * a.Vnp[i];
+            sw += coef * a.Wnp[i];
+        } else if (a.tscheme == 2) {
+            const double coef = a.R[i] / a.dt * m.c_vol[i];
+            app += 1.5 * coef;
+            su += coef * (2 * a.Unp[i] - 0.5 * a.Unp2[i]);
+            sv += coef * (2 * a.Vnp[i] - 0.5 * a.Vnp2[i]);
+            sw += coef * (2 * a.Wnp[i] - 0.5 * a.Wnp2[i]);
+        }
         int nj = 0;
         const double cx = m.c_x[0][i], cy = m.c_x[1][i], cz = m.c_x[2][i];
         const double ui = a.U[i], vi = a.V[i], wi = a.W[i];
@@ -1031,6 +1044,32 @@ void launch_remove_mean(double *b, int n, Red red, hipStream_t s) {
                        (const double *)b, mean, red.part, fin);
     hipLaunchKernelGGL(k_sub_scalar, dim3(grid_for(n)), dim3(kBlock), 0, s, n,
                        (const double *)mean, b);
+    CFD_LAUNCH_CHECK();
+}
+
+namespace {
+__global__ void k_transient(int n, int scheme, double dt, const double *__restrict__ R,
+                            const double *__restrict__ vol, const double *__restrict__ o1,
+                            const double *__restrict__ o2, double *__restrict__ Ap,
+                            double *__restrict__ src) {
+    CELL_LOOP(i, n) {
+        const double coef = R[i] / dt * vol[i];
+        if (scheme == 1) {
+            Ap[i] += coef;
+            src[i] += coef * o1[i];
+        } else if (scheme == 2) {
+            Ap[i] += 1.5 * coef;
+            src[i] += coef * (2 * o1[i] - 0.5 * o2[i]);
+        }
+    }
+}
+}  // namespace
+
+void launch_transient(int n, int scheme, double dt, const double *R, const double *vol,
+                      const double *old1, const double *old2, double *Ap, double *src,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_transient, dim3(grid_for(n)), dim3(kBlock), 0, s, n, scheme, dt, R,
+                       vol, old1, old2, Ap, src);
     CFD_LAUNCH_CHECK();
 }
 

@@ -41,6 +41,10 @@ void cfd_params_default(cfd_params *p) {
     p->energy = cfd_solve_cfg{CFD_GMRES, CFD_PC_SSOR, 1.3, 1.e-8, 500}; // navier_scalar.cpp:374
     p->solid = cfd_solve_cfg{CFD_CG, CFD_PC_SSOR, 1.3, 1.e-8, 500};    // HeatConductionSolver.cpp:90
     p->solid_heat_source = 0.;
+    p->time_scheme = 1;      // CycasSolver::TimeScheme (src/BasicSolver.cpp:10)
+    p->dt = 0.;              // src/BasicSolver.cpp:9
+    p->total_time = 0.;      // src/navier.cpp:371
+    p->max_outer_step = 50;  // src/navier.cpp:367
     p->pres_precision = 0;
     p->multi_rhs = 0;
     p->pres_project = 0;

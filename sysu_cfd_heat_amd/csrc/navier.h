@@ -45,6 +45,7 @@ class HTSolver {
     bool colour_ready = false;
     double URF = 1.0;  // src/Preprocessor.cpp:506
     DBuf<double> T, R, D, g, BT, BndQ, xsol, bs, src, diag, val, gtmp;
+    DBuf<double> Tnp, ApE;  // transient runs
     SolveStats last;
     long long iters_total = 0;
     void allocate(hipStream_t s);
@@ -52,6 +53,7 @@ class HTSolver {
     void SetBCTemperature();
     void BuildScalarMatrix();
     int UpdateEnergy();
+    void SaveTransientOldData();
 };
 
 // NavierStokesSolver (src/navier.h:13-127), steady, DensityModel 0
@@ -77,6 +79,11 @@ class NSSolver {
     DBuf<double> As_diag, As_val, Ap_diag, Ap_val;
     DBuf<double> Prod, TES, EDS, ApTE, ApED, VisTE, VisED, zeros;
     DBuf<double> res_dev, gtmp, red_part;
+    // transient runs: history of SaveTransientOldData (src/navier.cpp:190-241)
+    // and the energy equation's unsteady diagonal / source
+    DBuf<double> Unp, Vnp, Wnp, Tnp, TEp, EDp, Unp2, Vnp2, Wnp2, Tnp2, TEp2, EDp2, EnS, EnAp;
+    double cur_time = 0.;
+    void SaveTransientOldData();
     DBuf<BcScal> bcs;
     DBuf<int> errflag;
     double Residual[10] = {0};

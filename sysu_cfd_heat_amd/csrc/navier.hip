@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
+#include <limits>
 
 namespace cfd {
 
@@ -144,7 +146,10 @@ NSSolver::~NSSolver() {
 int NSSolver::setup(const cfd_raw_mesh &raw, const cfd_params &p, std::string &err) {
     P = p;
     if (P.density_model != 0) { err = "only DensityModel 0 (constant density) is supported"; return -1; }
-    if (!P.steady) { err = "only steady runs are supported"; return -1; }
+    if (!P.steady && (P.time_scheme < 1 || P.time_scheme > 2 || !(P.dt > 0.))) {
+        err = "transient runs need TimeScheme 1 or 2 and dt > 0";
+        return -1;
+    }
     if (P.nregion > kMaxRegions - 1) { err = "too many regions"; return -1; }
     reg = RegTab();
     for (int r = 0; r < P.nregion; ++r) {
@@ -299,6 +304,10 @@ int NSSolver::finish_setup(std::string &err) {
     alloc_n({&gU, &gV, &gW, &gP, &gPhi}, 3 * C);
     alloc_n({&BR, &BU, &BV, &BW, &BP, &BT, &BTE, &BED, &BndQ}, B);
     alloc_n({&As_val, &Ap_val}, K * C);
+    if (!P.steady)
+        alloc_n({&Unp, &Vnp, &Wnp, &Tnp, &TEp, &EDp, &Unp2, &Vnp2, &Wnp2, &Tnp2, &TEp2, &EDp2,
+                 &EnS, &EnAp},
+                C);
     RUFace.alloc(std::max<size_t>(F, 1));
     res_dev.alloc(10);
     gtmp.alloc(std::max<size_t>(3 * C, 2 * (size_t)hm.nvrt) + 1);
@@ -353,6 +362,9 @@ void NSSolver::InitFlowField() {
     RUFace.zero(s);
     CFD_HIP(hipStreamSynchronize(s));
     if (ht) ht->InitFlowField();
+    // the history starts as the initial field (NSSolve saves it again at
+    // the start of every time step)
+    if (!P.steady) SaveTransientOldData();
 }
 
 Colouring *NSSolver::colouring_for(const cfd_solve_cfg &c) {
@@ -395,6 +407,11 @@ void NSSolver::BuildVelocityMatrix() {
     a.urf = P.urf[0];
     for (int k = 0; k < 3; ++k) a.grav[k] = P.gravity[k];
     a.diag = As_diag; a.val = As_val; a.bu = bu; a.bv = bv; a.bw = bw; a.Apr = Apr;
+    if (!P.steady) {
+        a.tscheme = P.time_scheme;
+        a.dt = P.dt;
+        a.Unp = Unp; a.Vnp = Vnp; a.Wnp = Wnp; a.Unp2 = Unp2; a.Vnp2 = Vnp2; a.Wnp2 = Wnp2;
+    }
     launch_build_velocity(dm(), a, errflag, s);
     last_as = "velocity";
 }
@@ -509,6 +526,10 @@ int NSSolver::UpdateTurKEpsilon() {
     a.VisED = VisED;
     launch_ke_sources(dm(), a, s);
     launch_ke_wall(dm(), a, s);
+    if (!P.steady) {  // src/navier_scalar.cpp:72-91 (EDSource is rebuilt after the k solve)
+        launch_transient(hm.ncel, P.time_scheme, P.dt, R, dm().c_vol, TEp, TEp2, ApTE, TES, s);
+        launch_transient(hm.ncel, P.time_scheme, P.dt, R, dm().c_vol, EDp, EDp2, ApED, EDS, s);
+    }
     Colouring *col = colouring_for(P.ke);
     BuildScalarMatrix(TE, BTE, VisTE, TES, ApTE, 0);
     last_as = "k";
@@ -541,7 +562,14 @@ int NSSolver::UpdateEnergy() {
     launch_kcond(dm(), VisLam, VisTur, P.cp, P.prl, P.prte, kcond, s);
     SetBCTemperature();
     if (ht) launch_solid2fluid(ht->dmo.dm, ht->T, ht->g, hm.ncb, BT, s);
-    BuildScalarMatrix(T, BT, kcond, zeros, zeros, 1);
+    if (!P.steady) {  // src/navier_scalar.cpp:352-367
+        launch_fill(EnS, hm.ncel, 0., s);
+        launch_fill(EnAp, hm.ncel, 0., s);
+        launch_transient(hm.ncel, P.time_scheme, P.dt, R, dm().c_vol, Tnp, Tnp2, EnAp, EnS, s);
+        BuildScalarMatrix(T, BT, kcond, EnS, EnAp, 1);
+    } else {
+        BuildScalarMatrix(T, BT, kcond, zeros, zeros, 1);
+    }
     last_as = "T";
     CFD_HIP(hipMemcpyAsync(xsol, T, sizeof(double) * hm.ncel, hipMemcpyDeviceToDevice, s));
     last_t = solve(As(), bs, xsol, P.energy, colouring_for(P.energy), 0);
@@ -591,15 +619,56 @@ int NSSolver::Iterate(double &resmax) {
     return 0;
 }
 
+// NSSolve (src/navier.cpp:13-97): steady runs MaxStep-1 outer iterations at
+// most; transient runs advance dt until total_time, each time step iterating
+// until ResMax < 1e-4, a 1000-fold drop from the step's first iteration, or
+// MaxOuterStep-1 iterations
 int NSSolver::NSSolve(int &iters) {
     iters = 0;
-    for (int it = 1; it < P.max_step; ++it) {
-        double r;
-        if (Iterate(r)) return -1;
-        iters = it;
-        if (r < P.residual_steady) break;
+    const int max_outer = P.steady ? P.max_step : P.max_outer_step;
+    const double dt = P.steady ? std::numeric_limits<double>::max() : P.dt;  // CYCASHUGE_D
+    cur_time = 0.;
+    cur_time += dt;
+    for (int step = 1; step <= P.max_step; ++step) {
+        if (!P.steady) SaveTransientOldData();
+        double r0 = 0.;
+        for (int it = 1; it < max_outer; ++it) {
+            double r;
+            if (Iterate(r)) return -1;
+            ++iters;
+            if (P.steady) {
+                if (r < P.residual_steady) break;
+            } else {
+                if (it == 1) r0 = r;
+                if (r < 1.e-4 || r0 / (r + 1.e-16) > 1000.) break;
+            }
+        }
+        if (cur_time >= P.total_time) break;
+        cur_time += dt;
     }
     return 0;
+}
+
+// SaveTransientOldData (src/navier.cpp:190-241): for the 2nd-order scheme
+// both history levels take the current value, as the reference's
+// Unp2[i] = Unp[i] = Un[i] does
+void NSSolver::SaveTransientOldData() {
+    const size_t bytes = sizeof(double) * hm.ncel;
+    const DBuf<double> *src[6] = {&U, &V, &W, &T, &TE, &ED};
+    DBuf<double> *l1[6] = {&Unp, &Vnp, &Wnp, &Tnp, &TEp, &EDp};
+    DBuf<double> *l2[6] = {&Unp2, &Vnp2, &Wnp2, &Tnp2, &TEp2, &EDp2};
+    for (int k = 0; k < 6; ++k) {
+        CFD_HIP(hipMemcpyAsync(l1[k]->p, src[k]->p, bytes, hipMemcpyDeviceToDevice, s));
+        if (P.time_scheme == 2)
+            CFD_HIP(hipMemcpyAsync(l2[k]->p, src[k]->p, bytes, hipMemcpyDeviceToDevice, s));
+    }
+    if (ht) ht->SaveTransientOldData();
+}
+
+// HeatConductionSolver::SaveTransientOldData (src/HeatConductionSolver.cpp:
+// 306-326), TimeScheme 1
+void HTSolver::SaveTransientOldData() {
+    CFD_HIP(hipMemcpyAsync(Tnp.p, T.p, sizeof(double) * hm.ncel, hipMemcpyDeviceToDevice, ns->s));
 }
 
 // ------------------------------------------------------------ solid --
@@ -608,6 +677,7 @@ void HTSolver::allocate(hipStream_t s) {
     alloc_n({&T, &R, &D, &xsol, &bs, &src, &diag}, C);
     alloc_n({&g}, 3 * C);
     alloc_n({&BT, &BndQ}, B);
+    if (!ns->P.steady) alloc_n({&Tnp, &ApE}, C);
     val.alloc(std::max<size_t>(K * C, 1));
     gtmp.alloc(std::max<size_t>(3 * C, 2 * (size_t)hm.nvrt) + 1);
     for (auto *b : {&T, &R, &D, &xsol, &bs, &src, &diag, &g, &BT, &BndQ, &val}) b->zero(s);
@@ -646,7 +716,8 @@ void HTSolver::BuildScalarMatrix() {
     double *out[1] = {g};
     launch_gradient(dmo.dm, 1, phi, bphi, out, ns->P.limiter, gtmp, ns->s);
     ScaArgs a;
-    a.Phi = T; a.BPhi = BT; a.D = D; a.src = src; a.App = nullptr; a.RUFace = nullptr; a.g = g;
+    a.Phi = T; a.BPhi = BT; a.D = D; a.src = src; a.App = ns->P.steady ? nullptr : ApE.p;
+    a.RUFace = nullptr; a.g = g;
     a.BndQ = BndQ; a.urf = URF; a.mode = 2;
     a.diag = diag; a.val = val; a.bs = bs;
     launch_build_scalar(dmo.dm, a, ns->errflag, ns->s);
@@ -657,6 +728,13 @@ int HTSolver::UpdateEnergy() {
     hipStream_t s = ns->s;
     launch_scale_source(dmo.dm, ns->P.solid_heat_source, src, s);
     SetBCTemperature();
+    if (!ns->P.steady) {
+        // src/HeatConductionSolver.cpp:68-84 with the solid's own TimeScheme,
+        // CycasSolver's default 1 (Euler): NavierStokesSolver::TimeScheme
+        // shadows it (src/navier.h:29), so the fluid's choice never reaches it
+        launch_fill(ApE, hm.ncel, 0., s);
+        launch_transient(hm.ncel, 1, ns->P.dt, R, dmo.dm.c_vol, Tnp, Tnp, ApE, src, s);
+    }
     BuildScalarMatrix();
     CFD_HIP(hipMemcpyAsync(xsol, T, sizeof(double) * hm.ncel, hipMemcpyDeviceToDevice, s));
     Colouring *col = nullptr;
@@ -830,6 +908,7 @@ int cfd_ns_call(cfd_ns *h, const char *method) {
         else if (m == "UpdateEnergy") rc = n.UpdateEnergy();
         else if (m == "SolidSolve") rc = n.ht ? n.ht->UpdateEnergy() : -1;
         else if (m == "InitFlowField") n.InitFlowField();
+        else if (m == "SaveTransientOldData") n.SaveTransientOldData();
         else if (m == "GradientU" || m == "GradientV" || m == "GradientW" || m == "GradientP" ||
                  m == "GradientT") {
             char c = m[8];
@@ -960,6 +1039,91 @@ int cfd_ns_profile_read(const cfd_ns *h, double out[8]) {
     out[6] = 24.0 * n.hm.ncel + 8.0 * (double)nnz;
     out[7] = 0.;
     return 0;
+}
+
+int cfd_ns_write_tecplot(cfd_ns *h, const char *path) {
+    return guarded([&]() -> int {
+        NSSolver &n = h->ns;
+        std::ofstream of(path);
+        if (!of) {
+            last_error() = std::string("cannot open ") + path;
+            return -1;
+        }
+        const int C = n.hm.ncel;
+        auto get = [&](const DBuf<double> &b, int cnt) {
+            std::vector<double> v(cnt);
+            b.download(v.data(), cnt, n.s);
+            return v;
+        };
+        auto arr = [&](const std::vector<double> &a) {  // OutArray2File (src/tools.cpp:74-81)
+            for (size_t i = 0; i < a.size(); ++i) {
+                of << a[i] << "  ";
+                if (i % 5 == 4) of << std::endl;
+            }
+            of << std::endl;
+        };
+        auto zeros = [&](int cnt) {  // OutPlaceHolder2File (src/tools.cpp:66-72)
+            for (int i = 0; i < cnt; ++i) {
+                of << 0 << " ";
+                if (i % 5 == 4) of << std::endl;
+            }
+            of << std::endl;
+        };
+        auto zone = [&](const HostMesh &m, int nvar) {
+            of << "zone n=" << m.nvrt << ", e=" << m.ncel << ", VARLOCATION=([1-3]=NODAL,[4-"
+               << nvar << "]=CELLCENTERED)" << "DATAPACKING=BLOCK, ZONETYPE=FEBRICK" << std::endl;
+            for (int j = 0; j < 3; ++j) {
+                for (int i = 0; i < m.nvrt; ++i) {
+                    of << m.vert[3 * (size_t)i + j] << " ";
+                    if (i % 5 == 4) of << std::endl;
+                }
+                of << std::endl;
+            }
+        };
+        auto conn = [&](const HostMesh &m) {
+            for (int i = 0; i < m.ncel; ++i) {
+                for (int j = 0; j < 8; ++j) of << m.cell_vert[8 * (size_t)i + j] + 1 << " ";
+                of << std::endl;
+            }
+        };
+        // the header as the reference writes it (no comma between "z" and "p")
+        of << "variables=" << "\"x\"," << "\"y\"," << "\"z\"" << "\"p\"," << "\"u\","
+           << "\"v\"," << "\"w\"," << "\"ro\"," << "\"T\"," << "\"Mach\"," << "\"mu\"";
+        int nvar = 11;
+        if (n.P.turb_model == 1) {
+            of << "\"te\"" << "\"ed\"";
+            nvar = 13;
+        }
+        of << std::endl;
+        zone(n.hm, nvar);
+        auto u = get(n.U, C), v = get(n.V, C), w = get(n.W, C);
+        arr(get(n.P_, C));
+        arr(u);
+        arr(v);
+        arr(w);
+        arr(get(n.R, C));
+        arr(get(n.T, C));
+        std::vector<double> t(C);
+        for (int i = 0; i < C; ++i) t[i] = std::sqrt(u[i] * u[i] + v[i] * v[i] + w[i] * w[i]);
+        arr(t);
+        auto ml = get(n.VisLam, C), mt = get(n.VisTur, C);
+        for (int i = 0; i < C; ++i) t[i] = ml[i] + mt[i];
+        arr(t);
+        if (n.P.turb_model == 1) {
+            arr(get(n.TE, C));
+            arr(get(n.ED, C));
+        }
+        conn(n.hm);
+        if (n.ht) {
+            const HostMesh &sm = n.ht->hm;
+            zone(sm, nvar);
+            for (int k = 0; k < 5; ++k) zeros(sm.ncel);
+            arr(get(n.ht->T, sm.ncel));
+            for (int k = 10; k != nvar + 1; ++k) zeros(sm.ncel);
+            conn(sm);
+        }
+        return of.good() ? 0 : -1;
+    });
 }
 
 int cfd_ns_stats(const cfd_ns *h, double out[8]) {

@@ -1,0 +1,58 @@
+"""Tecplot field output in the reference's layout (src/navier.cpp:264-340,
+src/HeatConductionSolver.cpp:330-363), read back and compared with the
+device fields (ostream's default 6 significant digits)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HEADER = ('variables="x","y","z""p","u","v","w","ro","T","Mach","mu""te""ed"')
+
+
+def _zones(path):
+    lines = open(path).read().split("\n")
+    assert lines[0] == HEADER
+    toks = " ".join(lines[1:]).split()
+    zones, k = [], 0
+    while k < len(toks):
+        assert toks[k] == "zone", toks[k]
+        nv = int(toks[k + 1][2:-1])
+        nc = int(toks[k + 2][2:-1])
+        assert toks[k + 3] == "VARLOCATION=([1-3]=NODAL,[4-13]=CELLCENTERED)DATAPACKING=BLOCK,"
+        assert toks[k + 4] == "ZONETYPE=FEBRICK"
+        k += 5
+        xyz = np.array(toks[k:k + 3 * nv], float).reshape(3, nv)
+        k += 3 * nv
+        cellv = np.array(toks[k:k + 10 * nc], float).reshape(10, nc)
+        k += 10 * nc
+        conn = np.array(toks[k:k + 8 * nc], int).reshape(nc, 8)
+        k += 8 * nc
+        zones.append((xyz, cellv, conn))
+    return zones
+
+
+def test_tecplot_output_matches_fields(tmp_path):
+    import sysu_cfd_heat_amd as P
+    from sysu_cfd_heat_amd import cases
+    raw, p = cases.channel(nx=10, ny=4, nz=4, max_step=4)
+    s = P.NSSolver(raw, p)
+    for _ in range(2):
+        s.call("Iterate")
+    path = str(tmp_path / "res0000.dat")
+    s.write_tecplot(path)
+    fluid, solid = _zones(path)
+    fm, sm = P.Mesh(raw, p, 0), P.Mesh(raw, p, 1)
+    for (xyz, cellv, conn), m, part in ((fluid, fm, 0), (solid, sm, 1)):
+        np.testing.assert_allclose(xyz.T, m.get("vert"), rtol=1e-5, atol=1e-6)
+        assert np.array_equal(conn, m.get("cell_vert") + 1)
+        T = s.get("Tn", part)
+        np.testing.assert_allclose(cellv[5], T, rtol=1e-5, atol=1e-12)
+    _, cellv, _ = fluid
+    u, v, w = s.get("Un"), s.get("Vn"), s.get("Wn")
+    for row, ref in ((0, s.get("Pn")), (1, u), (2, v), (3, w), (4, s.get("Rn")),
+                     (6, np.sqrt(u * u + v * v + w * w)), (7, s.get("VisLam") + s.get("VisTur")),
+                     (8, s.get("TE")), (9, s.get("ED"))):
+        np.testing.assert_allclose(cellv[row], ref, rtol=1e-5, atol=1e-12 * np.abs(ref).max())
+    _, cellv_s, _ = solid
+    assert np.all(cellv_s[[0, 1, 2, 3, 4, 6, 7, 8, 9]] == 0)
+    s.close()

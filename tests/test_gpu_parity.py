@@ -139,3 +139,56 @@ def test_simple_iterations_match_oracle(case):
     for eq in ("u", "p"):
         it, rr = gpu.last_solve(eq)
         assert rr < 1e-7, (eq, it, rr)
+
+
+# ---- transient runs (src/navier_velocity.cpp:103-118, navier_scalar.cpp:
+# 72-91,352-367, HeatConductionSolver.cpp:68-84, navier.cpp:13-97,190-241) --
+def _transient_pair(kind, scheme, dt, max_outer=3, total=None):
+    P, O = _solvers()
+    from sysu_cfd_heat_amd import cases
+    if kind == "cavity":
+        raw, p = cases.cavity(n=6, re=300.0, max_step=1000)
+    else:
+        raw, p = cases.channel(nx=10, ny=4, nz=4, tets=True, jitter=0.1, seed=2, re=500.0,
+                               max_step=1000)
+    p.steady = 0
+    p.time_scheme = scheme
+    p.dt = dt
+    p.max_outer_step = max_outer
+    p.total_time = 2 * dt if total is None else total
+    return P.NSSolver(raw, p), O.NSSolver(raw, p, mc=1)
+
+
+@pytest.mark.parametrize("kind,scheme", [("cavity", 1), ("channel", 2), ("channel", 1)])
+def test_transient_assembly_and_steps_match_oracle(kind, scheme):
+    gpu, ref = _transient_pair(kind, scheme, 0.02)
+    _random_state(gpu, ref, 4)
+    for s in (gpu, ref):
+        s.call("SaveTransientOldData")  # the random state becomes the history
+    _random_state(gpu, ref, 5)
+    gpu.call("BuildVelocityMatrix")
+    ref.call("BuildVelocityMatrix")
+    for nm in ("bu", "bv", "bw", "Apr"):
+        _cmp(gpu.get(nm), ref.get(nm), kind == "cavity", what=nm)
+    gpu.close()
+    gpu, ref = _transient_pair(kind, scheme, 0.02)
+    for _ in range(2):
+        for s in (gpu, ref):
+            s.call("SaveTransientOldData")
+            for _ in range(2):
+                s.call("Iterate")
+    names = ["Un", "Vn", "Wn", "Pn"] + (["TE", "ED", "Tn"] if kind == "channel" else [])
+    for nm in names:
+        e = rel_l2(gpu.get(nm), ref.get(nm))
+        assert e <= 1e-6, f"{nm}: rel-L2 {e:.3e}"
+    if kind == "channel":
+        assert rel_l2(gpu.get("Tn", part=1), ref.get("Tn", part=1)) <= 1e-6
+
+
+def test_transient_nssolve_loop_matches_oracle():
+    gpu, ref = _transient_pair("cavity", 1, 0.05, max_outer=4, total=0.149)
+    gpu.call("NSSolve")
+    ref.call("NSSolve")
+    assert gpu.stats()[0] == 9  # 3 time steps x 3 outer iterations
+    for nm in ("Un", "Vn", "Wn", "Pn"):
+        assert rel_l2(gpu.get(nm), ref.get(nm)) <= 1e-6, nm

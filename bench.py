@@ -46,6 +46,10 @@ def parse():
                     help="channel: the headline CHT channel (default); cavity: the 32^3 "
                          "Re=100 lid-driven cavity; batch: 64 uneven cavity/channel cases "
                          "packed per launch (extra lines, not the headline)")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="replay pressure CG iterations from a captured HIP graph (cavity and "
+                         "batch workloads; the channel run times its SpMV launches with events, "
+                         "which keeps them eager)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many MiB device-to-device (5 times) and exit: the "
                          "known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE "
@@ -146,6 +150,7 @@ def run_cavity(a):
     from sysu_cfd_heat_amd import NSSolver, cases
     raw, p = cases.cavity(n=32, re=100.0, max_step=a.warmup + a.steps + 2)
     gpu = NSSolver(raw, p)
+    gpu.set_option("graphs", a.graphs)
     for _ in range(a.warmup):
         gpu.call("Iterate")
     gpu._fn("reset_stats")(gpu.h)
@@ -166,8 +171,9 @@ def run_cavity(a):
                "sample": f"oracle, single thread: first {ns} outer iterations of the same case"}
     _line("SIMPLE iters/s, lid-driven cavity 32^3 Re=100", a.steps / el, "SIMPLE iters/s", a,
           1000 * el / a.steps, "cavity 32^3 hexahedra (32768 cells), Re=100, laminar",
-          {"cpu_baseline": cpu, "krylov": {"mom_iters_per_step": st[2] / a.steps,
-                                           "p_iters_per_step": st[3] / a.steps}})
+          {"cpu_baseline": cpu, "graphs": a.graphs,
+           "krylov": {"mom_iters_per_step": st[2] / a.steps,
+                      "p_iters_per_step": st[3] / a.steps}})
 
 
 def run_batch(a):
@@ -181,6 +187,7 @@ def run_batch(a):
     gpu = NSSolver.batch(cs)
     print(f"[bench] batch setup {time.time() - t0:.1f}s, {cells} cells in 64 cases",
           file=sys.stderr, flush=True)
+    gpu.set_option("graphs", a.graphs)
     for _ in range(a.warmup):
         gpu.call("Iterate")
     gpu._fn("reset_stats")(gpu.h)

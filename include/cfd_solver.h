@@ -242,6 +242,13 @@ int cfd_ns_reset_stats(cfd_ns *h);
 int cfd_ns_profile(cfd_ns *h, int enable);
 int cfd_ns_profile_read(const cfd_ns *h, double out[8]);
 
+/* Run-time options (no counterpart in the reference):
+ *   "graphs" 0/1: replay CG iterations from captured HIP graphs (default 1;
+ *   results are bit-identical either way)
+ * cfd_ns_get_option also reads "graph_launches" (CG iterations replayed). */
+int cfd_ns_set_option(cfd_ns *h, const char *name, double value);
+double cfd_ns_get_option(const cfd_ns *h, const char *name);
+
 /* Field output in the reference's Tecplot layout (NavierStokesSolver::Output
  * and Output2Tecplot, src/navier.cpp:264-340; HeatConductionSolver::
  * Output2Tecplot, src/HeatConductionSolver.cpp:330-363): the header line, one

@@ -91,6 +91,9 @@ def lib() -> C.CDLL:
     L.cfd_ns_create_batch.argtypes = [C.c_int, C.POINTER(RawMeshC), C.POINTER(ParamsC)]
     L.cfd_ns_case_range.argtypes = [vp, C.c_int, ip]
     L.cfd_ns_write_tecplot.argtypes = [vp, C.c_char_p]
+    L.cfd_ns_set_option.argtypes = [vp, C.c_char_p, C.c_double]
+    L.cfd_ns_get_option.argtypes = [vp, C.c_char_p]
+    L.cfd_ns_get_option.restype = C.c_double
     L.cfd_ns_free.argtypes = [vp]
     L.cfd_ns_counts.argtypes = [vp, C.c_int, ip]
     L.cfd_ns_call.argtypes = [vp, C.c_char_p]
@@ -402,6 +405,13 @@ class NSSolver:
         out = np.zeros(8)
         self._L.cfd_ns_profile_read(self.h, _dp(out))
         return out
+
+    def set_option(self, name: str, value: float) -> None:
+        if self._L.cfd_ns_set_option(self.h, name.encode(), float(value)):
+            raise RuntimeError(self._error())
+
+    def get_option(self, name: str) -> float:
+        return float(self._L.cfd_ns_get_option(self.h, name.encode()))
 
     def write_tecplot(self, path: str) -> None:
         """The reference's Tecplot output (fluid zone, solid zone with CHT)."""

@@ -1,5 +1,6 @@
 // Device Krylov solvers over ELL matrices (MI355X, gfx950).
 #pragma once
+#include <functional>
 #include <vector>
 
 #include "cfd_solver.h"
@@ -27,6 +28,7 @@ using EllF = EllT<float>;
 // contiguous launch per colour.
 struct Colouring {
     int n = 0, K = 0, ncolour = 0;
+    bool triangle_free = false;    // ILU(0) keeps A's off-diagonals
     const int *src_col = nullptr;  // device structure this was built for
     std::vector<int> off;          // colour block offsets, size ncolour+1
     std::vector<int> colour_host;
@@ -91,6 +93,12 @@ class Krylov {
     int check_first = 4;  // iterations before the first host check
     int check_max = 64;   // batch cap
 
+    // CG iterations replayed from HIP graphs of 1 and 8 iterations, captured
+    // once per matrix / vector / preconditioner configuration (results are
+    // identical to the eager launches). Not used while live timing is on.
+    bool use_graphs = true;
+    long long graph_launches = 0;
+
     // Live timing of the CG SpMV kernel (k_cg_pq): HIP events around each
     // producer launch on this solver's stream; only launches that did work
     // count.
@@ -110,6 +118,12 @@ class Krylov {
     double prof_ms_f32_ = 0.;
     long long prof_launches_f32_ = 0;
     hipEvent_t ev(size_t k);
+    struct GraphEntry {
+        std::vector<uintptr_t> key;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs_;
+    hipGraphExec_t graph_for(const std::vector<uintptr_t> &key, const std::function<void()> &rec);
     int next_batch(int batch, int &prev_it, double &prev_log, int maxit);
     int next_batch_log(int batch, int &prev_it, double &prev_log, int maxit, double lr, double lt);
     // batched cases

@@ -152,25 +152,41 @@ __global__ void k_add_f(int n, double *__restrict__ x, const float *__restrict__
 // product, orders below the stopping tolerance; the returned accuracy is the
 // true residual b - A x recomputed at the end. beta = 0 on the first
 // iteration, where p and q start at zero.
+// Each thread takes its rows two at a time (i and i + one grid stride) so
+// that two rows' gathers are in flight together; the rows a workgroup covers
+// are those of the plain grid-stride loop.
 template <class T>
 __global__ void k_cg_pq(EllT<T> A, const T *__restrict__ z, T *__restrict__ p, T *__restrict__ q,
                         KState *st, double *part, int fin) {
     if (st->done) return;
     const T beta = (T)st->beta;
-    const int n = A.n;
+    const int n = A.n, stride = gridDim.x * blockDim.x;
     double acc[1] = {0.};
-    RED_LOOP_X(i, n) {
-        const T zi = z[i];
-        const T pi = zi + beta * p[i];
-        T s = A.diag[i] * zi;
+    for (int i = fin ? n : xcd_block() * blockDim.x + threadIdx.x; i < n; i += 2 * stride) {
+        const int j = i + stride;
+        const bool hj = j < n;
+        const T zi = z[i], zj = hj ? z[j] : T(0);
+        T si = A.diag[i] * zi, sj = hj ? A.diag[j] * zj : T(0);
         for (int k = 0; k < A.K; ++k) {
-            size_t o = (size_t)k * n + i;
-            s += A.val[o] * z[A.col[o]];
+            const size_t oi = (size_t)k * n + i;
+            si += A.val[oi] * z[A.col[oi]];
+            if (hj) {
+                const size_t oj = (size_t)k * n + j;
+                sj += A.val[oj] * z[A.col[oj]];
+            }
         }
-        s += beta * q[i];
+        const T pi = zi + beta * p[i];
+        si += beta * q[i];
         p[i] = pi;
-        q[i] = s;
-        acc[0] += (double)pi * s;
+        q[i] = si;
+        acc[0] += (double)pi * si;
+        if (hj) {
+            const T pj = zj + beta * p[j];
+            sj += beta * q[j];
+            p[j] = pj;
+            q[j] = sj;
+            acc[0] += (double)pj * sj;
+        }
     }
     if (grid_reduce<1>(acc, part, fin)) {
         st->pq = acc[0];
@@ -532,6 +548,43 @@ __global__ void k_ilu_factor(int n, int K, const int *__restrict__ col, int r0, 
         }
     }
 }
+// ILU(0) of a matrix whose graph has no triangles (no two neighbours of a
+// row are neighbours of each other): no fill position of ILU(0) is then in
+// the pattern except the diagonal, so the factor keeps A's off-diagonals
+// (L = A_lower D^-1, U = A_upper) and only the diagonal changes. Same
+// operations, in the same order, as k_ilu_factor restricted to those updates.
+template <class T>
+__global__ void k_dilu_factor(int n, int K, const int *__restrict__ col,
+                              const T *__restrict__ val, int r0, int r1, T *__restrict__ ud) {
+    int p = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= r1) return;
+    int lc[kIluMaxK], ls[kIluMaxK], nl = 0;
+    for (int k = 0; k < K; ++k) {
+        int cl = col[(size_t)k * n + p];
+        if (cl < p) {
+            int q = nl++;
+            while (q > 0 && lc[q - 1] > cl) {
+                lc[q] = lc[q - 1];
+                ls[q] = ls[q - 1];
+                --q;
+            }
+            lc[q] = cl;
+            ls[q] = k;
+        }
+    }
+    for (int a = 0; a < nl; ++a) {
+        const int c = lc[a];
+        const T l = val[(size_t)ls[a] * n + p] / ud[c];
+        for (int q = 0; q < K; ++q) {
+            const size_t oc = (size_t)q * n + c;
+            if (col[oc] == p) {
+                ud[p] -= l * val[oc];
+                break;
+            }
+        }
+    }
+}
+
 // ILU(0) sweeps over colour blocks [r0, r1) of the permuted matrix, one row
 // per thread in XCD-aware workgroup order. Colour 0 has no earlier
 // neighbours, so its forward result is c itself: it is never launched and
@@ -539,7 +592,9 @@ __global__ void k_ilu_factor(int n, int K, const int *__restrict__ col, int r0, 
 // its backward step y = t / u_pp runs in its forward launch (fuse). With
 // rpart, every workgroup also writes its partial of c.y (CG's rho = r.z) to
 // rpart[blockIdx.x]; the launches of one application use consecutive slices.
-template <class T>
+// D (triangle-free graph): lu holds A's off-diagonals; the lower factor
+// entry is formed as a_pc / u_cc, as k_ilu_factor would have stored it.
+template <class T, bool D>
 __global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col, const T *__restrict__ lu,
                           const T *__restrict__ ud, int r0, int r1, int c0end,
                           const T *__restrict__ c, T *__restrict__ t, T *__restrict__ y, int fuse,
@@ -551,7 +606,7 @@ __global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col, const T *__
         for (int k = 0; k < K; ++k) {
             size_t o = (size_t)k * n + p;
             int cl = col[o];
-            if (cl < p) acc -= lu[o] * (cl < c0end ? c[cl] : t[cl]);
+            if (cl < p) acc -= (D ? lu[o] / ud[cl] : lu[o]) * (cl < c0end ? c[cl] : t[cl]);
         }
         if (fuse) {
             T yv = acc / ud[p];
@@ -566,7 +621,7 @@ __global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col, const T *__
         if (threadIdx.x == 0) rpart[blockIdx.x] = rz[0];
     }
 }
-template <class T>
+template <class T, bool D>
 __global__ void k_ilu_bwd(int n, int K, const int *__restrict__ col, const T *__restrict__ lu,
                           const T *__restrict__ ud, int r0, int r1, int c0end,
                           const T *__restrict__ c, const T *__restrict__ t, T *__restrict__ y,
@@ -647,6 +702,23 @@ void Colouring::build(const int *hc, int n_, int K_, const int *given, const int
             colour_host[i] = c;
         }
     }
+    // no triangles in the graph: no two neighbours of a row are neighbours of
+    // each other (then ILU(0) changes only the diagonal)
+    triangle_free = true;
+    for (int i = 0; i < n && triangle_free; ++i)
+        for (int a = 0; a < K && triangle_free; ++a) {
+            const int j = hc[(size_t)a * n + i];
+            if (j == i) continue;
+            for (int b = 0; b < K && triangle_free; ++b) {
+                const int k = hc[(size_t)b * n + j];
+                if (k == j || k == i) continue;
+                for (int c = 0; c < K; ++c)
+                    if (hc[(size_t)c * n + i] == k) {
+                        triangle_free = false;
+                        break;
+                    }
+            }
+        }
     ncolour = 0;
     for (int c : colour_host) ncolour = std::max(ncolour, c + 1);
     off.assign(ncolour + 1, 0);
@@ -691,6 +763,34 @@ Krylov::Krylov(hipStream_t s) : st_(s) {
 Krylov::~Krylov() {
     if (hstate_) (void)hipHostFree(hstate_);
     for (auto e : evs_) (void)hipEventDestroy(e);
+    for (auto &g : graphs_) (void)hipGraphExecDestroy(g.exec);
+}
+
+// Executable graph of the launches rec() makes on this stream, captured on
+// first use of key (every pointer and size the launches bake in).
+hipGraphExec_t Krylov::graph_for(const std::vector<uintptr_t> &key,
+                                 const std::function<void()> &rec) {
+    for (size_t i = 0; i < graphs_.size(); ++i)
+        if (graphs_[i].key == key) {  // most recently used moves to the back
+            GraphEntry g = graphs_[i];
+            graphs_.erase(graphs_.begin() + i);
+            graphs_.push_back(g);
+            return g.exec;
+        }
+    if (graphs_.size() >= 16) {  // bounded cache: drop the least recently used
+        (void)hipGraphExecDestroy(graphs_.front().exec);
+        graphs_.erase(graphs_.begin());
+    }
+    hipGraph_t graph = nullptr;
+    CFD_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+    rec();
+    CFD_HIP(hipStreamEndCapture(st_, &graph));
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    CFD_HIP(e);
+    graphs_.push_back(GraphEntry{key, exec});
+    return exec;
 }
 
 hipEvent_t Krylov::ev(size_t k) {
@@ -854,9 +954,19 @@ void Krylov::setup_pc(const EllT<T> &A, int pc, Colouring *col, KWork<T> &w) {
     if (pc != CFD_PC_ILU) return;
     const int n = A.n;
     if (A.K > kIluMaxK) throw HipError("ILU(0): more than 16 off-diagonals per row");
+    CFD_HIP(hipMemcpyAsync(w.ud.p, A.diag, sizeof(T) * n, hipMemcpyDeviceToDevice, st_));
+    if (col->triangle_free) {  // only the diagonal changes
+        for (int c = 0; c < col->ncolour; ++c) {
+            int r0 = col->off[c], r1 = col->off[c + 1];
+            if (r1 > r0)
+                hipLaunchKernelGGL(k_dilu_factor<T>, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_,
+                                   n, A.K, A.col, (const T *)A.val, r0, r1, w.ud.p);
+        }
+        CFD_LAUNCH_CHECK();
+        return;
+    }
     CFD_HIP(hipMemcpyAsync(w.lu.p, A.val, sizeof(T) * (size_t)A.K * n, hipMemcpyDeviceToDevice,
                            st_));
-    CFD_HIP(hipMemcpyAsync(w.ud.p, A.diag, sizeof(T) * n, hipMemcpyDeviceToDevice, st_));
     for (int c = 0; c < col->ncolour; ++c) {
         int r0 = col->off[c], r1 = col->off[c + 1];
         if (r1 > r0)
@@ -896,24 +1006,36 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
         const int nc = col->ncolour;
         const int c0end = nc > 1 ? col->off[1] : n;
         double *rp = rz_blocks ? part_.p : nullptr;
+        const bool dilu = col->triangle_free;  // the factor's off-diagonals are A's
+        const T *lu = dilu ? (const T *)A.val : (const T *)w.lu.p;
         int used = 0;
         for (int k = nc > 1 ? 1 : 0; k < nc; ++k) {  // colour 0 only when it is the last
             const int r0 = col->off[k], r1 = col->off[k + 1];
             if (r1 <= r0) continue;
             const bool last = k == nc - 1;
             const int g = grid8(r1 - r0);
-            hipLaunchKernelGGL(k_ilu_fwd<T>, dim3(g), dim3(kBlock), 0, st_, n, A.K, A.col,
-                               (const T *)w.lu.p, (const T *)w.ud.p, r0, r1, c0end, c, w.tmp.p,
-                               y, (int)last, (last && rp) ? rp + used : (double *)nullptr);
+            if (dilu)
+                hipLaunchKernelGGL((k_ilu_fwd<T, true>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
+                                   A.col, lu, (const T *)w.ud.p, r0, r1, c0end, c, w.tmp.p, y,
+                                   (int)last, (last && rp) ? rp + used : (double *)nullptr);
+            else
+                hipLaunchKernelGGL((k_ilu_fwd<T, false>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
+                                   A.col, lu, (const T *)w.ud.p, r0, r1, c0end, c, w.tmp.p, y,
+                                   (int)last, (last && rp) ? rp + used : (double *)nullptr);
             if (last && rp) used += g;
         }
         for (int k = nc - 2; k >= 0; --k) {
             const int r0 = col->off[k], r1 = col->off[k + 1];
             if (r1 <= r0) continue;
             const int g = grid8(r1 - r0);
-            hipLaunchKernelGGL(k_ilu_bwd<T>, dim3(g), dim3(kBlock), 0, st_, n, A.K, A.col,
-                               (const T *)w.lu.p, (const T *)w.ud.p, r0, r1, c0end, c,
-                               (const T *)w.tmp.p, y, rp ? rp + used : (double *)nullptr);
+            if (dilu)
+                hipLaunchKernelGGL((k_ilu_bwd<T, true>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
+                                   A.col, lu, (const T *)w.ud.p, r0, r1, c0end, c,
+                                   (const T *)w.tmp.p, y, rp ? rp + used : (double *)nullptr);
+            else
+                hipLaunchKernelGGL((k_ilu_bwd<T, false>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
+                                   A.col, lu, (const T *)w.ud.p, r0, r1, c0end, c,
+                                   (const T *)w.tmp.p, y, rp ? rp + used : (double *)nullptr);
             if (rp) used += g;
         }
         if (rz_blocks) *rz_blocks = used;
@@ -1091,8 +1213,59 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
     int batch = check_first, issued = 0, prev_it = 0;
     double prev_log = 1e300;
     size_t nev = 0;
+    // one CG iteration after the first pq launch (the same launches every
+    // iteration: p and q are updated in place)
+    auto rest_of_iteration = [&]() {
+        hipLaunchKernelGGL(k_cg_pq<T>, dim3(1), dim3(kBlock), 0, st_, A, (const T *)w.z.p,
+                           w.p0.p, w.q.p, state_.p, part_.p, Gs);
+        if (jac) {
+            CFD_REDUCE_LAUNCH((k_cg_update<true, T>), G, st_, n, x, w.r.p, (const T *)w.p0.p,
+                              (const T *)w.q.p, (const T *)A.diag, w.z.p, state_.p, part_.p,
+                              fin);
+        } else {
+            CFD_REDUCE_LAUNCH((k_cg_update<false, T>), G, st_, n, x, w.r.p, (const T *)w.p0.p,
+                              (const T *)w.q.p, (const T *)A.diag, (T *)nullptr, state_.p,
+                              part_.p, fin);
+            precond_rho();
+        }
+    };
+    // graphs of one and of kGraphIters iterations: a batch of b iterations is
+    // b / kGraphIters launches of the long one and the rest of the short one
+    constexpr int kGraphIters = 8;
+    hipGraphExec_t gexec = nullptr, gexec_long = nullptr;
+    if (use_graphs && !prof && pc != CFD_PC_NONE) {
+        std::vector<uintptr_t> key = {
+            sizeof(T), (uintptr_t)A.diag, (uintptr_t)A.val, (uintptr_t)A.col, (uintptr_t)x,
+            (uintptr_t)b, (uintptr_t)w.r.p, (uintptr_t)w.z.p, (uintptr_t)w.p0.p,
+            (uintptr_t)w.q.p, (uintptr_t)w.tmp.p, (uintptr_t)w.ud.p, (uintptr_t)w.lu.p,
+            (uintptr_t)n, (uintptr_t)A.K, (uintptr_t)pc, (uintptr_t)col,
+            (uintptr_t)(col ? col->ncolour : 0), (uintptr_t)(col ? col->triangle_free : 0)};
+        uint64_t om;
+        std::memcpy(&om, &cfg.omega, sizeof om);
+        key.push_back((uintptr_t)om);
+        if (col)  // the sweeps' launch ranges
+            for (int k = 0; k <= col->ncolour; ++k) key.push_back((uintptr_t)col->off[k]);
+        auto record = [&](int iters) {
+            for (int k = 0; k < iters; ++k) {
+                hipLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, A,
+                                   (const T *)w.z.p, w.p0.p, w.q.p, state_.p, part_.p, 0);
+                rest_of_iteration();
+            }
+        };
+        key.push_back(1);
+        gexec = graph_for(key, [&]() { record(1); });
+        key.back() = kGraphIters;
+        gexec_long = graph_for(key, [&]() { record(kGraphIters); });
+    }
     for (;;) {
-        for (int k = 0; k < batch; ++k) {
+        if (gexec) {
+            int k = 0;
+            for (; k + kGraphIters <= batch; k += kGraphIters)
+                CFD_HIP(hipGraphLaunch(gexec_long, st_));
+            for (; k < batch; ++k) CFD_HIP(hipGraphLaunch(gexec, st_));
+            graph_launches += batch;
+        }
+        for (int k = 0; k < (gexec ? 0 : batch); ++k) {
             if (prof) CFD_HIP(hipEventRecord(ev(2 * nev), st_));
             hipLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, A, (const T *)w.z.p,
                                w.p0.p, w.q.p, state_.p, part_.p, 0);
@@ -1100,18 +1273,7 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
                 CFD_HIP(hipEventRecord(ev(2 * nev + 1), st_));
                 ++nev;
             }
-            hipLaunchKernelGGL(k_cg_pq<T>, dim3(1), dim3(kBlock), 0, st_, A, (const T *)w.z.p,
-                               w.p0.p, w.q.p, state_.p, part_.p, Gs);
-            if (jac) {
-                CFD_REDUCE_LAUNCH((k_cg_update<true, T>), G, st_, n, x, w.r.p, (const T *)w.p0.p,
-                                  (const T *)w.q.p, (const T *)A.diag, w.z.p, state_.p,
-                                  part_.p, fin);
-            } else {
-                CFD_REDUCE_LAUNCH((k_cg_update<false, T>), G, st_, n, x, w.r.p,
-                                  (const T *)w.p0.p, (const T *)w.q.p, (const T *)A.diag,
-                                  (T *)nullptr, state_.p, part_.p, fin);
-                precond_rho();
-            }
+            rest_of_iteration();
         }
         CFD_LAUNCH_CHECK();
         issued += batch;

@@ -1041,6 +1041,24 @@ int cfd_ns_profile_read(const cfd_ns *h, double out[8]) {
     return 0;
 }
 
+int cfd_ns_set_option(cfd_ns *h, const char *name, double value) {
+    std::string k(name);
+    if (k == "graphs") {
+        h->ns.kry->use_graphs = value != 0.;
+        return 0;
+    }
+    last_error() = "unknown option " + k;
+    return -1;
+}
+
+double cfd_ns_get_option(const cfd_ns *h, const char *name) {
+    std::string k(name);
+    if (k == "graphs") return h->ns.kry->use_graphs ? 1. : 0.;
+    if (k == "graph_launches") return (double)h->ns.kry->graph_launches;
+    last_error() = "unknown option " + k;
+    return std::nan("");
+}
+
 int cfd_ns_write_tecplot(cfd_ns *h, const char *path) {
     return guarded([&]() -> int {
         NSSolver &n = h->ns;

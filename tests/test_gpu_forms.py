@@ -55,3 +55,32 @@ def test_mixed_precision_pressure():
     for nm in a:
         e = rel_l2(b[nm], a[nm])
         assert e <= 1e-6, (nm, e)
+
+
+@pytest.mark.parametrize("case", ["cavity", "channel"])
+def test_graph_replay_is_bit_identical(case):
+    """CG iterations replayed from a captured HIP graph give exactly the
+    eager launches' results, solve after solve (several SIMPLE iterations,
+    each with its own pressure solve, eager host work and synchronisations
+    between the replays)."""
+    import sysu_cfd_heat_amd as P
+    from sysu_cfd_heat_amd import cases
+    print(f"[forms] graphs {case}", flush=True)
+    if case == "cavity":
+        raw, p = cases.cavity(n=8, tets=True, jitter=0.1, seed=2, max_step=8)
+    else:
+        raw, p = cases.channel(nx=10, ny=4, nz=4, max_step=8)
+    out = []
+    for graphs in (0, 1):
+        s = P.NSSolver(raw, p)
+        s.set_option("graphs", graphs)
+        for _ in range(4):
+            s.call("Iterate")
+        out.append(({nm: s.get(nm) for nm in ("Un", "Vn", "Pn", "Tn")}, s.stats().copy(),
+                    s.get_option("graph_launches")))
+        s.close()
+    (a, sa, ga), (b, sb, gb) = out
+    assert ga == 0 and gb > 0
+    assert np.array_equal(sa[2:7], sb[2:7])  # Krylov iteration counts
+    for nm in a:
+        assert np.array_equal(a[nm], b[nm]), nm

@@ -125,7 +125,11 @@ typedef struct {
     double residual_steady;/* ResidualSteady */
     int solve_energy;      /* SolveEnergy */
     int turb_model;        /* TurModel: 0 laminar, 1 k-epsilon */
-    int density_model;     /* DensityModel: only 0 (constant) is supported */
+    int density_model;     /* DensityModel: 0 constant density, 1 perfect gas
+                              rho = (p + PressureReference) / (Rcpcv T),
+                              Rcpcv = cp - cp / 1.4; the pressure system is
+                              then non-symmetric and a CG configuration runs
+                              GMRES (src/navier_pressure.cpp:28-31) */
     int cht;               /* Solve3DHeatConduction */
     double gravity[3];
     double pressure_ref;   /* PressureReference */

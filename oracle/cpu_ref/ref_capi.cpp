@@ -153,6 +153,10 @@ int oracle_ns_call(void *h, const char *method) {
     else if (m == "SolidSolve") rc = ns.ht ? ns.ht->UpdateEnergy() : -1;
     else if (m == "InitFlowField") rc = ns.InitFlowField();
     else if (m == "SaveTransientOldData") ns.SaveTransientOldData();
+    else if (m == "EnergyViscSource") {
+        ns.ESource.assign(ns.M.ncel, 0.);
+        ns.energy_visc(ns.ESource.data());
+    }
     else if (m == "GradientU") rc = gradient(ns.M, ns.P.limiter, ns.Un.data(), ns.BU.data(), ns.dUdX.data());
     else if (m == "GradientV") rc = gradient(ns.M, ns.P.limiter, ns.Vn.data(), ns.BV.data(), ns.dVdX.data());
     else if (m == "GradientW") rc = gradient(ns.M, ns.P.limiter, ns.Wn.data(), ns.BW.data(), ns.dWdX.data());
@@ -180,6 +184,7 @@ static std::vector<double> *field_of(RefNS &ns, int part, const std::string &s,
         return nullptr;
     }
     if (s == "Residual") { *raw = ns.Residual; *rawn = 10; return nullptr; }
+    if (s == "ESource") return &ns.ESource;
     std::vector<double> *t[] = {&ns.Rn, &ns.Un, &ns.Vn, &ns.Wn, &ns.Pn, &ns.Tn, &ns.TE, &ns.ED,
                                 &ns.VisLam, &ns.VisTur, &ns.Apr, &ns.RUFace, &ns.dPdX, &ns.dUdX,
                                 &ns.dVdX, &ns.dWdX, &ns.dPhidX, &ns.BRo, &ns.BU, &ns.BV, &ns.BW,

@@ -1,6 +1,7 @@
 // ORACLE -- TEST INFRASTRUCTURE ONLY (see ref.h).
 // Restatement of NavierStokesSolver / HeatConductionSolver's per-iteration
-// path (steady, DensityModel 0). Names follow the reference's methods.
+// path (steady and transient, DensityModel 0 and 1). Names follow the
+// reference's methods.
 #include "ref_navier.h"
 
 #include <algorithm>
@@ -219,6 +220,7 @@ int RefNS::InitFlowField() {
         Rn[i] = iv[3];
         Tn[i] = iv[4];
         Pn[i] = 0.0;
+        if (gas()) Rn[i] = (Pn[i] + P.pressure_ref) / (rcpcv() * Tn[i]);
         VisLam[i] = iv[5];
         VisTur[i] = 0.;
         if (P.turb_model == 1) {
@@ -277,6 +279,7 @@ int RefNS::SetBCVelocity(double *br, double *bu_, double *bv_, double *bw_) {
             return err("no such boundary type in velocity bc");
         }
     }
+    if (gas()) return 0;  // no outlet correction for DensityModel 1 (:58)
     double min_ = 0., mout = 0., rate, aout = 0.;
     for (int i = 0; i < M.nbnd; ++i) {
         int t = reg[M.bnd[i].rid].type1;
@@ -599,10 +602,11 @@ int RefNS::CalRUFace2() {
     return 0;
 }
 
-// BuildPressureMatrix (src/navier_pressure.cpp:78-198), DensityModel 0
+// BuildPressureMatrix (src/navier_pressure.cpp:78-198)
 int RefNS::BuildPressureMatrix() {
     if (SetBCVelocity(BRo.data(), BU.data(), BV.data(), BW.data())) return -1;
     CalRUFace2();
+    if (gas()) return BuildPressureMatrixGas();
     Ap.constr(M.ncel, true);
     for (int i = 0; i < M.ncel; ++i) {
         const Cell &C = M.cell[i];
@@ -636,7 +640,62 @@ int RefNS::BuildPressureMatrix() {
     return 0;
 }
 
-// CorrectRUFace2 (src/navier_pressure.cpp:200-261), DensityModel 0
+// BuildPressureMatrix, DensityModel 1 (src/navier_pressure.cpp:94-100,
+// 118-137, 165-181): full non-symmetric rows, density-change terms upwinded
+// on the face flux
+int RefNS::BuildPressureMatrixGas() {
+    const double Rc = rcpcv();
+    Ap.constr(M.ncel, false);
+    for (int i = 0; i < M.ncel; ++i) {
+        const Cell &C = M.cell[i];
+        double valcen = 0., bpv = 0., Acn[7];
+        int cn[7], nj = 0;
+        if (!P.steady) valcen += -C.vol / (P.dt * Rc * Tn[i]);
+        for (int j = 0; j < C.nface; ++j) {
+            int iface = C.face[j];
+            const Face &F = M.face[iface];
+            if (i == F.c1) bpv += RUFace[iface];
+            else if (i == F.c2) bpv -= RUFace[iface];
+            else return err("face/cell mismatch");
+            int in = C.cell[j];
+            if (in < 0) {
+                int bnd = F.bnd, t = reg[M.bnd[bnd].rid].type1;
+                double RUnormal = RUFace[iface];
+                if (t == 2)
+                    valcen += cmin(RUnormal, 0.) / (BRo[bnd] * Rc * Tn[i]);
+                else if (t == 3)
+                    valcen += cmax(RUnormal, 0.) / (Rn[i] * Rc * Tn[i]);
+                continue;
+            }
+            double lambda, RUnormal;
+            if (i == F.c1) {
+                lambda = F.lambda;
+                RUnormal = RUFace[iface];
+            } else {
+                lambda = 1. - F.lambda;
+                RUnormal = -RUFace[iface];
+            }
+            double l2 = 1. - lambda;
+            double roapf = Rn[i] * Apr[i] * lambda + Rn[in] * Apr[in] * l2;
+            double vol = C.vol * lambda + M.cell[in].vol * l2;
+            double tmp = roapf * vol * F.rlencos;
+            valcen += -tmp;
+            double rof = lambda * Rn[i] + l2 * Rn[in];
+            if (RUnormal > 0.) valcen += -RUnormal / (rof * Rc * Tn[i]);
+            nj++;
+            Acn[nj] = tmp;
+            if (RUnormal < 0.) Acn[nj] += -RUnormal / (rof * Rc * Tn[in]);
+            cn[nj] = in;
+        }
+        Ap.set_len(i, nj + 1);
+        Ap.set_entry(i, 0, i, valcen);
+        for (int j = 1; j <= nj; ++j) Ap.set_entry(i, j, cn[j], Acn[j]);
+        bp[i] = bpv;
+    }
+    return 0;
+}
+
+// CorrectRUFace2 (src/navier_pressure.cpp:200-261)
 int RefNS::CorrectRUFace2(const double *dp) {
     for (int i = 0; i < M.nfac; ++i) {
         const Face &F = M.face[i];
@@ -650,6 +709,11 @@ int RefNS::CorrectRUFace2(const double *dp) {
         double l = F.lambda, l2 = 1. - l, dx1[3], dx2[3];
         sub3(dx1, F.xpac, M.cell[c1].x);
         sub3(dx2, F.xnac, M.cell[c2].x);
+        if (gas()) {  // ro' u (:228-234)
+            double rop = dp[c1] / (rcpcv() * Tn[c1]) * l + dp[c2] / (rcpcv() * Tn[c2]) * l2;
+            double rof = Rn[c1] * l + Rn[c2] * l2;
+            RUFace[i] += RUFace[i] * rop / rof;
+        }
         double roapf = Rn[c1] * Apr[c1] * l + Rn[c2] * Apr[c2] * l2;
         double vol = M.cell[c1].vol * l + M.cell[c2].vol * l2;
         double coef = roapf * vol * F.rlencos;
@@ -660,11 +724,12 @@ int RefNS::CorrectRUFace2(const double *dp) {
     return 0;
 }
 
-// CalculatePressure (src/navier_pressure.cpp:9-76), DensityModel 0
+// CalculatePressure (src/navier_pressure.cpp:9-76); DensityModel 1 solves with
+// GMRES where the configuration names CG (:28-31)
 int RefNS::CalculatePressure() {
     for (int i = 0; i < M.ncel; ++i) xsol[i] = 0.;
     if (BuildPressureMatrix()) return -1;
-    if (P.pres_project) {
+    if (P.pres_project && !gas()) {
         // extension: the pure-Neumann system is consistent only if the
         // imbalances sum to zero; remove their mean (no reference counterpart)
         double s = 0.;
@@ -672,7 +737,9 @@ int RefNS::CalculatePressure() {
         s /= M.ncel;
         for (int i = 0; i < M.ncel; ++i) bp[i] -= s;
     }
-    SolveCfg c = cfg_of(P.pres);
+    cfd_solve_cfg pc = P.pres;
+    if (gas() && pc.method == CFD_CG) pc.method = CFD_GMRES;
+    SolveCfg c = cfg_of(pc);
     last_pres = las_solve(Ap, xsol, bp, c);
     if (last_pres.relres > 1.) return err("pressure iteration cannot converge");
     SetBCDeltaP(BPre.data(), xsol.data());
@@ -680,7 +747,12 @@ int RefNS::CalculatePressure() {
     int pr = P.cell_pressure_ref;
     for (int i = 0; i < M.ncel; ++i) {
         Residual[3] += std::fabs(xsol[i] - xsol[pr]) * M.cell[i].vol;
-        Pn[i] = Pn[i] + URF[3] * (xsol[i] - xsol[pr]);
+        if (!gas()) {
+            Pn[i] = Pn[i] + URF[3] * (xsol[i] - xsol[pr]);
+        } else {  // perfect gas (:45-48)
+            Pn[i] = Pn[i] + URF[3] * xsol[i];
+            Rn[i] += xsol[i] / (rcpcv() * Tn[i]);
+        }
         double coef = M.cell[i].vol * Apr[i];
         Un[i] = Un[i] - coef * dPdX[3 * i];
         Vn[i] = Vn[i] - coef * dPdX[3 * i + 1];
@@ -872,7 +944,55 @@ int RefNS::UpdateTurKEpsilon() {
     return 0;
 }
 
-// UpdateEnergy (src/navier_scalar.cpp:257-397), steady, DensityModel 0
+// UpdateEnergy's viscous-work source, DensityModel 1 (src/navier_scalar.cpp:
+// 277-339): +fvis to a face's first cell, -fvis to its second, in face order
+void RefNS::energy_visc(double *ES) {
+    for (int i = 0; i < M.nfac; ++i) {
+        const Face &F = M.face[i];
+        int c1 = F.c1, c2 = F.c2;
+        double VisL, VisT, du[3], dv[3], dw[3], vxg, vyg, vzg;
+        if (c2 < 0) {
+            VisL = VisLam[c1];
+            VisT = VisTur[c1];
+            for (int k = 0; k < 3; ++k) {
+                du[k] = dUdX[3 * c1 + k];
+                dv[k] = dVdX[3 * c1 + k];
+                dw[k] = dWdX[3 * c1 + k];
+            }
+            vxg = Un[c1];
+            vyg = Vn[c1];
+            vzg = Wn[c1];
+        } else {
+            double l = F.lambda, l2 = 1. - l;
+            VisL = l * VisLam[c1] + l2 * VisLam[c2];
+            VisT = l * VisTur[c1] + l2 * VisTur[c2];
+            for (int k = 0; k < 3; ++k) {
+                du[k] = l * dUdX[3 * c1 + k] + l2 * dUdX[3 * c2 + k];
+                dv[k] = l * dVdX[3 * c1 + k] + l2 * dVdX[3 * c2 + k];
+                dw[k] = l * dWdX[3 * c1 + k] + l2 * dWdX[3 * c2 + k];
+            }
+            vxg = l * Un[c1] + l2 * Un[c2];
+            vyg = l * Vn[c1] + l2 * Vn[c2];
+            vzg = l * Wn[c1] + l2 * Wn[c2];
+        }
+        double vmul = VisL + VisT, vlac = -2. / 3. * vmul;
+        double div = du[0] + dv[1] + dw[2];
+        double txx = 2. * vmul * du[0] + vlac * div;
+        double tyy = 2. * vmul * dv[1] + vlac * div;
+        double tzz = 2. * vmul * dw[2] + vlac * div;
+        double txy = vmul * (du[1] + dv[0]);
+        double txz = vmul * (du[2] + dw[0]);
+        double tyz = vmul * (dv[2] + dw[1]);
+        double btx = vxg * txx + vyg * txy + vzg * txz;
+        double bty = vxg * txy + vyg * tyy + vzg * tyz;
+        double btz = vxg * txz + vyg * tyz + vzg * tzz;
+        double fvis = (btx * F.n[0] + bty * F.n[1] + btz * F.n[2]) / P.cp;
+        ES[c1] += fvis;
+        if (c2 >= 0) ES[c2] -= fvis;
+    }
+}
+
+// UpdateEnergy (src/navier_scalar.cpp:257-397)
 int RefNS::UpdateEnergy() {
     int n = M.ncel;
     std::vector<double> ES(n, 0.), ApE(n, 0.);
@@ -881,6 +1001,7 @@ int RefNS::UpdateEnergy() {
         kcond[i] = P.cp * (VisLam[i] / P.prl + VisTur[i] / P.prte) / P.cp;
         kcond[i] *= 100;
     }
+    if (gas()) energy_visc(ES.data());
     if (SetBCTemperature(BTem.data())) return -1;
     if (ht) ht->Solid2Fluid(BTem.data(), M.ncb, M.nbnd);
     if (!P.steady) transient_terms(ApE.data(), ES.data(), Tnp.data(), Tnp2.data());  // :352-367

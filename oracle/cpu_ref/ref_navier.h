@@ -32,7 +32,7 @@ struct RefHT {
     void Fluid2Solid(RefNS &f);
 };
 
-// NavierStokesSolver (src/navier.h:13-127), steady, DensityModel 0
+// NavierStokesSolver (src/navier.h:13-127), DensityModel 0 and 1
 struct RefNS {
     cfd_params P;
     std::map<int, Region> reg;
@@ -50,6 +50,11 @@ struct RefNS {
     std::vector<double> Rnp2, Unp2, Vnp2, Wnp2, Tnp2, TEp2, EDp2;
     double cur_time = 0.;
     double Residual[10] = {0};
+    std::vector<double> ESource;  // last EnergyViscSource call
+    // DensityModel 1 (perfect gas): gama = 1.4, cv = cp / gama, Rcpcv = cp - cv
+    // (src/navier.cpp:376-382)
+    bool gas() const { return P.density_model == 1; }
+    double rcpcv() const { return P.cp - P.cp / 1.4; }
     QMat As, Ap;
     SolveResult last_mom[3], last_pres, last_k, last_e, last_t;
     std::string error;
@@ -69,6 +74,8 @@ struct RefNS {
     int BuildVelocityMatrix();
     int CalculateVelocity();
     int CalRUFace2();
+    int BuildPressureMatrixGas();
+    void energy_visc(double *ES);
     int BuildPressureMatrix();
     int CorrectRUFace2(const double *dp);
     int CalculatePressure();

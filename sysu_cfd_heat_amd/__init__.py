@@ -284,7 +284,8 @@ FIELD_LEN = {  # name -> (count kind, components)
     "dWdX": ("c", 3), "dPhidX": ("c", 3), "BRo": ("b", 1), "BU": ("b", 1), "BV": ("b", 1),
     "BW": ("b", 1), "BPre": ("b", 1), "BTem": ("b", 1), "BTE": ("b", 1), "BED": ("b", 1),
     "BndQ": ("b", 1), "xsol": ("c", 1), "bu": ("c", 1), "bv": ("c", 1), "bw": ("c", 1),
-    "bp": ("c", 1), "bs": ("c", 1), "diffCoef": ("c", 1), "Residual": ("r", 10)}
+    "bp": ("c", 1), "bs": ("c", 1), "diffCoef": ("c", 1), "ESource": ("c", 1),
+    "Residual": ("r", 10)}
 
 
 def field_size(name: str, counts) -> int:

@@ -98,6 +98,34 @@ def channel(nx=128, ny=64, nz=64, tets=True, jitter=0.1, seed=1, re=1e4, turbule
     return raw, p
 
 
+def perfect_gas(p, t0=1.0, t_wall=None):
+    """Switch a case to DensityModel 1 (rho = p / (Rcpcv T), Rcpcv = cp - cp/1.4):
+    gas and inflow at temperature t0, PressureReference = Rcpcv t0 so that the
+    initial density is 1 (the inflow density is the inlet region's value, 1 in
+    these cases); fixed-temperature walls at t_wall when given."""
+    p.density_model = 1
+    p.pressure_ref = (p.cp - p.cp / 1.4) * t0
+    for k in range(p.nregion):
+        r = p.region[k]
+        if r.type1 == 2:
+            r.init[5] = t0            # inlet: u, v, w, p, ro, T, k, eps
+        elif r.type1 == 5 and r.type2 == 0:
+            r.init[4] = t0            # fluid body: u, v, w, ro, T, mu, k, eps
+        elif r.type1 == 1 and r.type2 == 0 and t_wall is not None:
+            r.fixed = t_wall
+    return p
+
+
+def gas_channel(nx=12, ny=6, nz=6, tets=True, jitter=0.1, seed=1, re=500.0, turbulence=False,
+                t0=1.0, t_wall=1.2, max_step=4, solvers="config", u_in=1.0):
+    """Perfect-gas channel: the channel geometry without the solid block, gas
+    at t0 heated by the bottom wall at t_wall, energy on (DensityModel 1)."""
+    raw, p = channel(nx, ny, nz, tets=tets, jitter=jitter, seed=seed, re=re,
+                     turbulence=turbulence, energy=True, cht=False, heat=0.0,
+                     max_step=max_step, solvers=solvers, u_in=u_in)
+    return raw, perfect_gas(p, t0, t_wall)
+
+
 def batch(ncase=64, seed=1, nmin=8000, nmax=500000, re_lo=100.0, re_hi=2000.0,
           turbulence=True, max_step=12):
     """Batched uneven cases: ncase independent problems, alternately a

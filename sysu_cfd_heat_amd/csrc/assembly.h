@@ -62,9 +62,10 @@ struct Red {
 void launch_gradient(const DevMesh &m, int nf, const double *const *phi,
                      const double *const *bphi, double *const *out, int limiter,
                      double *tmp, hipStream_t s);
+// mass_balance = 0 (DensityModel 1): no outlet correction
 void launch_bc_velocity(const DevMesh &m, const double *R, const double *U, const double *V,
                         const double *W, double *BR, double *BU, double *BV, double *BW,
-                        BcScal *sc, Red red, hipStream_t s);
+                        BcScal *sc, Red red, hipStream_t s, int mass_balance = 1);
 void launch_bc_pressure(const DevMesh &m, const double *P, double *BP, hipStream_t s);
 void launch_bc_temperature(const DevMesh &m, const double *T, double *BT, hipStream_t s);
 
@@ -94,14 +95,33 @@ void launch_rhie_chow(const DevMesh &m, const double *R, const double *U, const 
 void launch_build_pressure(const DevMesh &m, const double *R, const double *Apr,
                            const double *RUFace, double *diag, double *val, double *bp,
                            hipStream_t s);
+// Perfect gas (DensityModel 1, rho = p / (Rcpcv T)): the pressure matrix
+// has full non-symmetric rows with the density-change terms; inv_dt = 1/dt
+// adds the unsteady diagonal term (0 in steady runs)
+void launch_build_pressure_gas(const DevMesh &m, const double *R, const double *Apr,
+                               const double *RUFace, const double *T, const double *BR,
+                               double rcpcv, double dt, double *diag, double *val, double *bp,
+                               hipStream_t s);
+// T (DensityModel 1): the density-change flux ro' u is added first
 void launch_correct_ruface(const DevMesh &m, const double *R, const double *Apr,
                            const double *dp, const double *gdp, const double *BR,
                            const double *BU, const double *BV, const double *BW, double urf_p,
-                           double *RUFace, hipStream_t s);
-// P += urf (x - x_ref); U -= vol Apr dPdX ; Residual[3] += |x - x_ref| vol
+                           double *RUFace, hipStream_t s, const double *T = nullptr,
+                           double rcpcv = 0.);
+// P += urf (x - x_ref); U -= vol Apr dPdX ; Residual[3] += |x - x_ref| vol.
+// gasR (DensityModel 1): P += urf x and gasR += x / (Rcpcv T) instead
 void launch_pressure_update(const DevMesh &m, const double *x, int ref, double urf_p,
                             const double *Apr, const double *gx, double *P, double *U,
-                            double *V, double *W, double *res_slot, Red red, hipStream_t s);
+                            double *V, double *W, double *res_slot, Red red, hipStream_t s,
+                            double *gasR = nullptr, const double *T = nullptr,
+                            double rcpcv = 0.);
+// energy source of the viscous stresses' work (DensityModel 1,
+// src/navier_scalar.cpp:277-339): ES = sum over a cell's faces of
+// +-(tau . u)_f . n / cp, face values interpolated
+void launch_energy_visc(const DevMesh &m, const double *gU, const double *gV, const double *gW,
+                        const double *U, const double *V, const double *W,
+                        const double *VisLam, const double *VisTur, double cp, double *ES,
+                        hipStream_t s);
 // res_slot += sum |fld - x| vol ; fld = x (clip applied to x first if lo set)
 void launch_take_solution(const DevMesh &m, const double *x, double *fld, double *res_slot,
                           double clip_lo, int clip, Red red, hipStream_t s);

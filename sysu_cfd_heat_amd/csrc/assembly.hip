@@ -523,13 +523,128 @@ __global__ void k_build_pressure(DevMesh m, const double *__restrict__ R,
     }
 }
 
-// CorrectRUFace2 (src/navier_pressure.cpp:200-242), DensityModel 0
+// BuildPressureMatrix (src/navier_pressure.cpp:88-195), DensityModel 1: each
+// row from its own cell's side in face-slot order. Unsteady runs add
+// -vol / (dt Rcpcv T) to the diagonal; inlet and outlet faces add the
+// inflow / outflow part of the boundary flux over rho Rcpcv T; an interior
+// face with outflow adds -RU / (rho_f Rcpcv T_i) to the diagonal, one with
+// inflow adds -RU / (rho_f Rcpcv T_neighbour) to the neighbour entry.
+__global__ void k_build_pressure_gas(DevMesh m, const double *__restrict__ R,
+                                     const double *__restrict__ Apr,
+                                     const double *__restrict__ RUFace,
+                                     const double *__restrict__ T, const double *__restrict__ BR,
+                                     double rcpcv, double dt, double *__restrict__ diag,
+                                     double *__restrict__ val, double *__restrict__ bp) {
+    const int C = m.ncel, K = m.K;
+    CELL_LOOP(i, C) {
+        double valcen = 0., bpv = 0.;
+        if (dt > 0.) valcen += -m.c_vol[i] / (dt * rcpcv * T[i]);
+        int nj = 0;
+        const int nface = m.c_nface[i];
+        for (int j = 0; j < nface; ++j) {
+            const int f = m.c_face[j * C + i];
+            const int c1 = m.f_c1[f], c2 = m.f_c2[f];
+            if (i == c1) bpv += RUFace[f];
+            else bpv -= RUFace[f];
+            if (c2 < 0) {
+                const int b = m.f_bnd[f];
+                const int t1 = m.reg->type1[m.b_reg[b]];
+                const double ru = RUFace[f];
+                if (t1 == 2)
+                    valcen += cmin(ru, 0.) / (BR[b] * rcpcv * T[i]);
+                else if (t1 == 3)
+                    valcen += cmax(ru, 0.) / (R[i] * rcpcv * T[i]);
+                continue;
+            }
+            const int in = i == c1 ? c2 : c1;
+            const double lambda = i == c1 ? m.f_lambda[f] : 1. - m.f_lambda[f];
+            const double ru = i == c1 ? RUFace[f] : -RUFace[f];
+            const double l2 = 1. - lambda;
+            const double roapf = R[i] * Apr[i] * lambda + R[in] * Apr[in] * l2;
+            const double vol = m.c_vol[i] * lambda + m.c_vol[in] * l2;
+            const double tmp = roapf * vol * m.f_rlencos[f];
+            valcen += -tmp;
+            const double rof = lambda * R[i] + l2 * R[in];
+            if (ru > 0.) valcen += -ru / (rof * rcpcv * T[i]);
+            double a = tmp;
+            if (ru < 0.) a += -ru / (rof * rcpcv * T[in]);
+            if (nj < K) val[(size_t)nj * C + i] = a;
+            nj++;
+        }
+        for (int k = nj; k < K; ++k) val[(size_t)k * C + i] = 0.;
+        diag[i] = valcen;
+        bp[i] = bpv;
+    }
+}
+
+// UpdateEnergy's viscous-work source (src/navier_scalar.cpp:277-339): the
+// reference scatters +fvis to a face's first cell and -fvis to its second in
+// face order; each cell gathers its faces in ascending face id, the same sums
+// in the same order.
+__global__ void k_energy_visc(DevMesh m, const double *__restrict__ gU,
+                              const double *__restrict__ gV, const double *__restrict__ gW,
+                              const double *__restrict__ U, const double *__restrict__ V,
+                              const double *__restrict__ W, const double *__restrict__ VisLam,
+                              const double *__restrict__ VisTur, double cp,
+                              double *__restrict__ ES) {
+    const int C = m.ncel;
+    CELL_LOOP(i, C) {
+        double es = 0.;
+        const int nf = m.c_nface[i];
+        for (int t = 0; t < nf; ++t) {
+            const int f = m.c_gface[t * C + i];
+            const int c1 = m.f_c1[f], c2 = m.f_c2[f];
+            double visl, vist, ux, uy, uz, vx, vy, vz, wx, wy, wz, ug, vg, wg;
+            if (c2 < 0) {
+                visl = VisLam[c1]; vist = VisTur[c1];
+                ux = gU[c1]; uy = gU[C + c1]; uz = gU[2 * C + c1];
+                vx = gV[c1]; vy = gV[C + c1]; vz = gV[2 * C + c1];
+                wx = gW[c1]; wy = gW[C + c1]; wz = gW[2 * C + c1];
+                ug = U[c1]; vg = V[c1]; wg = W[c1];
+            } else {
+                const double l = m.f_lambda[f], l2 = 1. - l;
+                visl = l * VisLam[c1] + l2 * VisLam[c2];
+                vist = l * VisTur[c1] + l2 * VisTur[c2];
+                ux = l * gU[c1] + l2 * gU[c2];
+                uy = l * gU[C + c1] + l2 * gU[C + c2];
+                uz = l * gU[2 * C + c1] + l2 * gU[2 * C + c2];
+                vx = l * gV[c1] + l2 * gV[c2];
+                vy = l * gV[C + c1] + l2 * gV[C + c2];
+                vz = l * gV[2 * C + c1] + l2 * gV[2 * C + c2];
+                wx = l * gW[c1] + l2 * gW[c2];
+                wy = l * gW[C + c1] + l2 * gW[C + c2];
+                wz = l * gW[2 * C + c1] + l2 * gW[2 * C + c2];
+                ug = l * U[c1] + l2 * U[c2];
+                vg = l * V[c1] + l2 * V[c2];
+                wg = l * W[c1] + l2 * W[c2];
+            }
+            const double vmul = visl + vist, vlac = -2. / 3. * vmul;
+            const double div = ux + vy + wz;
+            const double txx = 2. * vmul * ux + vlac * div;
+            const double tyy = 2. * vmul * vy + vlac * div;
+            const double tzz = 2. * vmul * wz + vlac * div;
+            const double txy = vmul * (uy + vx);
+            const double txz = vmul * (uz + wx);
+            const double tyz = vmul * (vz + wy);
+            const double btx = ug * txx + vg * txy + wg * txz;
+            const double bty = ug * txy + vg * tyy + wg * tyz;
+            const double btz = ug * txz + vg * tyz + wg * tzz;
+            const double fvis = (btx * m.f_n[0][f] + bty * m.f_n[1][f] + btz * m.f_n[2][f]) / cp;
+            if (i == c1) es += fvis;
+            else es -= fvis;
+        }
+        ES[i] = es;
+    }
+}
+
+// CorrectRUFace2 (src/navier_pressure.cpp:200-242); T set: DensityModel 1
 __global__ void k_correct_ruface(DevMesh m, const double *__restrict__ R,
                                  const double *__restrict__ Apr, const double *__restrict__ dp,
                                  const double *__restrict__ g, const double *__restrict__ BR,
                                  const double *__restrict__ BU, const double *__restrict__ BV,
                                  const double *__restrict__ BW, double urf,
-                                 double *__restrict__ RUFace) {
+                                 double *__restrict__ RUFace, const double *__restrict__ T,
+                                 double rcpcv) {
     const int C = m.ncel;
     CELL_LOOP(f, m.nfac) {
         const int c1 = m.f_c1[f], c2 = m.f_c2[f];
@@ -544,6 +659,11 @@ __global__ void k_correct_ruface(DevMesh m, const double *__restrict__ R,
                      a2 = m.f_xp[2][f] - m.c_x[2][c1];
         const double b0 = m.f_xn[0][f] - m.c_x[0][c2], b1 = m.f_xn[1][f] - m.c_x[1][c2],
                      b2 = m.f_xn[2][f] - m.c_x[2][c2];
+        if (T) {  // DensityModel 1: ro' u (src/navier_pressure.cpp:228-234)
+            const double rop = dp[c1] / (rcpcv * T[c1]) * l + dp[c2] / (rcpcv * T[c2]) * l2;
+            const double rof = R[c1] * l + R[c2] * l2;
+            RUFace[f] += RUFace[f] * rop / rof;
+        }
         const double roapf = R[c1] * Apr[c1] * l + R[c2] * Apr[c2] * l2;
         const double vol = m.c_vol[c1] * l + m.c_vol[c2] * l2;
         const double coef = roapf * vol * m.f_rlencos[f];
@@ -558,14 +678,20 @@ __global__ void k_pressure_update(DevMesh m, const double *__restrict__ x, int r
                                   const double *__restrict__ Apr, const double *__restrict__ g,
                                   double *__restrict__ P, double *__restrict__ U,
                                   double *__restrict__ V, double *__restrict__ W,
-                                  double *res_slot, double *part, int fin) {
+                                  double *res_slot, double *gasR, const double *__restrict__ T,
+                                  double rcpcv, double *part, int fin) {
     const int C = m.ncel;
     double acc[1] = {0.};
     RED_LOOP(i, C) {
         const double xr = x[m.case_ref ? m.case_ref[m.c_case[i]] : ref];
         const double vol = m.c_vol[i], d = x[i] - xr;
         acc[0] += fabs(d) * vol;
-        P[i] = P[i] + urf * d;
+        if (gasR) {  // DensityModel 1 (src/navier_pressure.cpp:45-48)
+            P[i] = P[i] + urf * x[i];
+            gasR[i] += x[i] / (rcpcv * T[i]);
+        } else {
+            P[i] = P[i] + urf * d;
+        }
         const double coef = vol * Apr[i];
         U[i] = U[i] - coef * g[i];
         V[i] = V[i] - coef * g[C + i];
@@ -907,7 +1033,7 @@ void launch_gradient(const DevMesh &m, int nf, const double *const *phi,
 
 void launch_bc_velocity(const DevMesh &m, const double *R, const double *U, const double *V,
                         const double *W, double *BR, double *BU, double *BV, double *BW,
-                        BcScal *sc, Red red, hipStream_t s) {
+                        BcScal *sc, Red red, hipStream_t s, int mass_balance) {
     if (m.nbnd == 0) return;
     CFD_REDUCE_LAUNCH(k_bc_velocity, red_grid(m.nbnd), s, m, R, U, V, W,
                        BR, BU, BV, BW, sc, red.part, fin);
@@ -915,8 +1041,9 @@ void launch_bc_velocity(const DevMesh &m, const double *R, const double *U, cons
         hipLaunchKernelGGL(k_bc_case_sums, dim3(m.ncase), dim3(kBlock), 0, s, m,
                            (const double *)BR, (const double *)BU, (const double *)BV,
                            (const double *)BW, sc);
-    hipLaunchKernelGGL(k_bc_outlet, dim3(grid_for(m.nbnd)), dim3(kBlock), 0, s, m,
-                       (const BcScal *)sc, BU, BV, BW);
+    if (mass_balance)
+        hipLaunchKernelGGL(k_bc_outlet, dim3(grid_for(m.nbnd)), dim3(kBlock), 0, s, m,
+                           (const BcScal *)sc, BU, BV, BW);
     CFD_LAUNCH_CHECK();
 }
 
@@ -954,20 +1081,39 @@ void launch_build_pressure(const DevMesh &m, const double *R, const double *Apr,
     CFD_LAUNCH_CHECK();
 }
 
+void launch_build_pressure_gas(const DevMesh &m, const double *R, const double *Apr,
+                               const double *RUFace, const double *T, const double *BR,
+                               double rcpcv, double dt, double *diag, double *val, double *bp,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_build_pressure_gas, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m, R,
+                       Apr, RUFace, T, BR, rcpcv, dt, diag, val, bp);
+    CFD_LAUNCH_CHECK();
+}
+
+void launch_energy_visc(const DevMesh &m, const double *gU, const double *gV, const double *gW,
+                        const double *U, const double *V, const double *W,
+                        const double *VisLam, const double *VisTur, double cp, double *ES,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_energy_visc, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m, gU, gV, gW,
+                       U, V, W, VisLam, VisTur, cp, ES);
+    CFD_LAUNCH_CHECK();
+}
+
 void launch_correct_ruface(const DevMesh &m, const double *R, const double *Apr,
                            const double *dp, const double *gdp, const double *BR,
                            const double *BU, const double *BV, const double *BW, double urf_p,
-                           double *RUFace, hipStream_t s) {
+                           double *RUFace, hipStream_t s, const double *T, double rcpcv) {
     hipLaunchKernelGGL(k_correct_ruface, dim3(grid_for(m.nfac)), dim3(kBlock), 0, s, m, R, Apr,
-                       dp, gdp, BR, BU, BV, BW, urf_p, RUFace);
+                       dp, gdp, BR, BU, BV, BW, urf_p, RUFace, T, rcpcv);
     CFD_LAUNCH_CHECK();
 }
 
 void launch_pressure_update(const DevMesh &m, const double *x, int ref, double urf_p,
                             const double *Apr, const double *gx, double *P, double *U,
-                            double *V, double *W, double *res_slot, Red red, hipStream_t s) {
+                            double *V, double *W, double *res_slot, Red red, hipStream_t s,
+                            double *gasR, const double *T, double rcpcv) {
     CFD_REDUCE_LAUNCH(k_pressure_update, red_grid(m.ncel), s, m, x, ref,
-                       urf_p, Apr, gx, P, U, V, W, res_slot, red.part, fin);
+                       urf_p, Apr, gx, P, U, V, W, res_slot, gasR, T, rcpcv, red.part, fin);
     CFD_LAUNCH_CHECK();
 }
 

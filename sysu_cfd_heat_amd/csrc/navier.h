@@ -56,7 +56,8 @@ class HTSolver {
     void SaveTransientOldData();
 };
 
-// NavierStokesSolver (src/navier.h:13-127), steady, DensityModel 0
+// NavierStokesSolver (src/navier.h:13-127): steady and transient runs,
+// DensityModel 0 (constant density) and 1 (perfect gas)
 class NSSolver {
   public:
     cfd_params P{};
@@ -83,6 +84,10 @@ class NSSolver {
     // and the energy equation's unsteady diagonal / source
     DBuf<double> Unp, Vnp, Wnp, Tnp, TEp, EDp, Unp2, Vnp2, Wnp2, Tnp2, TEp2, EDp2, EnS, EnAp;
     double cur_time = 0.;
+    // perfect gas (DensityModel 1): rho = p / (Rcpcv T), Rcpcv = cp - cp / gama
+    // with gama = 1.4 (src/navier.cpp:376-382)
+    bool gas() const { return P.density_model == 1; }
+    double rcpcv() const { return P.cp - P.cp / 1.4; }
     void SaveTransientOldData();
     DBuf<BcScal> bcs;
     DBuf<int> errflag;

@@ -145,7 +145,14 @@ NSSolver::~NSSolver() {
 
 int NSSolver::setup(const cfd_raw_mesh &raw, const cfd_params &p, std::string &err) {
     P = p;
-    if (P.density_model != 0) { err = "only DensityModel 0 (constant density) is supported"; return -1; }
+    if (P.density_model != 0 && P.density_model != 1) {
+        err = "DensityModel 0 (constant density) or 1 (perfect gas)";
+        return -1;
+    }
+    if (P.density_model == 1 && P.pres_precision != 0) {
+        err = "DensityModel 1: the pressure system is not symmetric, no mixed-precision CG form";
+        return -1;
+    }
     if (!P.steady && (P.time_scheme < 1 || P.time_scheme > 2 || !(P.dt > 0.))) {
         err = "transient runs need TimeScheme 1 or 2 and dt > 0";
         return -1;
@@ -305,9 +312,9 @@ int NSSolver::finish_setup(std::string &err) {
     alloc_n({&BR, &BU, &BV, &BW, &BP, &BT, &BTE, &BED, &BndQ}, B);
     alloc_n({&As_val, &Ap_val}, K * C);
     if (!P.steady)
-        alloc_n({&Unp, &Vnp, &Wnp, &Tnp, &TEp, &EDp, &Unp2, &Vnp2, &Wnp2, &Tnp2, &TEp2, &EDp2,
-                 &EnS, &EnAp},
+        alloc_n({&Unp, &Vnp, &Wnp, &Tnp, &TEp, &EDp, &Unp2, &Vnp2, &Wnp2, &Tnp2, &TEp2, &EDp2},
                 C);
+    if (!P.steady || gas()) alloc_n({&EnS, &EnAp}, C);
     RUFace.alloc(std::max<size_t>(F, 1));
     res_dev.alloc(10);
     gtmp.alloc(std::max<size_t>(3 * C, 2 * (size_t)hm.nvrt) + 1);
@@ -337,6 +344,7 @@ void NSSolver::InitFlowField() {
     for (int i = 0; i < C; ++i) {
         const double *iv = reg.init[slot_of[hm.cell_rid[i]]];
         u[i] = iv[0]; v[i] = iv[1]; w[i] = iv[2]; r[i] = iv[3]; t[i] = iv[4];
+        if (gas()) r[i] = (pz[i] + P.pressure_ref) / (rcpcv() * t[i]);
         ml[i] = iv[5];
         if (P.turb_model == 1) {
             k[i] = iv[6];
@@ -384,7 +392,7 @@ SolveStats NSSolver::solve(const Ell &A, const double *b, double *x, const cfd_s
 }
 
 void NSSolver::SetBCVelocity() {
-    launch_bc_velocity(dm(), R, U, V, W, BR, BU, BV, BW, bcs.p, red(), s);
+    launch_bc_velocity(dm(), R, U, V, W, BR, BU, BV, BW, bcs.p, red(), s, gas() ? 0 : 1);
 }
 void NSSolver::SetBCPressure() { launch_bc_pressure(dm(), P_, BP, s); }
 void NSSolver::SetBCDeltaP() { launch_bc_pressure(dm(), xsol, BP, s); }
@@ -467,24 +475,33 @@ void NSSolver::CalRUFace2() {
 void NSSolver::BuildPressureMatrix() {
     SetBCVelocity();
     CalRUFace2();
-    launch_build_pressure(dm(), R, Apr, RUFace, Ap_diag, Ap_val, bp, s);
+    if (gas())
+        launch_build_pressure_gas(dm(), R, Apr, RUFace, T, BR, rcpcv(), P.steady ? 0. : P.dt,
+                                  Ap_diag, Ap_val, bp, s);
+    else
+        launch_build_pressure(dm(), R, Apr, RUFace, Ap_diag, Ap_val, bp, s);
 }
 
 void NSSolver::CorrectRUFace2() {
-    launch_correct_ruface(dm(), R, Apr, xsol, gP, BR, BU, BV, BW, P.urf[3], RUFace, s);
+    launch_correct_ruface(dm(), R, Apr, xsol, gP, BR, BU, BV, BW, P.urf[3], RUFace, s,
+                          gas() ? (const double *)T : nullptr, rcpcv());
 }
 
-// CalculatePressure (src/navier_pressure.cpp:9-76), DensityModel 0
+// CalculatePressure (src/navier_pressure.cpp:9-76). DensityModel 1 solves the
+// non-symmetric system with GMRES where the configuration names CG (the
+// reference's choice, :28-31), and corrects density with the pressure.
 int NSSolver::CalculatePressure() {
     launch_fill(xsol, hm.ncel, 0., s);
     BuildPressureMatrix();
-    if (P.pres_project) {
+    if (P.pres_project && !gas()) {
         if (cases.ncase() > 1) launch_remove_case_means(dm(), bp, red(), s);
         else launch_remove_mean(bp, hm.ncel, red(), s);
     }
     pressure_guess();
     kry->prof = prof_pressure;
-    last_p = solve(Ap(), bp, xsol, P.pres, colouring_for(P.pres), P.pres_precision);
+    cfd_solve_cfg pc = P.pres;
+    if (gas() && pc.method == CFD_CG) pc.method = CFD_GMRES;
+    last_p = solve(Ap(), bp, xsol, pc, colouring_for(pc), P.pres_precision);
     kry->prof = false;
     stats[3] += last_p.iters;
     if (!(last_p.relres <= 1.)) {
@@ -498,7 +515,7 @@ int NSSolver::CalculatePressure() {
     double *out[1] = {gP};
     launch_gradient(dm(), 1, phi, bphi, out, P.limiter, gtmp, s);
     launch_pressure_update(dm(), xsol, P.cell_pressure_ref, P.urf[3], Apr, gP, P_, U, V, W,
-                           res_dev.p + 3, red(), s);
+                           res_dev.p + 3, red(), s, gas() ? (double *)R : nullptr, T, rcpcv());
     SetBCVelocity();
     CorrectRUFace2();
     return 0;
@@ -557,15 +574,19 @@ int NSSolver::UpdateTurKEpsilon() {
     return 0;
 }
 
-// UpdateEnergy (src/navier_scalar.cpp:257-397), steady, DensityModel 0
+// UpdateEnergy (src/navier_scalar.cpp:257-397); DensityModel 1 adds the
+// viscous stresses' work to the source (:277-339)
 int NSSolver::UpdateEnergy() {
     launch_kcond(dm(), VisLam, VisTur, P.cp, P.prl, P.prte, kcond, s);
+    if (gas()) launch_energy_visc(dm(), gU, gV, gW, U, V, W, VisLam, VisTur, P.cp, EnS, s);
     SetBCTemperature();
     if (ht) launch_solid2fluid(ht->dmo.dm, ht->T, ht->g, hm.ncb, BT, s);
-    if (!P.steady) {  // src/navier_scalar.cpp:352-367
-        launch_fill(EnS, hm.ncel, 0., s);
+    if (!P.steady || gas()) {
+        if (!gas()) launch_fill(EnS, hm.ncel, 0., s);
         launch_fill(EnAp, hm.ncel, 0., s);
-        launch_transient(hm.ncel, P.time_scheme, P.dt, R, dm().c_vol, Tnp, Tnp2, EnAp, EnS, s);
+        if (!P.steady)  // src/navier_scalar.cpp:352-367
+            launch_transient(hm.ncel, P.time_scheme, P.dt, R, dm().c_vol, Tnp, Tnp2, EnAp, EnS,
+                             s);
         BuildScalarMatrix(T, BT, kcond, EnS, EnAp, 1);
     } else {
         BuildScalarMatrix(T, BT, kcond, zeros, zeros, 1);
@@ -822,6 +843,9 @@ FieldRef field_of(cfd_ns_s *h, int part, const std::string &s) {
                {"bp", &n.bp, C, false},      {"bs", &n.bs, C, false}};
     for (auto &e : tab)
         if (s == e.nm) return FieldRef{e.b, e.n, e.v3};
+    // energy source of the last UpdateEnergy / EnergyViscSource (transient or
+    // perfect-gas runs)
+    if (s == "ESource" && n.EnS.n >= C) return FieldRef{&n.EnS, C, false};
     return f;
 }
 
@@ -909,6 +933,14 @@ int cfd_ns_call(cfd_ns *h, const char *method) {
         else if (m == "SolidSolve") rc = n.ht ? n.ht->UpdateEnergy() : -1;
         else if (m == "InitFlowField") n.InitFlowField();
         else if (m == "SaveTransientOldData") n.SaveTransientOldData();
+        else if (m == "EnergyViscSource") {
+            if (n.EnS.n < (size_t)n.hm.ncel) {
+                last_error() = "EnergyViscSource: perfect-gas or transient runs only";
+                return -1;
+            }
+            launch_energy_visc(n.dm(), n.gU, n.gV, n.gW, n.U, n.V, n.W, n.VisLam, n.VisTur,
+                               n.P.cp, n.EnS, n.s);
+        }
         else if (m == "GradientU" || m == "GradientV" || m == "GradientW" || m == "GradientP" ||
                  m == "GradientT") {
             char c = m[8];
@@ -979,7 +1011,7 @@ int cfd_ns_matrix(cfd_ns *h, const char *which, int *row_len, int *col, double *
         DBuf<double> *dg, *vl;
         bool upper = false;
         if (w == "As") { dg = &n.As_diag; vl = &n.As_val; }
-        else if (w == "Ap") { dg = &n.Ap_diag; vl = &n.Ap_val; upper = true; }
+        else if (w == "Ap") { dg = &n.Ap_diag; vl = &n.Ap_val; upper = !n.gas(); }
         else if (w == "SolidAs" && n.ht) { dg = &n.ht->diag; vl = &n.ht->val; hm = &n.ht->hm; dmo = &n.ht->dmo; }
         else { last_error() = "unknown matrix"; return -1; }
         const int C = hm->ncel, K = dmo->dm.K;
@@ -1122,7 +1154,11 @@ int cfd_ns_write_tecplot(cfd_ns *h, const char *path) {
         arr(get(n.R, C));
         arr(get(n.T, C));
         std::vector<double> t(C);
-        for (int i = 0; i < C; ++i) t[i] = std::sqrt(u[i] * u[i] + v[i] * v[i] + w[i] * w[i]);
+        auto pn = get(n.P_, C), rn = get(n.R, C);
+        for (int i = 0; i < C; ++i)  // Mach (perfect gas) or velocity magnitude
+            t[i] = n.gas() ? std::sqrt((u[i] * u[i] + v[i] * v[i] + w[i] * w[i]) /
+                                       (1.4 * (pn[i] + n.P.pressure_ref) / rn[i]))
+                           : std::sqrt(u[i] * u[i] + v[i] * v[i] + w[i] * w[i]);
         arr(t);
         auto ml = get(n.VisLam, C), mt = get(n.VisTur, C);
         for (int i = 0; i < C; ++i) t[i] = ml[i] + mt[i];

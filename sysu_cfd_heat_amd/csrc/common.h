@@ -125,6 +125,20 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 // the fin partials of each value in fixed order and returns true in thread 0
 // with the totals in v. Results are bit-reproducible for a fixed grid size.
 // part must hold NV * (producer grid) doubles.
+// This thread's share of the sum of cnt partials part[0..cnt) in the
+// finishing workgroup: four interleaved sums keep four loads in flight; they
+// are combined in a fixed order. block_sum of the shares gives the total.
+__device__ __forceinline__ double fin_share(const double *part, int cnt) {
+    double a4[4] = {0., 0., 0., 0.};
+    const int B = blockDim.x;
+    int b = threadIdx.x;
+    for (; b + 3 * B < cnt; b += 4 * B)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) a4[j] += part[b + j * B];
+    for (; b < cnt; b += B) a4[0] += part[b];
+    return (a4[0] + a4[1]) + (a4[2] + a4[3]);
+}
+
 template <int NV>
 __device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *part, int fin) {
     if (fin == 0) {
@@ -134,26 +148,9 @@ __device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *part, int f
             for (int k = 0; k < NV; ++k) part[k * gridDim.x + blockIdx.x] = v[k];
         return false;
     }
-    // four interleaved sums per thread keep four loads in flight; they are
-    // combined in a fixed order
-    double a4[4][NV];
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) a4[j][k] = 0.;
-    const int B = blockDim.x;
-    int b = threadIdx.x;
-    for (; b + 3 * B < fin; b += 4 * B)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int k = 0; k < NV; ++k) a4[j][k] += part[k * fin + b + j * B];
-    for (; b < fin; b += B)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) a4[0][k] += part[k * fin + b];
     double acc[NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = (a4[0][k] + a4[1][k]) + (a4[2][k] + a4[3][k]);
+    for (int k = 0; k < NV; ++k) acc[k] = fin_share(part + (size_t)k * fin, fin);
     block_sum<NV>(acc);
     if (threadIdx.x == 0)
 #pragma unroll
@@ -193,6 +190,10 @@ inline int spmv_grid(long long n) {
 // size of every reduction-partials buffer (doubles): NV * grid for the
 // largest grid and NV in use, and the per-colour partials of a sweep
 constexpr int kPartSize = 12 * 8192;
+// a CG iteration keeps the update's partials below this offset while the
+// preconditioner writes those of rho = r.z above it (one finishing launch
+// reads both)
+constexpr int kPartRho = 2 * kRedGrid;
 
 // Grid-stride loop of a reducing kernel in XCD-aware workgroup order.
 #define RED_LOOP_X(i, n)                                                               \

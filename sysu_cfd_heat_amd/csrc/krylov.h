@@ -146,9 +146,11 @@ class Krylov {
     void setup_pc(const EllT<T> &A, int pc, Colouring *col, KWork<T> &w);
     template <class T>
     // rz_blocks: ILU only; also write the workgroup partials of c.y to the
-    // head of the partials buffer and return their count
+    // partials buffer from offset kPartRho and return their count (-1 when
+    // they do not fit: the caller reduces c.y itself). st: the ILU sweeps do
+    // nothing once that solve has stopped.
     void apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, Colouring *col,
-                  KWork<T> &w, int *rz_blocks = nullptr);
+                  KWork<T> &w, int *rz_blocks = nullptr, const KState *st = nullptr);
     template <class T>
     SolveStats cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &cfg, Colouring *col,
                   int pc, KWork<T> &w, bool per_case = false);

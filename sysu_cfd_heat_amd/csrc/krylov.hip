@@ -227,6 +227,25 @@ __global__ void k_cg_update(int n, T *__restrict__ x, T *__restrict__ r, const T
     }
 }
 
+// The finishing launches of the update and of rho = r.z as one workgroup:
+// upart holds the update producer's gu partials of ||r||^2, rpart the nr
+// partials of r.z written by the preconditioner. Same sums, in the same order,
+// and the same state changes as the two separate finishing launches.
+__global__ void k_cg_fin(KState *st, const double *upart, int gu, const double *rpart, int nr) {
+    if (st->done) return;
+    double v[2] = {fin_share(upart, gu), fin_share(rpart, nr)};
+    block_sum<2>(v);
+    if (threadIdx.x != 0) return;
+    st->iter += 1;
+    st->rho_old = st->rho;
+    st->rr = v[0];
+    st->rnorm = sqrt(v[0]);
+    st->done = rtc(st->rnorm, st->bnorm, st->eps) || st->iter >= st->maxit;
+    if (st->done) return;
+    st->rho = v[1];
+    st->beta = st->iter == 0 ? 0. : v[1] / st->rho_old;
+}
+
 // rho = r.z ; beta = rho / rho_old (0 on the first iteration)
 template <class T>
 __global__ void k_cg_rho(int n, const T *__restrict__ r, const T *__restrict__ z, KState *st,
@@ -592,13 +611,16 @@ __global__ void k_dilu_factor(int n, int K, const int *__restrict__ col,
 // its backward step y = t / u_pp runs in its forward launch (fuse). With
 // rpart, every workgroup also writes its partial of c.y (CG's rho = r.z) to
 // rpart[blockIdx.x]; the launches of one application use consecutive slices.
+// With st, the sweeps do nothing once the solve has stopped (iterations
+// issued past convergence).
 // D (triangle-free graph): lu holds A's off-diagonals; the lower factor
 // entry is formed as a_pc / u_cc, as k_ilu_factor would have stored it.
 template <class T, bool D>
 __global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col, const T *__restrict__ lu,
                           const T *__restrict__ ud, int r0, int r1, int c0end,
                           const T *__restrict__ c, T *__restrict__ t, T *__restrict__ y, int fuse,
-                          double *rpart) {
+                          double *rpart, const KState *st) {
+    if (st && st->done) return;
     const int p = r0 + xcd_block() * blockDim.x + threadIdx.x;
     double rz[1] = {0.};
     if (p < r1) {
@@ -625,7 +647,8 @@ template <class T, bool D>
 __global__ void k_ilu_bwd(int n, int K, const int *__restrict__ col, const T *__restrict__ lu,
                           const T *__restrict__ ud, int r0, int r1, int c0end,
                           const T *__restrict__ c, const T *__restrict__ t, T *__restrict__ y,
-                          double *rpart) {
+                          double *rpart, const KState *st) {
+    if (st && st->done) return;
     const int p = r0 + xcd_block() * blockDim.x + threadIdx.x;
     double rz[1] = {0.};
     if (p < r1) {
@@ -979,8 +1002,8 @@ void Krylov::setup_pc(const EllT<T> &A, int pc, Colouring *col, KWork<T> &w) {
 // A is the permuted matrix for SSOR / ILU
 template <class T>
 void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, Colouring *col,
-                      KWork<T> &w, int *rz_blocks) {
-    if (rz_blocks) *rz_blocks = 0;
+                      KWork<T> &w, int *rz_blocks, const KState *st) {
+    if (rz_blocks) *rz_blocks = -1;
     const int n = A.n;
     if (pc == CFD_PC_NONE) {
         CFD_HIP(hipMemcpyAsync(y, c, sizeof(T) * n, hipMemcpyDeviceToDevice, st_));
@@ -1005,7 +1028,12 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
     } else {
         const int nc = col->ncolour;
         const int c0end = nc > 1 ? col->off[1] : n;
-        double *rp = rz_blocks ? part_.p : nullptr;
+        int nbk = 0;  // workgroups of the launches that write partials
+        for (int k = 0; k < nc; ++k) {
+            const int r0 = col->off[k], r1 = col->off[k + 1];
+            if (r1 > r0) nbk += grid8(r1 - r0);  // last forward + every backward colour
+        }
+        double *rp = (rz_blocks && nbk <= kPartSize - kPartRho) ? part_.p + kPartRho : nullptr;
         const bool dilu = col->triangle_free;  // the factor's off-diagonals are A's
         const T *lu = dilu ? (const T *)A.val : (const T *)w.lu.p;
         int used = 0;
@@ -1017,11 +1045,11 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
             if (dilu)
                 hipLaunchKernelGGL((k_ilu_fwd<T, true>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
                                    A.col, lu, (const T *)w.ud.p, r0, r1, c0end, c, w.tmp.p, y,
-                                   (int)last, (last && rp) ? rp + used : (double *)nullptr);
+                                   (int)last, (last && rp) ? rp + used : (double *)nullptr, st);
             else
                 hipLaunchKernelGGL((k_ilu_fwd<T, false>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
                                    A.col, lu, (const T *)w.ud.p, r0, r1, c0end, c, w.tmp.p, y,
-                                   (int)last, (last && rp) ? rp + used : (double *)nullptr);
+                                   (int)last, (last && rp) ? rp + used : (double *)nullptr, st);
             if (last && rp) used += g;
         }
         for (int k = nc - 2; k >= 0; --k) {
@@ -1031,14 +1059,14 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
             if (dilu)
                 hipLaunchKernelGGL((k_ilu_bwd<T, true>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
                                    A.col, lu, (const T *)w.ud.p, r0, r1, c0end, c,
-                                   (const T *)w.tmp.p, y, rp ? rp + used : (double *)nullptr);
+                                   (const T *)w.tmp.p, y, rp ? rp + used : (double *)nullptr, st);
             else
                 hipLaunchKernelGGL((k_ilu_bwd<T, false>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
                                    A.col, lu, (const T *)w.ud.p, r0, r1, c0end, c,
-                                   (const T *)w.tmp.p, y, rp ? rp + used : (double *)nullptr);
+                                   (const T *)w.tmp.p, y, rp ? rp + used : (double *)nullptr, st);
             if (rp) used += g;
         }
-        if (rz_blocks) *rz_blocks = used;
+        if (rz_blocks && rp) *rz_blocks = used;
     }
     CFD_LAUNCH_CHECK();
 }
@@ -1192,18 +1220,24 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
     const double dev_tol = per_case ? 0. : cfg.tol;  // batched: the host decides
     const int n = A.n, G = red_grid(n), Gs = spmv_grid(n);
     const bool jac = pc == CFD_PC_JACOBI;
-    // z = M^-1 r and rho = r.z (ILU folds the partials of r.z into its sweeps)
-    auto precond_rho = [&]() {
-        if (pc == CFD_PC_ILU) {
-            int nbk = 0;
-            apply_pc(A, pc, cfg.omega, (const T *)w.r.p, w.z.p, col, w, &nbk);
-            hipLaunchKernelGGL(k_cg_rho<T>, dim3(1), dim3(kBlock), 0, st_, n, (const T *)w.r.p,
-                               (const T *)w.z.p, state_.p, part_.p, nbk);
-        } else {
-            apply_pc(A, pc, cfg.omega, (const T *)w.r.p, w.z.p, col, w);
-            CFD_REDUCE_LAUNCH(k_cg_rho<T>, G, st_, n, (const T *)w.r.p, (const T *)w.z.p,
-                              state_.p, part_.p, fin);
+    // z = M^-1 r and the partials of rho = r.z above kPartRho (the ILU sweeps
+    // write them themselves); returns their count
+    double *rpart = part_.p + kPartRho;
+    auto precond_rho_partials = [&](const KState *stp) -> int {
+        int nbk = -1;
+        apply_pc(A, pc, cfg.omega, (const T *)w.r.p, w.z.p, col, w,
+                 pc == CFD_PC_ILU ? &nbk : nullptr, stp);
+        if (nbk < 0) {
+            nbk = G;
+            hipLaunchKernelGGL(k_cg_rho<T>, dim3(G), dim3(kBlock), 0, st_, n, (const T *)w.r.p,
+                               (const T *)w.z.p, state_.p, rpart, 0);
         }
+        return nbk;
+    };
+    auto precond_rho = [&]() {
+        const int nbk = precond_rho_partials(nullptr);
+        hipLaunchKernelGGL(k_cg_rho<T>, dim3(1), dim3(kBlock), 0, st_, n, (const T *)w.r.p,
+                           (const T *)w.z.p, state_.p, rpart, nbk);
     };
     CFD_REDUCE_LAUNCH(k_residual<T>, Gs, st_, A, b, (const T *)x, w.r.p, w.p0.p, w.q.p,
                       jac ? w.z.p : (T *)nullptr, state_.p, part_.p, fin, 1, dev_tol,
@@ -1223,10 +1257,13 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
                               (const T *)w.q.p, (const T *)A.diag, w.z.p, state_.p, part_.p,
                               fin);
         } else {
-            CFD_REDUCE_LAUNCH((k_cg_update<false, T>), G, st_, n, x, w.r.p, (const T *)w.p0.p,
-                              (const T *)w.q.p, (const T *)A.diag, (T *)nullptr, state_.p,
-                              part_.p, fin);
-            precond_rho();
+            // update producer, preconditioner, then one finishing launch for both
+            hipLaunchKernelGGL((k_cg_update<false, T>), dim3(G), dim3(kBlock), 0, st_, n, x,
+                               w.r.p, (const T *)w.p0.p, (const T *)w.q.p, (const T *)A.diag,
+                               (T *)nullptr, state_.p, part_.p, 0);
+            const int nbk = precond_rho_partials(state_.p);
+            hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(kBlock), 0, st_, state_.p,
+                               (const double *)part_.p, G, (const double *)rpart, nbk);
         }
     };
     // graphs of one and of kGraphIters iterations: a batch of b iterations is

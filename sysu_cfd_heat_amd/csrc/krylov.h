@@ -33,6 +33,9 @@ struct Colouring {
     std::vector<int> off;          // colour block offsets, size ncolour+1
     std::vector<int> colour_host;
     DBuf<int> perm, iperm, pcol;
+    // pcol lists each row's earlier-colour neighbours first: nlow[p] of them;
+    // slot k of permuted row p is slot smap[k][p] of the natural row
+    DBuf<unsigned char> smap, nlow;
     // host_col: the K x n slot-major column array; given_colour: optional
     void build(const int *host_col, int n, int K, const int *given_colour, const int *dev_col,
                hipStream_t s);

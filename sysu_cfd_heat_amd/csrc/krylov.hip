@@ -616,7 +616,8 @@ __global__ void k_dilu_factor(int n, int K, const int *__restrict__ col,
 // D (triangle-free graph): lu holds A's off-diagonals; the lower factor
 // entry is formed as a_pc / u_cc, as k_ilu_factor would have stored it.
 template <class T, bool D>
-__global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col, const T *__restrict__ lu,
+__global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col,
+                          const unsigned char *__restrict__ nlow, const T *__restrict__ lu,
                           const T *__restrict__ ud, int r0, int r1, int c0end,
                           const T *__restrict__ c, T *__restrict__ t, T *__restrict__ y, int fuse,
                           double *rpart, const KState *st) {
@@ -625,10 +626,11 @@ __global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col, const T *__
     double rz[1] = {0.};
     if (p < r1) {
         T acc = c[p];
-        for (int k = 0; k < K; ++k) {
-            size_t o = (size_t)k * n + p;
-            int cl = col[o];
-            if (cl < p) acc -= (D ? lu[o] / ud[cl] : lu[o]) * (cl < c0end ? c[cl] : t[cl]);
+        const int nl = nlow[p];
+        for (int k = 0; k < nl; ++k) {
+            const size_t o = (size_t)k * n + p;
+            const int cl = col[o];
+            acc -= (D ? lu[o] / ud[cl] : lu[o]) * (cl < c0end ? c[cl] : t[cl]);
         }
         if (fuse) {
             T yv = acc / ud[p];
@@ -644,7 +646,8 @@ __global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col, const T *__
     }
 }
 template <class T, bool D>
-__global__ void k_ilu_bwd(int n, int K, const int *__restrict__ col, const T *__restrict__ lu,
+__global__ void k_ilu_bwd(int n, int K, const int *__restrict__ col,
+                          const unsigned char *__restrict__ nlow, const T *__restrict__ lu,
                           const T *__restrict__ ud, int r0, int r1, int c0end,
                           const T *__restrict__ c, const T *__restrict__ t, T *__restrict__ y,
                           double *rpart, const KState *st) {
@@ -653,10 +656,10 @@ __global__ void k_ilu_bwd(int n, int K, const int *__restrict__ col, const T *__
     double rz[1] = {0.};
     if (p < r1) {
         T acc = p < c0end ? c[p] : t[p];
-        for (int k = 0; k < K; ++k) {
-            size_t o = (size_t)k * n + p;
-            int cl = col[o];
-            if (cl > p) acc -= lu[o] * y[cl];
+        for (int k = nlow[p]; k < K; ++k) {
+            const size_t o = (size_t)k * n + p;
+            const int cl = col[o];
+            if (cl > p) acc -= lu[o] * y[cl];  // padding slots hold the row itself
         }
         T yv = acc / ud[p];
         y[p] = yv;
@@ -668,14 +671,17 @@ __global__ void k_ilu_bwd(int n, int K, const int *__restrict__ col, const T *__
     }
 }
 
-// permuted copies: A'[p] = A[perm[p]] (columns via the cached pcol)
+// permuted copies: A'[p] = A[perm[p]], slot k of row p taken from slot
+// smap[k][p] of the source row (columns via the cached pcol)
 __global__ void k_perm_matrix(int n, int K, const int *__restrict__ perm,
+                              const unsigned char *__restrict__ smap,
                               const double *__restrict__ diag, const double *__restrict__ val,
                               double *__restrict__ pdiag, double *__restrict__ pval) {
     GRID_LOOP(p, n) {
         int o = perm[p];
         pdiag[p] = diag[o];
-        for (int k = 0; k < K; ++k) pval[(size_t)k * n + p] = val[(size_t)k * n + o];
+        for (int k = 0; k < K; ++k)
+            pval[(size_t)k * n + p] = val[(size_t)smap[(size_t)k * n + p] * n + o];
     }
 }
 __global__ void k_perm_gather(int n, const int *__restrict__ perm, const double *__restrict__ a,
@@ -753,15 +759,35 @@ void Colouring::build(const int *hc, int n_, int K_, const int *given, const int
         hp[p] = i;
         hip_[i] = p;
     }
+    // permuted columns with each row's earlier-colour (lower) entries first,
+    // in slot order, then the rest in slot order: a forward sweep reads the
+    // first nlow[p] slots only, a backward sweep the others
     std::vector<int> hpc((size_t)K * n);
-    for (int k = 0; k < K; ++k)
-        for (int p = 0; p < n; ++p) hpc[(size_t)k * n + p] = hip_[hc[(size_t)k * n + hp[p]]];
+    std::vector<unsigned char> hsm((size_t)K * n), hnl(n);
+    for (int p = 0; p < n; ++p) {
+        int j = 0;
+        for (int pass = 0; pass < 2; ++pass)
+            for (int k = 0; k < K; ++k) {
+                const int cl = hip_[hc[(size_t)k * n + hp[p]]];
+                if ((cl < p) != (pass == 0)) continue;
+                hpc[(size_t)j * n + p] = cl;
+                hsm[(size_t)j * n + p] = (unsigned char)k;
+                ++j;
+            }
+        int nl = 0;
+        while (nl < K && hpc[(size_t)nl * n + p] < p) ++nl;
+        hnl[p] = (unsigned char)nl;
+    }
     perm.alloc(n);
     iperm.alloc(n);
     pcol.alloc((size_t)K * n);
     perm.upload(hp.data(), n, s);
     iperm.upload(hip_.data(), n, s);
     pcol.upload(hpc.data(), (size_t)K * n, s);
+    smap.alloc((size_t)K * n);
+    nlow.alloc(n);
+    smap.upload(hsm.data(), (size_t)K * n, s);
+    nlow.upload(hnl.data(), n, s);
 }
 
 // --------------------------------------------------------------- Krylov --
@@ -1044,11 +1070,11 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
             const int g = grid8(r1 - r0);
             if (dilu)
                 hipLaunchKernelGGL((k_ilu_fwd<T, true>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
-                                   A.col, lu, (const T *)w.ud.p, r0, r1, c0end, c, w.tmp.p, y,
+                                   A.col, (const unsigned char *)col->nlow.p, lu, (const T *)w.ud.p, r0, r1, c0end, c, w.tmp.p, y,
                                    (int)last, (last && rp) ? rp + used : (double *)nullptr, st);
             else
                 hipLaunchKernelGGL((k_ilu_fwd<T, false>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
-                                   A.col, lu, (const T *)w.ud.p, r0, r1, c0end, c, w.tmp.p, y,
+                                   A.col, (const unsigned char *)col->nlow.p, lu, (const T *)w.ud.p, r0, r1, c0end, c, w.tmp.p, y,
                                    (int)last, (last && rp) ? rp + used : (double *)nullptr, st);
             if (last && rp) used += g;
         }
@@ -1058,11 +1084,11 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
             const int g = grid8(r1 - r0);
             if (dilu)
                 hipLaunchKernelGGL((k_ilu_bwd<T, true>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
-                                   A.col, lu, (const T *)w.ud.p, r0, r1, c0end, c,
+                                   A.col, (const unsigned char *)col->nlow.p, lu, (const T *)w.ud.p, r0, r1, c0end, c,
                                    (const T *)w.tmp.p, y, rp ? rp + used : (double *)nullptr, st);
             else
                 hipLaunchKernelGGL((k_ilu_bwd<T, false>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
-                                   A.col, lu, (const T *)w.ud.p, r0, r1, c0end, c,
+                                   A.col, (const unsigned char *)col->nlow.p, lu, (const T *)w.ud.p, r0, r1, c0end, c,
                                    (const T *)w.tmp.p, y, rp ? rp + used : (double *)nullptr, st);
             if (rp) used += g;
         }
@@ -1078,7 +1104,7 @@ void Krylov::precond(const Ell &A, int pc, double omega, const double *c, double
     if (pc == CFD_PC_SSOR || pc == CFD_PC_ILU) {
         if (!col) throw HipError("multicolour preconditioner needs a colouring");
         hipLaunchKernelGGL(k_perm_matrix, dim3(grid_for(n)), dim3(kBlock), 0, st_, n, A.K,
-                           col->perm.p, A.diag, A.val, wd_.pdiag.p, wd_.pval.p);
+                           col->perm.p, (const unsigned char *)col->smap.p, A.diag, A.val, wd_.pdiag.p, wd_.pval.p);
         Ell P{n, A.K, wd_.pdiag.p, wd_.pval.p, col->pcol.p};
         setup_pc(P, pc, col, wd_);
         hipLaunchKernelGGL(k_perm_gather, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
@@ -1105,7 +1131,7 @@ SolveStats Krylov::solve(const Ell &A0, const double *b0, double *x0, const cfd_
     if (perm) {
         if (!col || col->n != n) throw HipError("multicolour preconditioner needs a colouring");
         hipLaunchKernelGGL(k_perm_matrix, dim3(grid_for(n)), dim3(kBlock), 0, st_, n, A0.K,
-                           col->perm.p, A0.diag, A0.val, wd_.pdiag.p, wd_.pval.p);
+                           col->perm.p, (const unsigned char *)col->smap.p, A0.diag, A0.val, wd_.pdiag.p, wd_.pval.p);
         hipLaunchKernelGGL(k_perm_gather, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
                            col->perm.p, b0, wd_.pb.p);
         hipLaunchKernelGGL(k_perm_gather, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
@@ -1149,7 +1175,7 @@ SolveStats Krylov::solve_mixed(const Ell &A0, const double *b0, double *x0,
     if (perm) {
         if (!col || col->n != n) throw HipError("multicolour preconditioner needs a colouring");
         hipLaunchKernelGGL(k_perm_matrix, dim3(grid_for(n)), dim3(kBlock), 0, st_, n, A0.K,
-                           col->perm.p, A0.diag, A0.val, wd_.pdiag.p, wd_.pval.p);
+                           col->perm.p, (const unsigned char *)col->smap.p, A0.diag, A0.val, wd_.pdiag.p, wd_.pval.p);
         hipLaunchKernelGGL(k_perm_gather, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
                            col->perm.p, b0, wd_.pb.p);
         hipLaunchKernelGGL(k_perm_gather, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,

@@ -249,6 +249,11 @@ int cfd_ns_profile_read(const cfd_ns *h, double out[8]);
 /* Run-time options (no counterpart in the reference):
  *   "graphs" 0/1: replay CG iterations from captured HIP graphs (default 1;
  *   results are bit-identical either way)
+ *   "overlap_solid" 0/1: run the solid conduction solve of an outer
+ *   iteration on its own stream beside the next iteration's momentum,
+ *   pressure and k-epsilon steps (default 1; results are identical, a solid
+ *   failure is reported by the next call; every call that reads or changes
+ *   the solver waits for it first, "Synchronize" waits for all device work)
  * cfd_ns_get_option also reads "graph_launches" (CG iterations replayed). */
 int cfd_ns_set_option(cfd_ns *h, const char *name, double value);
 double cfd_ns_get_option(const cfd_ns *h, const char *name);

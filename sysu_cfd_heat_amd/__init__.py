@@ -407,6 +407,11 @@ class NSSolver:
         self._L.cfd_ns_profile_read(self.h, _dp(out))
         return out
 
+    def sync(self) -> None:
+        """Wait for all device work of the solver (including a solid solve
+        still running beside the fluid steps)."""
+        self.call("Synchronize")
+
     def set_option(self, name: str, value: float) -> None:
         if self._L.cfd_ns_set_option(self.h, name.encode(), float(value)):
             raise RuntimeError(self._error())

@@ -2,9 +2,11 @@
 // NavierStokesSolver / HeatConductionSolver method names and launch the HIP
 // kernels of assembly.hip / krylov.hip.
 #pragma once
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "assembly.h"
@@ -48,6 +50,17 @@ class HTSolver {
     DBuf<double> Tnp, ApE;  // transient runs
     SolveStats last;
     long long iters_total = 0;
+    // the solid has its own stream and Krylov state, so that its solve can
+    // run beside the fluid's next momentum and pressure steps (NSSolver::
+    // run_solid); every solid launch goes to hs
+    hipStream_t hs = nullptr;
+    std::unique_ptr<Krylov> kry;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    std::thread worker;
+    bool running = false;
+    int worker_rc = 0;
+    std::string worker_err;
+    ~HTSolver();
     void allocate(hipStream_t s);
     void InitFlowField();
     void SetBCTemperature();
@@ -84,11 +97,22 @@ class NSSolver {
     // and the energy equation's unsteady diagonal / source
     DBuf<double> Unp, Vnp, Wnp, Tnp, TEp, EDp, Unp2, Vnp2, Wnp2, Tnp2, TEp2, EDp2, EnS, EnAp;
     double cur_time = 0.;
+    // The solid solve of an outer iteration needs only the fluid's energy
+    // step and is needed only by the next one: with overlap_solid it runs on
+    // the solid's stream from a worker thread while the next iteration's
+    // momentum, pressure and k-epsilon steps run. Results are identical; a
+    // solid failure is reported by the next call. run_solid: f (which
+    // launches on ht->hs) after the fluid stream's work, waited for by the
+    // fluid stream; async: returns at once. solid_join: waits for a running
+    // solve and returns its status.
+    bool overlap_solid = true;
+    int run_solid(const std::function<int()> &f, bool async);
+    int solid_join();
     // perfect gas (DensityModel 1): rho = p / (Rcpcv T), Rcpcv = cp - cp / gama
     // with gama = 1.4 (src/navier.cpp:376-382)
     bool gas() const { return P.density_model == 1; }
     double rcpcv() const { return P.cp - P.cp / 1.4; }
-    void SaveTransientOldData();
+    int SaveTransientOldData();
     DBuf<BcScal> bcs;
     DBuf<int> errflag;
     double Residual[10] = {0};
@@ -128,7 +152,7 @@ class NSSolver {
     SolveStats solve(const Ell &A, const double *b, double *x, const cfd_solve_cfg &c,
                      Colouring *col, int precision);
 
-    void InitFlowField();
+    int InitFlowField();
     void SetBCVelocity();
     void SetBCPressure();
     void SetBCDeltaP();

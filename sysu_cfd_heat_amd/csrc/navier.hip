@@ -139,8 +139,51 @@ static void alloc_n(std::initializer_list<DBuf<double> *> bufs, size_t n) {
 
 // ------------------------------------------------------------- solver --
 NSSolver::~NSSolver() {
+    if (ht && ht->worker.joinable()) ht->worker.join();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+}
+
+int NSSolver::solid_join() {
+    if (!ht || !ht->running) return 0;
+    ht->worker.join();
+    ht->running = false;
+    CFD_HIP(hipEventRecord(ht->ev_out, ht->hs));
+    CFD_HIP(hipStreamWaitEvent(s, ht->ev_out, 0));
+    if (ht->worker_rc) {
+        last_error() = ht->worker_err;
+        return -1;
+    }
+    return 0;
+}
+
+int NSSolver::run_solid(const std::function<int()> &f, bool async) {
+    if (solid_join()) return -1;
+    CFD_HIP(hipEventRecord(ht->ev_in, s));
+    CFD_HIP(hipStreamWaitEvent(ht->hs, ht->ev_in, 0));
+    int dev = 0;
+    CFD_HIP(hipGetDevice(&dev));
+    auto body = [this, f, dev]() {
+        int rc;
+        try {
+            CFD_HIP(hipSetDevice(dev));  // a new host thread starts on device 0
+            rc = f();
+        } catch (const std::exception &e) {
+            last_error() = e.what();
+            rc = -1;
+        }
+        ht->worker_rc = rc;
+        ht->worker_err = rc ? last_error() : std::string();
+    };
+    if (async) {
+        ht->running = true;
+        ht->worker = std::thread(body);
+        return 0;
+    }
+    body();
+    CFD_HIP(hipEventRecord(ht->ev_out, ht->hs));
+    CFD_HIP(hipStreamWaitEvent(s, ht->ev_out, 0));
+    return ht->worker_rc;
 }
 
 int NSSolver::setup(const cfd_raw_mesh &raw, const cfd_params &p, std::string &err) {
@@ -331,13 +374,16 @@ int NSSolver::finish_setup(std::string &err) {
         ht->dmo.build(ht->hm, reg, ht->breg, s);
         ht->allocate(s);
     }
-    InitFlowField();
+    if (InitFlowField()) {
+        err = last_error();
+        return -1;
+    }
     CFD_HIP(hipStreamSynchronize(s));
     return 0;
 }
 
 // InitFlowField (src/navier.cpp:113-188), regions only (no initflow line)
-void NSSolver::InitFlowField() {
+int NSSolver::InitFlowField() {
     const int C = hm.ncel, B = hm.nbnd;
     std::vector<double> u(C), v(C), w(C), r(C), t(C), pz(C, 0.), ml(C), mt(C, 0.), k(C, 0.),
         e(C, 0.);
@@ -369,10 +415,11 @@ void NSSolver::InitFlowField() {
     BT.upload(bt.data(), B, s); BndQ.upload(q.data(), B, s);
     RUFace.zero(s);
     CFD_HIP(hipStreamSynchronize(s));
-    if (ht) ht->InitFlowField();
+    if (ht && run_solid([this] { ht->InitFlowField(); return 0; }, false)) return -1;
     // the history starts as the initial field (NSSolve saves it again at
     // the start of every time step)
-    if (!P.steady) SaveTransientOldData();
+    if (!P.steady) return SaveTransientOldData();
+    return 0;
 }
 
 Colouring *NSSolver::colouring_for(const cfd_solve_cfg &c) {
@@ -580,7 +627,10 @@ int NSSolver::UpdateEnergy() {
     launch_kcond(dm(), VisLam, VisTur, P.cp, P.prl, P.prte, kcond, s);
     if (gas()) launch_energy_visc(dm(), gU, gV, gW, U, V, W, VisLam, VisTur, P.cp, EnS, s);
     SetBCTemperature();
-    if (ht) launch_solid2fluid(ht->dmo.dm, ht->T, ht->g, hm.ncb, BT, s);
+    if (ht) {
+        if (solid_join()) return -1;  // the solid solve of the previous iteration
+        launch_solid2fluid(ht->dmo.dm, ht->T, ht->g, hm.ncb, BT, s);
+    }
     if (!P.steady || gas()) {
         if (!gas()) launch_fill(EnS, hm.ncel, 0., s);
         launch_fill(EnAp, hm.ncel, 0., s);
@@ -621,7 +671,7 @@ int NSSolver::Iterate(double &resmax) {
     if (P.turb_model == 1 && UpdateTurKEpsilon()) return -1;
     if (P.solve_energy) {
         if (UpdateEnergy()) return -1;
-        if (ht && ht->UpdateEnergy()) return -1;
+        if (ht && run_solid([this] { return ht->UpdateEnergy(); }, overlap_solid)) return -1;
     }
     CFD_HIP(hipEventRecord(ev1, s));
     res_dev.download(Residual, 10, s);
@@ -651,7 +701,7 @@ int NSSolver::NSSolve(int &iters) {
     cur_time = 0.;
     cur_time += dt;
     for (int step = 1; step <= P.max_step; ++step) {
-        if (!P.steady) SaveTransientOldData();
+        if (!P.steady && SaveTransientOldData()) return -1;
         double r0 = 0.;
         for (int it = 1; it < max_outer; ++it) {
             double r;
@@ -667,13 +717,13 @@ int NSSolver::NSSolve(int &iters) {
         if (cur_time >= P.total_time) break;
         cur_time += dt;
     }
-    return 0;
+    return solid_join();
 }
 
 // SaveTransientOldData (src/navier.cpp:190-241): for the 2nd-order scheme
 // both history levels take the current value, as the reference's
 // Unp2[i] = Unp[i] = Un[i] does
-void NSSolver::SaveTransientOldData() {
+int NSSolver::SaveTransientOldData() {
     const size_t bytes = sizeof(double) * hm.ncel;
     const DBuf<double> *src[6] = {&U, &V, &W, &T, &TE, &ED};
     DBuf<double> *l1[6] = {&Unp, &Vnp, &Wnp, &Tnp, &TEp, &EDp};
@@ -683,13 +733,14 @@ void NSSolver::SaveTransientOldData() {
         if (P.time_scheme == 2)
             CFD_HIP(hipMemcpyAsync(l2[k]->p, src[k]->p, bytes, hipMemcpyDeviceToDevice, s));
     }
-    if (ht) ht->SaveTransientOldData();
+    if (ht) return run_solid([this] { ht->SaveTransientOldData(); return 0; }, false);
+    return 0;
 }
 
 // HeatConductionSolver::SaveTransientOldData (src/HeatConductionSolver.cpp:
 // 306-326), TimeScheme 1
 void HTSolver::SaveTransientOldData() {
-    CFD_HIP(hipMemcpyAsync(Tnp.p, T.p, sizeof(double) * hm.ncel, hipMemcpyDeviceToDevice, ns->s));
+    CFD_HIP(hipMemcpyAsync(Tnp.p, T.p, sizeof(double) * hm.ncel, hipMemcpyDeviceToDevice, hs));
 }
 
 // ------------------------------------------------------------ solid --
@@ -702,6 +753,19 @@ void HTSolver::allocate(hipStream_t s) {
     val.alloc(std::max<size_t>(K * C, 1));
     gtmp.alloc(std::max<size_t>(3 * C, 2 * (size_t)hm.nvrt) + 1);
     for (auto *b : {&T, &R, &D, &xsol, &bs, &src, &diag, &g, &BT, &BndQ, &val}) b->zero(s);
+    CFD_HIP(hipStreamCreateWithFlags(&hs, hipStreamNonBlocking));
+    CFD_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
+    CFD_HIP(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
+    kry.reset(new Krylov(hs));
+}
+
+HTSolver::~HTSolver() {
+    if (worker.joinable()) worker.join();
+    kry.reset();
+    if (hs) (void)hipStreamSynchronize(hs);
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
+    if (hs) (void)hipStreamDestroy(hs);
 }
 
 // HeatConductionSolver::InitFlowField (src/HeatConductionSolver.cpp:269-304)
@@ -720,33 +784,33 @@ void HTSolver::InitFlowField() {
             else if (n.reg.type2[sl] == 1) q[b] = n.reg.fixed[sl];
         }
     }
-    T.upload(t.data(), C, n.s); R.upload(r.data(), C, n.s); D.upload(d.data(), C, n.s);
-    BT.upload(bt.data(), B, n.s); BndQ.upload(q.data(), B, n.s);
+    T.upload(t.data(), C, hs); R.upload(r.data(), C, hs); D.upload(d.data(), C, hs);
+    BT.upload(bt.data(), B, hs); BndQ.upload(q.data(), B, hs);
     SetBCTemperature();
     const double *phi[1] = {T}, *bphi[1] = {BT};
     double *out[1] = {g};
-    launch_gradient(dmo.dm, 1, phi, bphi, out, n.P.limiter, gtmp, n.s);
-    CFD_HIP(hipStreamSynchronize(n.s));
+    launch_gradient(dmo.dm, 1, phi, bphi, out, n.P.limiter, gtmp, hs);
+    CFD_HIP(hipStreamSynchronize(hs));
 }
 
-void HTSolver::SetBCTemperature() { launch_solid_bc_temperature(dmo.dm, T, D, BndQ, BT, ns->s); }
+void HTSolver::SetBCTemperature() { launch_solid_bc_temperature(dmo.dm, T, D, BndQ, BT, hs); }
 
 // HeatConductionSolver::BuildScalarMatrix (src/HeatConductionSolver.cpp:110-223)
 void HTSolver::BuildScalarMatrix() {
     const double *phi[1] = {T}, *bphi[1] = {BT};
     double *out[1] = {g};
-    launch_gradient(dmo.dm, 1, phi, bphi, out, ns->P.limiter, gtmp, ns->s);
+    launch_gradient(dmo.dm, 1, phi, bphi, out, ns->P.limiter, gtmp, hs);
     ScaArgs a;
     a.Phi = T; a.BPhi = BT; a.D = D; a.src = src; a.App = ns->P.steady ? nullptr : ApE.p;
     a.RUFace = nullptr; a.g = g;
     a.BndQ = BndQ; a.urf = URF; a.mode = 2;
     a.diag = diag; a.val = val; a.bs = bs;
-    launch_build_scalar(dmo.dm, a, ns->errflag, ns->s);
+    launch_build_scalar(dmo.dm, a, ns->errflag, hs);
 }
 
 // HeatConductionSolver::UpdateEnergy (src/HeatConductionSolver.cpp:47-106)
 int HTSolver::UpdateEnergy() {
-    hipStream_t s = ns->s;
+    hipStream_t s = hs;
     launch_scale_source(dmo.dm, ns->P.solid_heat_source, src, s);
     SetBCTemperature();
     if (!ns->P.steady) {
@@ -768,7 +832,7 @@ int HTSolver::UpdateEnergy() {
         col = &colour;
     }
     Ell A{hm.ncel, dmo.dm.K, diag.p, val.p, dmo.dm.ell_col};
-    last = ns->kry->solve(A, bs, xsol, ns->P.solid, col);
+    last = kry->solve(A, bs, xsol, ns->P.solid, col);
     ns->stats[6] += last.iters;
     if (last.iters >= ns->P.solid.maxit && last.relres > ns->P.solid.tol) {
         last_error() = "solid energy cannot converge";
@@ -795,6 +859,10 @@ int guarded(F f) {
         return -1;
     }
 }
+
+// entry points that read or change solver state first wait for a solid
+// solve still running beside the fluid steps
+int join_solid(const cfd_ns *h) { return const_cast<cfd_ns *>(h)->ns.solid_join(); }
 
 struct FieldRef {
     DBuf<double> *buf = nullptr;
@@ -915,6 +983,7 @@ int cfd_ns_call(cfd_ns *h, const char *method) {
     return guarded([&]() -> int {
         NSSolver &n = h->ns;
         std::string m(method);
+        if (m != "Iterate" && join_solid(h)) return -1;
         int rc = 0;
         if (m == "NSSolve") { int it; rc = n.NSSolve(it); }
         else if (m == "Iterate") { double r; rc = n.Iterate(r); }
@@ -930,9 +999,14 @@ int cfd_ns_call(cfd_ns *h, const char *method) {
         else if (m == "SetBCTemperature") n.SetBCTemperature();
         else if (m == "UpdateTurKEpsilon") rc = n.UpdateTurKEpsilon();
         else if (m == "UpdateEnergy") rc = n.UpdateEnergy();
-        else if (m == "SolidSolve") rc = n.ht ? n.ht->UpdateEnergy() : -1;
-        else if (m == "InitFlowField") n.InitFlowField();
-        else if (m == "SaveTransientOldData") n.SaveTransientOldData();
+        else if (m == "SolidSolve")
+            rc = n.ht ? n.run_solid([&n] { return n.ht->UpdateEnergy(); }, false) : -1;
+        else if (m == "Synchronize") {
+            rc = n.solid_join();
+            CFD_HIP(hipStreamSynchronize(n.s));
+        }
+        else if (m == "InitFlowField") rc = n.InitFlowField();
+        else if (m == "SaveTransientOldData") rc = n.SaveTransientOldData();
         else if (m == "EnergyViscSource") {
             if (n.EnS.n < (size_t)n.hm.ncel) {
                 last_error() = "EnergyViscSource: perfect-gas or transient runs only";
@@ -968,6 +1042,7 @@ int cfd_ns_call(cfd_ns *h, const char *method) {
 
 int cfd_ns_field(cfd_ns *h, int part, const char *name, double *host, size_t count, int dir) {
     return guarded([&]() -> int {
+        if (join_solid(h)) return -1;
         NSSolver &n = h->ns;
         std::string s(name);
         if (s == "Residual" && part == 0) {
@@ -1004,6 +1079,7 @@ int cfd_ns_field(cfd_ns *h, int part, const char *name, double *host, size_t cou
 
 int cfd_ns_matrix(cfd_ns *h, const char *which, int *row_len, int *col, double *val) {
     return guarded([&]() -> int {
+        if (join_solid(h)) return -1;
         NSSolver &n = h->ns;
         std::string w(which);
         const HostMesh *hm = &n.hm;
@@ -1036,6 +1112,7 @@ int cfd_ns_matrix(cfd_ns *h, const char *which, int *row_len, int *col, double *
 }
 
 int cfd_ns_last_solve(const cfd_ns *h, const char *eq, int *iters, double *relres) {
+    if (guarded([&] { return join_solid(h); })) return -1;
     const NSSolver &n = h->ns;
     std::string e(eq);
     const SolveStats *s = e == "u" ? &n.last_u : e == "v" ? &n.last_v : e == "w" ? &n.last_w
@@ -1074,9 +1151,15 @@ int cfd_ns_profile_read(const cfd_ns *h, double out[8]) {
 }
 
 int cfd_ns_set_option(cfd_ns *h, const char *name, double value) {
+    if (guarded([&] { return join_solid(h); })) return -1;
     std::string k(name);
     if (k == "graphs") {
         h->ns.kry->use_graphs = value != 0.;
+        if (h->ns.ht) h->ns.ht->kry->use_graphs = value != 0.;
+        return 0;
+    }
+    if (k == "overlap_solid") {
+        h->ns.overlap_solid = value != 0.;
         return 0;
     }
     last_error() = "unknown option " + k;
@@ -1084,8 +1167,10 @@ int cfd_ns_set_option(cfd_ns *h, const char *name, double value) {
 }
 
 double cfd_ns_get_option(const cfd_ns *h, const char *name) {
+    if (guarded([&] { return join_solid(h); })) return std::nan("");
     std::string k(name);
     if (k == "graphs") return h->ns.kry->use_graphs ? 1. : 0.;
+    if (k == "overlap_solid") return h->ns.overlap_solid ? 1. : 0.;
     if (k == "graph_launches") return (double)h->ns.kry->graph_launches;
     last_error() = "unknown option " + k;
     return std::nan("");
@@ -1093,6 +1178,7 @@ double cfd_ns_get_option(const cfd_ns *h, const char *name) {
 
 int cfd_ns_write_tecplot(cfd_ns *h, const char *path) {
     return guarded([&]() -> int {
+        if (join_solid(h)) return -1;
         NSSolver &n = h->ns;
         std::ofstream of(path);
         if (!of) {
@@ -1181,10 +1267,12 @@ int cfd_ns_write_tecplot(cfd_ns *h, const char *path) {
 }
 
 int cfd_ns_stats(const cfd_ns *h, double out[8]) {
+    if (guarded([&] { return join_solid(h); })) return -1;
     std::memcpy(out, h->ns.stats, sizeof(h->ns.stats));
     return 0;
 }
 int cfd_ns_reset_stats(cfd_ns *h) {
+    if (guarded([&] { return join_solid(h); })) return -1;
     std::memset(h->ns.stats, 0, sizeof(h->ns.stats));
     return 0;
 }

@@ -84,3 +84,41 @@ def test_graph_replay_is_bit_identical(case):
     assert np.array_equal(sa[2:7], sb[2:7])  # Krylov iteration counts
     for nm in a:
         assert np.array_equal(a[nm], b[nm]), nm
+
+
+@pytest.mark.parametrize("steady", [True, False])
+def test_solid_overlap_is_bit_identical(steady):
+    """The solid solve run beside the next iteration's fluid steps (its own
+    stream and host thread) gives exactly the results of the sequential
+    order, iteration after iteration, with reads between the iterations."""
+    import sysu_cfd_heat_amd as P
+    from sysu_cfd_heat_amd import cases
+    print(f"[forms] solid overlap steady={steady}", flush=True)
+    raw, p = cases.channel(nx=12, ny=6, nz=6, max_step=1000)
+    if not steady:
+        p.steady, p.dt, p.time_scheme, p.max_outer_step, p.total_time = 0, 0.02, 2, 3, 0.06
+    out = []
+    for overlap in (0, 1):
+        s = P.NSSolver(raw, p)
+        s.set_option("overlap_solid", overlap)
+        assert s.get_option("overlap_solid") == overlap
+        hist = []
+        if steady:
+            for k in range(4):
+                s.call("Iterate")
+                if k == 1:  # a read in between waits for the running solid solve
+                    hist.append(s.get("Tn", part=1).copy())
+        else:
+            s.call("NSSolve")
+        s.sync()
+        f = {nm: s.get(nm) for nm in ("Un", "Pn", "Tn", "TE")}
+        f["solid"] = s.get("Tn", part=1)
+        out.append((f, hist, s.stats().copy(), s.last_solve("solid")))
+        s.close()
+    (a, ha, sa, la), (b, hb, sb, lb) = out
+    for nm in a:
+        assert np.array_equal(a[nm], b[nm]), nm
+    for x, y in zip(ha, hb):
+        assert np.array_equal(x, y)
+    assert np.array_equal(sa[[0, 2, 3, 4, 5, 6]], sb[[0, 2, 3, 4, 5, 6]])
+    assert la == lb and sa[6] > 0

@@ -46,6 +46,11 @@ def parse():
                     help="channel: the headline CHT channel (default); cavity: the 32^3 "
                          "Re=100 lid-driven cavity; batch: 64 uneven cavity/channel cases "
                          "packed per launch (extra lines, not the headline)")
+    ap.add_argument("--pres-precision", type=int, default=0,
+                    help="pressure solve: 0 FP64 CG (the headline), 1 mixed precision (FP32 "
+                         "inner CG, FP64 residual and update)")
+    ap.add_argument("--mixed-inner-tol", type=float, default=1e-5,
+                    help="mixed-precision pressure: reduction of each FP32 inner CG")
     ap.add_argument("--graphs", type=int, default=1,
                     help="replay pressure CG iterations from a captured HIP graph (cavity and "
                          "batch workloads; the channel run times its SpMV launches with events, "
@@ -230,7 +235,9 @@ def main():
     t0 = time.time()
     raw, p = cases.channel(nx=a.nx, ny=a.ny, nz=a.nz, tets=True, jitter=0.1, seed=1,
                            max_step=a.warmup + a.steps + 2)
+    p.pres_precision = a.pres_precision
     gpu = NSSolver(raw, p)
+    gpu.set_option("mixed_inner_tol", a.mixed_inner_tol)
     setup_s = time.time() - t0
     print(f"[bench] setup {setup_s:.1f}s cells={raw.counts[1]} fluid={gpu.counts(0)[1]} "
           f"solid={gpu.counts(1)[1]}", file=sys.stderr, flush=True)
@@ -298,7 +305,9 @@ def main():
         "metric": "SIMPLE iters/s at 3.1M tets + SpMV HBM GB/s; p-solve iters learned vs plain",
         "value": value, "unit": "SIMPLE iters/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if a.pres_precision == 0 else "f64 (pressure: f32 inner, f64 refinement)",
+        "data": "synthetic",
         "config": {"workload": f"CHT channel {a.nx}x{a.ny}x{a.nz} hex x6 jittered tets "
                                f"({raw.counts[1]} cells), k-eps Re=1e4, energy + solid",
                    "cells": raw.counts[1], "solvers": "BiCGSTAB+Jacobi / CG+ILU(0), 1e-8",

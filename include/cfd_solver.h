@@ -254,6 +254,8 @@ int cfd_ns_profile_read(const cfd_ns *h, double out[8]);
  *   pressure and k-epsilon steps (default 1; results are identical, a solid
  *   failure is reported by the next call; every call that reads or changes
  *   the solver waits for it first, "Synchronize" waits for all device work)
+ *   "mixed_inner_tol" in (0, 1): relative reduction of each FP32 inner CG of
+ *   the mixed-precision pressure solve (default 1e-5)
  * cfd_ns_get_option also reads "graph_launches" (CG iterations replayed). */
 int cfd_ns_set_option(cfd_ns *h, const char *name, double value);
 double cfd_ns_get_option(const cfd_ns *h, const char *name);

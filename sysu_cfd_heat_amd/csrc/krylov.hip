@@ -866,7 +866,7 @@ int Krylov::next_batch_log(int batch, int &prev_it, double &prev_log, int maxit,
                            double lt) {
     const KState &h = *hstate_;
     int nb = batch * 2;
-    if (h.iter > prev_it && prev_log > lr) {
+    if (h.iter > prev_it && prev_log < 1e299 && prev_log > lr) {  // 1e300: no rate yet
         double rate = (prev_log - lr) / (h.iter - prev_it);
         double need = (lr - lt) / rate;
         nb = need < 1. ? 1 : (int)std::ceil(need);

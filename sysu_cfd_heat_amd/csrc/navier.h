@@ -106,6 +106,9 @@ class NSSolver {
     // fluid stream; async: returns at once. solid_join: waits for a running
     // solve and returns its status.
     bool overlap_solid = true;
+    // mixed-precision pressure solves: the reduction each FP32 inner CG
+    // makes before the FP64 residual is recomputed
+    double mixed_inner_tol = 1e-5;
     int run_solid(const std::function<int()> &f, bool async);
     int solid_join();
     // perfect gas (DensityModel 1): rho = p / (Rcpcv T), Rcpcv = cp - cp / gama

@@ -433,8 +433,8 @@ Colouring *NSSolver::colouring_for(const cfd_solve_cfg &c) {
 
 SolveStats NSSolver::solve(const Ell &A, const double *b, double *x, const cfd_solve_cfg &c,
                            Colouring *col, int precision) {
-    // FP32 inner CG to 1e-4 per refinement, FP64 residual and update
-    if (precision == 1 && c.method == CFD_CG) return kry->solve_mixed(A, b, x, c, col, 1e-4);
+    // FP32 inner CG to mixed_inner_tol per refinement, FP64 residual and update
+    if (precision == 1 && c.method == CFD_CG) return kry->solve_mixed(A, b, x, c, col, mixed_inner_tol);
     return kry->solve(A, b, x, c, col);
 }
 
@@ -1162,6 +1162,14 @@ int cfd_ns_set_option(cfd_ns *h, const char *name, double value) {
         h->ns.overlap_solid = value != 0.;
         return 0;
     }
+    if (k == "mixed_inner_tol") {
+        if (!(value > 0. && value < 1.)) {
+            last_error() = "mixed_inner_tol must lie in (0, 1)";
+            return -1;
+        }
+        h->ns.mixed_inner_tol = value;
+        return 0;
+    }
     last_error() = "unknown option " + k;
     return -1;
 }
@@ -1171,6 +1179,7 @@ double cfd_ns_get_option(const cfd_ns *h, const char *name) {
     std::string k(name);
     if (k == "graphs") return h->ns.kry->use_graphs ? 1. : 0.;
     if (k == "overlap_solid") return h->ns.overlap_solid ? 1. : 0.;
+    if (k == "mixed_inner_tol") return h->ns.mixed_inner_tol;
     if (k == "graph_launches") return (double)h->ns.kry->graph_launches;
     last_error() = "unknown option " + k;
     return std::nan("");

@@ -122,11 +122,16 @@ def learned_vs_plain(a):
         p.pres_precision = prec
         s = P.NSSolver(raw, p)
         s.set_guess(mode, net if mode == 2 else None)
-        worst = 0.0
+        worst, secs = 0.0, 0.0
         for _ in range(iters):
+            s.sync()
+            t0 = time.perf_counter()
             s.call("Iterate")
+            s.sync()
+            secs += time.perf_counter() - t0
             worst = max(worst, s.last_solve("p")[1])
-        res[name] = {"p_iters": int(s.stats()[3]), "worst_true_relres": worst}
+        res[name] = {"p_iters": int(s.stats()[3]), "worst_true_relres": worst,
+                     "ms_per_step": 1000.0 * secs / iters}
         s.close()
         print(f"[bench] {name}: {res[name]}", file=sys.stderr, flush=True)
     plain = res["plain_fp64"]["p_iters"]
@@ -137,7 +142,9 @@ def learned_vs_plain(a):
                      f"loss {first:.3f} -> {last:.3f}",
             **res,
             "plain_over_learned_mixed": plain / max(res["learned_mixed"]["p_iters"], 1),
-            "plain_over_learned_fp64": plain / max(res["learned_fp64"]["p_iters"], 1)}
+            "plain_over_learned_fp64": plain / max(res["learned_fp64"]["p_iters"], 1),
+            "time_plain_over_learned_mixed": res["plain_fp64"]["ms_per_step"]
+            / max(res["learned_mixed"]["ms_per_step"], 1e-9)}
 
 
 def _line(metric, value, unit, a, ms, workload, extra):

@@ -615,7 +615,11 @@ __global__ void k_dilu_factor(int n, int K, const int *__restrict__ col,
 // issued past convergence).
 // D (triangle-free graph): lu holds A's off-diagonals; the lower factor
 // entry is formed as a_pc / u_cc, as k_ilu_factor would have stored it.
-template <class T, bool D>
+// KK > 0: the slot count as a constant (4 for tetrahedra, 6 for hexahedra),
+// so that the column loads of every slot, then the gathers, are issued
+// together instead of one slot's chain after the other; KK = 0: any K.
+// The products are summed in slot order either way.
+template <class T, bool D, int KK>
 __global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col,
                           const unsigned char *__restrict__ nlow, const T *__restrict__ lu,
                           const T *__restrict__ ud, int r0, int r1, int c0end,
@@ -627,10 +631,28 @@ __global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col,
     if (p < r1) {
         T acc = c[p];
         const int nl = nlow[p];
-        for (int k = 0; k < nl; ++k) {
-            const size_t o = (size_t)k * n + p;
-            const int cl = col[o];
-            acc -= (D ? lu[o] / ud[cl] : lu[o]) * (cl < c0end ? c[cl] : t[cl]);
+        if (KK > 0) {
+            int cl[KK > 0 ? KK : 1];
+            T a[KK > 0 ? KK : 1], v[KK > 0 ? KK : 1];
+#pragma unroll
+            for (int k = 0; k < KK; ++k) cl[k] = k < nl ? col[(size_t)k * n + p] : 0;
+#pragma unroll
+            for (int k = 0; k < KK; ++k) {
+                if (k < nl) {
+                    const T l = lu[(size_t)k * n + p];
+                    a[k] = D ? l / ud[cl[k]] : l;
+                    v[k] = cl[k] < c0end ? c[cl[k]] : t[cl[k]];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < KK; ++k)
+                if (k < nl) acc -= a[k] * v[k];
+        } else {
+            for (int k = 0; k < nl; ++k) {
+                const size_t o = (size_t)k * n + p;
+                const int cl = col[o];
+                acc -= (D ? lu[o] / ud[cl] : lu[o]) * (cl < c0end ? c[cl] : t[cl]);
+            }
         }
         if (fuse) {
             T yv = acc / ud[p];
@@ -645,7 +667,7 @@ __global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col,
         if (threadIdx.x == 0) rpart[blockIdx.x] = rz[0];
     }
 }
-template <class T, bool D>
+template <class T, bool D, int KK>
 __global__ void k_ilu_bwd(int n, int K, const int *__restrict__ col,
                           const unsigned char *__restrict__ nlow, const T *__restrict__ lu,
                           const T *__restrict__ ud, int r0, int r1, int c0end,
@@ -656,10 +678,29 @@ __global__ void k_ilu_bwd(int n, int K, const int *__restrict__ col,
     double rz[1] = {0.};
     if (p < r1) {
         T acc = p < c0end ? c[p] : t[p];
-        for (int k = nlow[p]; k < K; ++k) {
-            const size_t o = (size_t)k * n + p;
-            const int cl = col[o];
-            if (cl > p) acc -= lu[o] * y[cl];  // padding slots hold the row itself
+        const int nl = nlow[p];
+        if (KK > 0) {
+            // padding slots hold the row itself and are skipped
+            int cl[KK > 0 ? KK : 1];
+            T a[KK > 0 ? KK : 1], v[KK > 0 ? KK : 1];
+#pragma unroll
+            for (int k = 0; k < KK; ++k) cl[k] = k >= nl ? col[(size_t)k * n + p] : p;
+#pragma unroll
+            for (int k = 0; k < KK; ++k) {
+                if (cl[k] > p) {
+                    a[k] = lu[(size_t)k * n + p];
+                    v[k] = y[cl[k]];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < KK; ++k)
+                if (cl[k] > p) acc -= a[k] * v[k];
+        } else {
+            for (int k = nl; k < K; ++k) {
+                const size_t o = (size_t)k * n + p;
+                const int cl = col[o];
+                if (cl > p) acc -= lu[o] * y[cl];  // padding slots hold the row itself
+            }
         }
         T yv = acc / ud[p];
         y[p] = yv;
@@ -1063,33 +1104,48 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
         const bool dilu = col->triangle_free;  // the factor's off-diagonals are A's
         const T *lu = dilu ? (const T *)A.val : (const T *)w.lu.p;
         int used = 0;
+        // one sweep launch for slot count KK (see k_ilu_fwd)
+        auto sweep = [&](auto kk, bool fwd, int r0, int r1, int g, int last, double *rpp) {
+            constexpr int KK = decltype(kk)::value;
+            const unsigned char *nl = col->nlow.p;
+            if (fwd) {
+                if (dilu)
+                    hipLaunchKernelGGL((k_ilu_fwd<T, true, KK>), dim3(g), dim3(kBlock), 0, st_, n,
+                                       A.K, A.col, nl, lu, (const T *)w.ud.p, r0, r1, c0end, c,
+                                       w.tmp.p, y, last, rpp, st);
+                else
+                    hipLaunchKernelGGL((k_ilu_fwd<T, false, KK>), dim3(g), dim3(kBlock), 0, st_,
+                                       n, A.K, A.col, nl, lu, (const T *)w.ud.p, r0, r1, c0end, c,
+                                       w.tmp.p, y, last, rpp, st);
+            } else {
+                if (dilu)
+                    hipLaunchKernelGGL((k_ilu_bwd<T, true, KK>), dim3(g), dim3(kBlock), 0, st_, n,
+                                       A.K, A.col, nl, lu, (const T *)w.ud.p, r0, r1, c0end, c,
+                                       (const T *)w.tmp.p, y, rpp, st);
+                else
+                    hipLaunchKernelGGL((k_ilu_bwd<T, false, KK>), dim3(g), dim3(kBlock), 0, st_,
+                                       n, A.K, A.col, nl, lu, (const T *)w.ud.p, r0, r1, c0end, c,
+                                       (const T *)w.tmp.p, y, rpp, st);
+            }
+        };
+        auto sweep_k = [&](bool fwd, int r0, int r1, int g, int last, double *rpp) {
+            if (A.K == 4) sweep(std::integral_constant<int, 4>(), fwd, r0, r1, g, last, rpp);
+            else if (A.K == 6) sweep(std::integral_constant<int, 6>(), fwd, r0, r1, g, last, rpp);
+            else sweep(std::integral_constant<int, 0>(), fwd, r0, r1, g, last, rpp);
+        };
         for (int k = nc > 1 ? 1 : 0; k < nc; ++k) {  // colour 0 only when it is the last
             const int r0 = col->off[k], r1 = col->off[k + 1];
             if (r1 <= r0) continue;
             const bool last = k == nc - 1;
             const int g = grid8(r1 - r0);
-            if (dilu)
-                hipLaunchKernelGGL((k_ilu_fwd<T, true>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
-                                   A.col, (const unsigned char *)col->nlow.p, lu, (const T *)w.ud.p, r0, r1, c0end, c, w.tmp.p, y,
-                                   (int)last, (last && rp) ? rp + used : (double *)nullptr, st);
-            else
-                hipLaunchKernelGGL((k_ilu_fwd<T, false>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
-                                   A.col, (const unsigned char *)col->nlow.p, lu, (const T *)w.ud.p, r0, r1, c0end, c, w.tmp.p, y,
-                                   (int)last, (last && rp) ? rp + used : (double *)nullptr, st);
+            sweep_k(true, r0, r1, g, (int)last, (last && rp) ? rp + used : (double *)nullptr);
             if (last && rp) used += g;
         }
         for (int k = nc - 2; k >= 0; --k) {
             const int r0 = col->off[k], r1 = col->off[k + 1];
             if (r1 <= r0) continue;
             const int g = grid8(r1 - r0);
-            if (dilu)
-                hipLaunchKernelGGL((k_ilu_bwd<T, true>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
-                                   A.col, (const unsigned char *)col->nlow.p, lu, (const T *)w.ud.p, r0, r1, c0end, c,
-                                   (const T *)w.tmp.p, y, rp ? rp + used : (double *)nullptr, st);
-            else
-                hipLaunchKernelGGL((k_ilu_bwd<T, false>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
-                                   A.col, (const unsigned char *)col->nlow.p, lu, (const T *)w.ud.p, r0, r1, c0end, c,
-                                   (const T *)w.tmp.p, y, rp ? rp + used : (double *)nullptr, st);
+            sweep_k(false, r0, r1, g, 0, rp ? rp + used : (double *)nullptr);
             if (rp) used += g;
         }
         if (rz_blocks && rp) *rz_blocks = used;

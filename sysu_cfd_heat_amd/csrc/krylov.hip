@@ -29,15 +29,37 @@ __device__ __forceinline__ bool rtc(double rn, double bn, double eps) {
     return bn > 0. ? rn < eps * bn : rn < eps;
 }
 
+// s + sum over the off-diagonal slots of row i of val * x[col], in slot
+// order. For K = 4 (tetrahedra) and 6 (hexahedra) the slot count is a
+// constant, so every slot's column load and then every gather are issued
+// together; K is the same for every thread, so the choice does not diverge.
+template <class T, int KK>
+__device__ __forceinline__ T row_sum_k(const EllT<T> &A, int i, const T *__restrict__ x, T s) {
+    const size_t n = A.n;
+    int c[KK];
+    T v[KK];
+#pragma unroll
+    for (int k = 0; k < KK; ++k) c[k] = A.col[k * n + i];
+#pragma unroll
+    for (int k = 0; k < KK; ++k) v[k] = A.val[k * n + i] * x[c[k]];
+#pragma unroll
+    for (int k = 0; k < KK; ++k) s += v[k];
+    return s;
+}
 template <class T>
-__device__ __forceinline__ T row_dot(const EllT<T> &A, int i, const T *__restrict__ x) {
+__device__ __forceinline__ T row_sum(const EllT<T> &A, int i, const T *__restrict__ x, T s) {
+    if (A.K == 4) return row_sum_k<T, 4>(A, i, x, s);
+    if (A.K == 6) return row_sum_k<T, 6>(A, i, x, s);
     const int n = A.n;
-    T s = A.diag[i] * x[i];
     for (int k = 0; k < A.K; ++k) {
         size_t o = (size_t)k * n + i;
         s += A.val[o] * x[A.col[o]];
     }
     return s;
+}
+template <class T>
+__device__ __forceinline__ T row_dot(const EllT<T> &A, int i, const T *__restrict__ x) {
+    return row_sum(A, i, x, A.diag[i] * x[i]);
 }
 
 #define GRID_LOOP(i, n) \
@@ -166,15 +188,8 @@ __global__ void k_cg_pq(EllT<T> A, const T *__restrict__ z, T *__restrict__ p, T
         const int j = i + stride;
         const bool hj = j < n;
         const T zi = z[i], zj = hj ? z[j] : T(0);
-        T si = A.diag[i] * zi, sj = hj ? A.diag[j] * zj : T(0);
-        for (int k = 0; k < A.K; ++k) {
-            const size_t oi = (size_t)k * n + i;
-            si += A.val[oi] * z[A.col[oi]];
-            if (hj) {
-                const size_t oj = (size_t)k * n + j;
-                sj += A.val[oj] * z[A.col[oj]];
-            }
-        }
+        T si = row_sum(A, i, z, A.diag[i] * zi);
+        T sj = hj ? row_sum(A, j, z, A.diag[j] * zj) : T(0);
         const T pi = zi + beta * p[i];
         si += beta * q[i];
         p[i] = pi;

@@ -630,51 +630,109 @@ __global__ void k_dilu_factor(int n, int K, const int *__restrict__ col,
 // issued past convergence).
 // D (triangle-free graph): lu holds A's off-diagonals; the lower factor
 // entry is formed as a_pc / u_cc, as k_ilu_factor would have stored it.
+// rows per thread of the ILU(0) sweep launches: one keeps two rounds of
+// waves per launch, which overlap better than two rows per thread
+constexpr int kSweepRows = 1;  // 2 measured slower (82.7 vs 78.5 us per application)
+
 // KK > 0: the slot count as a constant (4 for tetrahedra, 6 for hexahedra),
 // so that the column loads of every slot, then the gathers, are issued
 // together instead of one slot's chain after the other; KK = 0: any K.
 // The products are summed in slot order either way.
 template <class T, bool D, int KK>
+__device__ __forceinline__ T ilu_fwd_row(int n, int K, const int *__restrict__ col,
+                                         const unsigned char *__restrict__ nlow,
+                                         const T *__restrict__ lu, const T *__restrict__ ud,
+                                         int p, int c0end, const T *__restrict__ c,
+                                         const T *__restrict__ t) {
+    T acc = c[p];
+    const int nl = nlow[p];
+    if (KK > 0) {
+        int cl[KK > 0 ? KK : 1];
+        T a[KK > 0 ? KK : 1], v[KK > 0 ? KK : 1];
+#pragma unroll
+        for (int k = 0; k < KK; ++k) cl[k] = k < nl ? col[(size_t)k * n + p] : 0;
+#pragma unroll
+        for (int k = 0; k < KK; ++k) {
+            if (k < nl) {
+                const T l = lu[(size_t)k * n + p];
+                a[k] = D ? l / ud[cl[k]] : l;
+                v[k] = cl[k] < c0end ? c[cl[k]] : t[cl[k]];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KK; ++k)
+            if (k < nl) acc -= a[k] * v[k];
+    } else {
+        for (int k = 0; k < nl; ++k) {
+            const size_t o = (size_t)k * n + p;
+            const int cl = col[o];
+            acc -= (D ? lu[o] / ud[cl] : lu[o]) * (cl < c0end ? c[cl] : t[cl]);
+        }
+    }
+    return acc;
+}
+template <class T, int KK>
+__device__ __forceinline__ T ilu_bwd_row(int n, int K, const int *__restrict__ col,
+                                         const unsigned char *__restrict__ nlow,
+                                         const T *__restrict__ lu, int p, int c0end,
+                                         const T *__restrict__ c, const T *__restrict__ t,
+                                         const T *__restrict__ y) {
+    T acc = p < c0end ? c[p] : t[p];
+    const int nl = nlow[p];
+    if (KK > 0) {
+        // padding slots hold the row itself and are skipped
+        int cl[KK > 0 ? KK : 1];
+        T a[KK > 0 ? KK : 1], v[KK > 0 ? KK : 1];
+#pragma unroll
+        for (int k = 0; k < KK; ++k) cl[k] = k >= nl ? col[(size_t)k * n + p] : p;
+#pragma unroll
+        for (int k = 0; k < KK; ++k) {
+            if (cl[k] > p) {
+                a[k] = lu[(size_t)k * n + p];
+                v[k] = y[cl[k]];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KK; ++k)
+            if (cl[k] > p) acc -= a[k] * v[k];
+    } else {
+        for (int k = nl; k < K; ++k) {
+            const size_t o = (size_t)k * n + p;
+            const int cl = col[o];
+            if (cl > p) acc -= lu[o] * y[cl];
+        }
+    }
+    return acc;
+}
+
+// One sweep launch covers rows [r0, r1) as R interleaved parts of h rows:
+// thread q takes rows r0 + q, r0 + h + q, ... so that R rows' loads are in
+// flight together and the launch needs R times fewer waves.
+template <class T, bool D, int KK, int R>
 __global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col,
                           const unsigned char *__restrict__ nlow, const T *__restrict__ lu,
-                          const T *__restrict__ ud, int r0, int r1, int c0end,
+                          const T *__restrict__ ud, int r0, int r1, int h, int c0end,
                           const T *__restrict__ c, T *__restrict__ t, T *__restrict__ y, int fuse,
                           double *rpart, const KState *st) {
     if (st && st->done) return;
-    const int p = r0 + xcd_block() * blockDim.x + threadIdx.x;
+    const int q = xcd_block() * blockDim.x + threadIdx.x;
     double rz[1] = {0.};
-    if (p < r1) {
-        T acc = c[p];
-        const int nl = nlow[p];
-        if (KK > 0) {
-            int cl[KK > 0 ? KK : 1];
-            T a[KK > 0 ? KK : 1], v[KK > 0 ? KK : 1];
+    T acc[R];
 #pragma unroll
-            for (int k = 0; k < KK; ++k) cl[k] = k < nl ? col[(size_t)k * n + p] : 0;
+    for (int r = 0; r < R; ++r) {
+        const int p = r0 + r * h + q;
+        if (q < h && p < r1) acc[r] = ilu_fwd_row<T, D, KK>(n, K, col, nlow, lu, ud, p, c0end, c, t);
+    }
 #pragma unroll
-            for (int k = 0; k < KK; ++k) {
-                if (k < nl) {
-                    const T l = lu[(size_t)k * n + p];
-                    a[k] = D ? l / ud[cl[k]] : l;
-                    v[k] = cl[k] < c0end ? c[cl[k]] : t[cl[k]];
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < KK; ++k)
-                if (k < nl) acc -= a[k] * v[k];
-        } else {
-            for (int k = 0; k < nl; ++k) {
-                const size_t o = (size_t)k * n + p;
-                const int cl = col[o];
-                acc -= (D ? lu[o] / ud[cl] : lu[o]) * (cl < c0end ? c[cl] : t[cl]);
-            }
-        }
+    for (int r = 0; r < R; ++r) {
+        const int p = r0 + r * h + q;
+        if (!(q < h && p < r1)) continue;
         if (fuse) {
-            T yv = acc / ud[p];
+            T yv = acc[r] / ud[p];
             y[p] = yv;
-            rz[0] = (double)c[p] * yv;
+            rz[0] += (double)c[p] * yv;
         } else {
-            t[p] = acc;
+            t[p] = acc[r];
         }
     }
     if (rpart) {
@@ -682,44 +740,28 @@ __global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col,
         if (threadIdx.x == 0) rpart[blockIdx.x] = rz[0];
     }
 }
-template <class T, bool D, int KK>
+template <class T, bool D, int KK, int R>
 __global__ void k_ilu_bwd(int n, int K, const int *__restrict__ col,
                           const unsigned char *__restrict__ nlow, const T *__restrict__ lu,
-                          const T *__restrict__ ud, int r0, int r1, int c0end,
+                          const T *__restrict__ ud, int r0, int r1, int h, int c0end,
                           const T *__restrict__ c, const T *__restrict__ t, T *__restrict__ y,
                           double *rpart, const KState *st) {
     if (st && st->done) return;
-    const int p = r0 + xcd_block() * blockDim.x + threadIdx.x;
+    const int q = xcd_block() * blockDim.x + threadIdx.x;
     double rz[1] = {0.};
-    if (p < r1) {
-        T acc = p < c0end ? c[p] : t[p];
-        const int nl = nlow[p];
-        if (KK > 0) {
-            // padding slots hold the row itself and are skipped
-            int cl[KK > 0 ? KK : 1];
-            T a[KK > 0 ? KK : 1], v[KK > 0 ? KK : 1];
+    T acc[R];
 #pragma unroll
-            for (int k = 0; k < KK; ++k) cl[k] = k >= nl ? col[(size_t)k * n + p] : p;
+    for (int r = 0; r < R; ++r) {
+        const int p = r0 + r * h + q;
+        if (q < h && p < r1) acc[r] = ilu_bwd_row<T, KK>(n, K, col, nlow, lu, p, c0end, c, t, y);
+    }
 #pragma unroll
-            for (int k = 0; k < KK; ++k) {
-                if (cl[k] > p) {
-                    a[k] = lu[(size_t)k * n + p];
-                    v[k] = y[cl[k]];
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < KK; ++k)
-                if (cl[k] > p) acc -= a[k] * v[k];
-        } else {
-            for (int k = nl; k < K; ++k) {
-                const size_t o = (size_t)k * n + p;
-                const int cl = col[o];
-                if (cl > p) acc -= lu[o] * y[cl];  // padding slots hold the row itself
-            }
-        }
-        T yv = acc / ud[p];
+    for (int r = 0; r < R; ++r) {
+        const int p = r0 + r * h + q;
+        if (!(q < h && p < r1)) continue;
+        T yv = acc[r] / ud[p];
         y[p] = yv;
-        rz[0] = (double)c[p] * yv;
+        rz[0] += (double)c[p] * yv;
     }
     if (rpart) {
         block_sum<1>(rz);
@@ -1113,34 +1155,37 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
         int nbk = 0;  // workgroups of the launches that write partials
         for (int k = 0; k < nc; ++k) {
             const int r0 = col->off[k], r1 = col->off[k + 1];
-            if (r1 > r0) nbk += grid8(r1 - r0);  // last forward + every backward colour
+            if (r1 > r0) nbk += grid8((r1 - r0 + kSweepRows - 1) / kSweepRows);  // last fwd + every bwd
         }
         double *rp = (rz_blocks && nbk <= kPartSize - kPartRho) ? part_.p + kPartRho : nullptr;
         const bool dilu = col->triangle_free;  // the factor's off-diagonals are A's
         const T *lu = dilu ? (const T *)A.val : (const T *)w.lu.p;
         int used = 0;
-        // one sweep launch for slot count KK (see k_ilu_fwd)
+        // one sweep launch for slot count KK (see k_ilu_fwd), kSweepRows rows
+        // per thread over h = ceil(rows / kSweepRows)
         auto sweep = [&](auto kk, bool fwd, int r0, int r1, int g, int last, double *rpp) {
             constexpr int KK = decltype(kk)::value;
+            constexpr int R = kSweepRows;
+            const int h = (r1 - r0 + R - 1) / R;
             const unsigned char *nl = col->nlow.p;
             if (fwd) {
                 if (dilu)
-                    hipLaunchKernelGGL((k_ilu_fwd<T, true, KK>), dim3(g), dim3(kBlock), 0, st_, n,
-                                       A.K, A.col, nl, lu, (const T *)w.ud.p, r0, r1, c0end, c,
-                                       w.tmp.p, y, last, rpp, st);
+                    hipLaunchKernelGGL((k_ilu_fwd<T, true, KK, R>), dim3(g), dim3(kBlock), 0, st_,
+                                       n, A.K, A.col, nl, lu, (const T *)w.ud.p, r0, r1, h, c0end,
+                                       c, w.tmp.p, y, last, rpp, st);
                 else
-                    hipLaunchKernelGGL((k_ilu_fwd<T, false, KK>), dim3(g), dim3(kBlock), 0, st_,
-                                       n, A.K, A.col, nl, lu, (const T *)w.ud.p, r0, r1, c0end, c,
-                                       w.tmp.p, y, last, rpp, st);
+                    hipLaunchKernelGGL((k_ilu_fwd<T, false, KK, R>), dim3(g), dim3(kBlock), 0,
+                                       st_, n, A.K, A.col, nl, lu, (const T *)w.ud.p, r0, r1, h,
+                                       c0end, c, w.tmp.p, y, last, rpp, st);
             } else {
                 if (dilu)
-                    hipLaunchKernelGGL((k_ilu_bwd<T, true, KK>), dim3(g), dim3(kBlock), 0, st_, n,
-                                       A.K, A.col, nl, lu, (const T *)w.ud.p, r0, r1, c0end, c,
-                                       (const T *)w.tmp.p, y, rpp, st);
+                    hipLaunchKernelGGL((k_ilu_bwd<T, true, KK, R>), dim3(g), dim3(kBlock), 0, st_,
+                                       n, A.K, A.col, nl, lu, (const T *)w.ud.p, r0, r1, h, c0end,
+                                       c, (const T *)w.tmp.p, y, rpp, st);
                 else
-                    hipLaunchKernelGGL((k_ilu_bwd<T, false, KK>), dim3(g), dim3(kBlock), 0, st_,
-                                       n, A.K, A.col, nl, lu, (const T *)w.ud.p, r0, r1, c0end, c,
-                                       (const T *)w.tmp.p, y, rpp, st);
+                    hipLaunchKernelGGL((k_ilu_bwd<T, false, KK, R>), dim3(g), dim3(kBlock), 0,
+                                       st_, n, A.K, A.col, nl, lu, (const T *)w.ud.p, r0, r1, h,
+                                       c0end, c, (const T *)w.tmp.p, y, rpp, st);
             }
         };
         auto sweep_k = [&](bool fwd, int r0, int r1, int g, int last, double *rpp) {
@@ -1152,14 +1197,14 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
             const int r0 = col->off[k], r1 = col->off[k + 1];
             if (r1 <= r0) continue;
             const bool last = k == nc - 1;
-            const int g = grid8(r1 - r0);
+            const int g = grid8((r1 - r0 + kSweepRows - 1) / kSweepRows);
             sweep_k(true, r0, r1, g, (int)last, (last && rp) ? rp + used : (double *)nullptr);
             if (last && rp) used += g;
         }
         for (int k = nc - 2; k >= 0; --k) {
             const int r0 = col->off[k], r1 = col->off[k + 1];
             if (r1 <= r0) continue;
-            const int g = grid8(r1 - r0);
+            const int g = grid8((r1 - r0 + kSweepRows - 1) / kSweepRows);
             sweep_k(false, r0, r1, g, 0, rp ? rp + used : (double *)nullptr);
             if (rp) used += g;
         }

@@ -46,6 +46,10 @@ def parse():
                     help="channel: the headline CHT channel (default); cavity: the 32^3 "
                          "Re=100 lid-driven cavity; batch: 64 uneven cavity/channel cases "
                          "packed per launch (extra lines, not the headline)")
+    ap.add_argument("--solvers", default="config", choices=["config", "reference"],
+                    help="config: BiCGSTAB+Jacobi transport, CG+ILU(0) pressure and solid, "
+                         "1e-8 (the headline); reference: the solvers hard-coded at the "
+                         "reference's call sites (GMRES+SSOR transport, CG+SSOR pressure 1e-6)")
     ap.add_argument("--pres-precision", type=int, default=0,
                     help="pressure solve: 0 FP64 CG (the headline), 1 mixed precision (FP32 "
                          "inner CG, FP64 residual and update)")
@@ -160,7 +164,7 @@ def run_cavity(a):
     """Laminar lid-driven cavity, 32^3 hexahedra, Re = 100, BiCGSTAB+Jacobi
     momentum, CG+ILU(0) pressure, 1e-8: SIMPLE iterations per second."""
     from sysu_cfd_heat_amd import NSSolver, cases
-    raw, p = cases.cavity(n=32, re=100.0, max_step=a.warmup + a.steps + 2)
+    raw, p = cases.cavity(n=32, re=100.0, max_step=a.warmup + a.steps + 2, solvers=a.solvers)
     gpu = NSSolver(raw, p)
     gpu.set_option("graphs", a.graphs)
     for _ in range(a.warmup):
@@ -241,7 +245,7 @@ def main():
 
     t0 = time.time()
     raw, p = cases.channel(nx=a.nx, ny=a.ny, nz=a.nz, tets=True, jitter=0.1, seed=1,
-                           max_step=a.warmup + a.steps + 2)
+                           max_step=a.warmup + a.steps + 2, solvers=a.solvers)
     p.pres_precision = a.pres_precision
     gpu = NSSolver(raw, p)
     gpu.set_option("mixed_inner_tol", a.mixed_inner_tol)
@@ -290,7 +294,7 @@ def main():
         ny = max(1, a.ny * cnx // a.nx)
         nz = max(1, a.nz * cnx // a.nx)
         craw, cp = cases.channel(nx=cnx, ny=ny, nz=nz, tets=True, jitter=0.1, seed=1,
-                                 max_step=a.cpu_sample + 2)
+                                 max_step=a.cpu_sample + 2, solvers=a.solvers)
         t2 = time.time()
         ref = O.NSSolver(craw, cp, mc=1)
         print(f"[bench] oracle setup {time.time() - t2:.1f}s", file=sys.stderr, flush=True)
@@ -317,7 +321,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"CHT channel {a.nx}x{a.ny}x{a.nz} hex x6 jittered tets "
                                f"({raw.counts[1]} cells), k-eps Re=1e4, energy + solid",
-                   "cells": raw.counts[1], "solvers": "BiCGSTAB+Jacobi / CG+ILU(0), 1e-8",
+                   "cells": raw.counts[1],
+                   "solvers": "BiCGSTAB+Jacobi / CG+ILU(0), 1e-8" if a.solvers == "config" else
+                   "reference: GMRES+SSOR / CG+SSOR(1.3) 1e-6",
                    "parallelism": "single GPU"},
         "roofline": roof, "cpu_baseline": cpu, "krylov": krylov, "p_iters_learned": learned,
         "device_s_per_step": st[1] / max(st[0], 1),

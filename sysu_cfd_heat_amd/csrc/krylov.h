@@ -148,9 +148,9 @@ class Krylov {
     template <class T>
     void setup_pc(const EllT<T> &A, int pc, Colouring *col, KWork<T> &w);
     template <class T>
-    // rz_blocks: ILU only; also write the workgroup partials of c.y to the
+    // rz_blocks: ILU and SSOR; also write the workgroup partials of c.y to the
     // partials buffer from offset kPartRho and return their count (-1 when
-    // they do not fit: the caller reduces c.y itself). st: the ILU sweeps do
+    // they do not fit: the caller reduces c.y itself). st: the sweeps do
     // nothing once that solve has stopped.
     void apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, Colouring *col,
                   KWork<T> &w, int *rz_blocks = nullptr, const KState *st = nullptr);

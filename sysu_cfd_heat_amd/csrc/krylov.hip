@@ -141,8 +141,6 @@ __global__ void k_copy(int n, const double *__restrict__ a, double *__restrict__
     GRID_LOOP(i, n) b[i] = a[i];
 }
 
-template <class T>
-__global__ void k_scale(int n, double f, T *__restrict__ y) { GRID_LOOP(i, n) y[i] = T(f) * y[i]; }
 
 template <class T>
 __global__ void k_jacobi(int n, const T *__restrict__ d, const T *__restrict__ c,
@@ -506,33 +504,122 @@ __global__ void k_add(int n, double *__restrict__ x, const double *__restrict__ 
 // ----------------------------------------------- multicolour sweeps --
 // Rows [r0, r1) form one colour block of the permuted matrix; a neighbour
 // is "lower" iff its permuted index is below the row's.
-template <class T>
-__global__ void k_ssor_fwd(EllT<T> A, T omega, int r0, int r1, const T *__restrict__ c,
-                           T *__restrict__ t) {
-    const int n = A.n;
-    int p = r0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= r1) return;
+// SSOR: M = (D + wL) D^-1 (D + wU) / (w (2 - w)) over colour blocks of the
+// permuted matrix (rows list their earlier-colour neighbours first, nlow[p]
+// of them). y = M^-1 c: forward t = (D + wL)^-1 c, backward
+// y = s (D + wU)^-1 D t with s = w (2 - w) applied inside the recursion
+// (y_p = (s d_p t_p - w sum a_pc y_c) / d_p), so no scaling pass follows.
+// As for ILU(0): colour 0's forward result c / d is formed where it is read,
+// the last colour's backward step runs in its forward launch, and with rpart
+// every workgroup writes its partial of c.y. KK as in the ILU sweeps.
+template <class T, int KK>
+__device__ __forceinline__ T ssor_fwd_row(int n, const int *__restrict__ col,
+                                          const unsigned char *__restrict__ nlow,
+                                          const T *__restrict__ val, const T *__restrict__ d,
+                                          T omega, int p, int c0end, const T *__restrict__ c,
+                                          const T *__restrict__ t) {
     T acc = c[p];
-    for (int k = 0; k < A.K; ++k) {
-        size_t o = (size_t)k * n + p;
-        int cl = A.col[o];
-        if (cl < p) acc -= omega * A.val[o] * t[cl];
+    const int nl = nlow[p];
+    if (KK > 0) {
+        int cl[KK > 0 ? KK : 1];
+        T a[KK > 0 ? KK : 1], v[KK > 0 ? KK : 1];
+#pragma unroll
+        for (int k = 0; k < KK; ++k) cl[k] = k < nl ? col[(size_t)k * n + p] : 0;
+#pragma unroll
+        for (int k = 0; k < KK; ++k) {
+            if (k < nl) {
+                a[k] = val[(size_t)k * n + p];
+                v[k] = cl[k] < c0end ? c[cl[k]] / d[cl[k]] : t[cl[k]];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KK; ++k)
+            if (k < nl) acc -= omega * a[k] * v[k];
+    } else {
+        for (int k = 0; k < nl; ++k) {
+            const size_t o = (size_t)k * n + p;
+            const int cl = col[o];
+            acc -= omega * val[o] * (cl < c0end ? c[cl] / d[cl] : t[cl]);
+        }
     }
-    t[p] = acc / A.diag[p];
+    return acc / d[p];
 }
-template <class T>
-__global__ void k_ssor_bwd(EllT<T> A, T omega, int r0, int r1, const T *__restrict__ t,
-                           T *__restrict__ y) {
-    const int n = A.n;
-    int p = r0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= r1) return;
-    T acc = A.diag[p] * t[p];
-    for (int k = 0; k < A.K; ++k) {
-        size_t o = (size_t)k * n + p;
-        int cl = A.col[o];
-        if (cl > p) acc -= omega * A.val[o] * y[cl];
+template <class T, int KK>
+__device__ __forceinline__ T ssor_bwd_row(int n, int K, const int *__restrict__ col,
+                                          const unsigned char *__restrict__ nlow,
+                                          const T *__restrict__ val, const T *__restrict__ d,
+                                          T omega, T scale, int p, T tp,
+                                          const T *__restrict__ y) {
+    const T dp = d[p];
+    T acc = scale * (dp * tp);
+    const int nl = nlow[p];
+    if (KK > 0) {
+        int cl[KK > 0 ? KK : 1];
+        T a[KK > 0 ? KK : 1], v[KK > 0 ? KK : 1];
+#pragma unroll
+        for (int k = 0; k < KK; ++k) cl[k] = k >= nl ? col[(size_t)k * n + p] : p;
+#pragma unroll
+        for (int k = 0; k < KK; ++k) {
+            if (cl[k] > p) {
+                a[k] = val[(size_t)k * n + p];
+                v[k] = y[cl[k]];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KK; ++k)
+            if (cl[k] > p) acc -= omega * a[k] * v[k];
+    } else {
+        for (int k = nl; k < K; ++k) {
+            const size_t o = (size_t)k * n + p;
+            const int cl = col[o];
+            if (cl > p) acc -= omega * val[o] * y[cl];  // padding slots hold the row itself
+        }
     }
-    y[p] = acc / A.diag[p];
+    return acc / dp;
+}
+template <class T, int KK>
+__global__ void k_ssor_fwd(int n, int K, const int *__restrict__ col,
+                           const unsigned char *__restrict__ nlow, const T *__restrict__ val,
+                           const T *__restrict__ d, T omega, T scale, int r0, int r1, int c0end,
+                           const T *__restrict__ c, T *__restrict__ t, T *__restrict__ y,
+                           int fuse, double *rpart, const KState *st) {
+    if (st && st->done) return;
+    const int p = r0 + xcd_block() * blockDim.x + threadIdx.x;
+    double rz[1] = {0.};
+    if (p < r1) {
+        const T tp = ssor_fwd_row<T, KK>(n, col, nlow, val, d, omega, p, c0end, c, t);
+        if (fuse) {  // the last colour has no later neighbours
+            const T yv = (scale * (d[p] * tp)) / d[p];
+            y[p] = yv;
+            rz[0] = (double)c[p] * yv;
+        } else {
+            t[p] = tp;
+        }
+    }
+    if (rpart) {
+        block_sum<1>(rz);
+        if (threadIdx.x == 0) rpart[blockIdx.x] = rz[0];
+    }
+}
+template <class T, int KK>
+__global__ void k_ssor_bwd(int n, int K, const int *__restrict__ col,
+                           const unsigned char *__restrict__ nlow, const T *__restrict__ val,
+                           const T *__restrict__ d, T omega, T scale, int r0, int r1, int c0end,
+                           const T *__restrict__ c, const T *__restrict__ t, T *__restrict__ y,
+                           double *rpart, const KState *st) {
+    if (st && st->done) return;
+    const int p = r0 + xcd_block() * blockDim.x + threadIdx.x;
+    double rz[1] = {0.};
+    if (p < r1) {
+        const T tp = p < c0end ? c[p] / d[p] : t[p];
+        const T yv = ssor_bwd_row<T, KK>(n, K, col, nlow, val, d, omega, scale, p, tp, y);
+        y[p] = yv;
+        rz[0] = (double)c[p] * yv;
+    }
+    if (rpart) {
+        block_sum<1>(rz);
+        if (threadIdx.x == 0) rpart[blockIdx.x] = rz[0];
+    }
 }
 
 constexpr int kIluMaxK = 16;
@@ -1135,20 +1222,49 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
         hipLaunchKernelGGL(k_jacobi<T>, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
                            (const T *)A.diag, c, y);
     } else if (pc == CFD_PC_SSOR) {
-        for (int k = 0; k < col->ncolour; ++k) {
-            int r0 = col->off[k], r1 = col->off[k + 1];
-            if (r1 > r0)
-                hipLaunchKernelGGL(k_ssor_fwd<T>, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_,
-                                   A, (T)omega, r0, r1, c, w.tmp.p);
+        const int nc = col->ncolour;
+        const int c0end = nc > 1 ? col->off[1] : n;
+        int nbk = 0;
+        for (int k = 0; k < nc; ++k) {
+            const int r0 = col->off[k], r1 = col->off[k + 1];
+            if (r1 > r0) nbk += grid8(r1 - r0);  // last forward + every backward colour
         }
-        for (int k = col->ncolour - 1; k >= 0; --k) {
-            int r0 = col->off[k], r1 = col->off[k + 1];
-            if (r1 > r0)
-                hipLaunchKernelGGL(k_ssor_bwd<T>, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_,
-                                   A, (T)omega, r0, r1, (const T *)w.tmp.p, y);
+        double *rp = (rz_blocks && nbk <= kPartSize - kPartRho) ? part_.p + kPartRho : nullptr;
+        const T om = (T)omega, sc = (T)(omega * (2. - omega));
+        auto sweep = [&](auto kk, bool fwd, int r0, int r1, int g, int last, double *rpp) {
+            constexpr int KK = decltype(kk)::value;
+            const unsigned char *nl = col->nlow.p;
+            if (fwd)
+                hipLaunchKernelGGL((k_ssor_fwd<T, KK>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
+                                   A.col, nl, (const T *)A.val, (const T *)A.diag, om, sc, r0,
+                                   r1, c0end, c, w.tmp.p, y, last, rpp, st);
+            else
+                hipLaunchKernelGGL((k_ssor_bwd<T, KK>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
+                                   A.col, nl, (const T *)A.val, (const T *)A.diag, om, sc, r0,
+                                   r1, c0end, c, (const T *)w.tmp.p, y, rpp, st);
+        };
+        auto sweep_k = [&](bool fwd, int r0, int r1, int g, int last, double *rpp) {
+            if (A.K == 4) sweep(std::integral_constant<int, 4>(), fwd, r0, r1, g, last, rpp);
+            else if (A.K == 6) sweep(std::integral_constant<int, 6>(), fwd, r0, r1, g, last, rpp);
+            else sweep(std::integral_constant<int, 0>(), fwd, r0, r1, g, last, rpp);
+        };
+        int used = 0;
+        for (int k = nc > 1 ? 1 : 0; k < nc; ++k) {  // colour 0 only when it is the last
+            const int r0 = col->off[k], r1 = col->off[k + 1];
+            if (r1 <= r0) continue;
+            const bool last = k == nc - 1;
+            const int g = grid8(r1 - r0);
+            sweep_k(true, r0, r1, g, (int)last, (last && rp) ? rp + used : (double *)nullptr);
+            if (last && rp) used += g;
         }
-        hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
-                           omega * (2. - omega), y);
+        for (int k = nc - 2; k >= 0; --k) {
+            const int r0 = col->off[k], r1 = col->off[k + 1];
+            if (r1 <= r0) continue;
+            const int g = grid8(r1 - r0);
+            sweep_k(false, r0, r1, g, 0, rp ? rp + used : (double *)nullptr);
+            if (rp) used += g;
+        }
+        if (rz_blocks && rp) *rz_blocks = used;
     } else {
         const int nc = col->ncolour;
         const int c0end = nc > 1 ? col->off[1] : n;
@@ -1368,7 +1484,7 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
     auto precond_rho_partials = [&](const KState *stp) -> int {
         int nbk = -1;
         apply_pc(A, pc, cfg.omega, (const T *)w.r.p, w.z.p, col, w,
-                 pc == CFD_PC_ILU ? &nbk : nullptr, stp);
+                 (pc == CFD_PC_ILU || pc == CFD_PC_SSOR) ? &nbk : nullptr, stp);
         if (nbk < 0) {
             nbk = G;
             hipLaunchKernelGGL(k_cg_rho<T>, dim3(G), dim3(kBlock), 0, st_, n, (const T *)w.r.p,

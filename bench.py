@@ -42,10 +42,13 @@ def parse():
                     help="p-solve iterations: learned guess (+ mixed precision) vs plain FP64")
     ap.add_argument("--learn-nx", type=int, default=32,
                     help="training mesh nx (ny, nz = nx/2) for the learned guess")
-    ap.add_argument("--workload", default="channel", choices=["channel", "cavity", "batch"],
+    ap.add_argument("--workload", default="channel",
+                    choices=["channel", "cavity", "batch", "laspack"],
                     help="channel: the headline CHT channel (default); cavity: the 32^3 "
                          "Re=100 lid-driven cavity; batch: 64 uneven cavity/channel cases "
-                         "packed per launch (extra lines, not the headline)")
+                         "packed per launch; laspack: the LASPack-compatible boundary called "
+                         "as the reference calls it, host matrix and vectors crossing PCIe "
+                         "(extra lines, not the headline)")
     ap.add_argument("--solvers", default="config", choices=["config", "reference"],
                     help="config: BiCGSTAB+Jacobi transport, CG+ILU(0) pressure and solid, "
                          "1e-8 (the headline); reference: the solvers hard-coded at the "
@@ -226,6 +229,22 @@ def run_batch(a):
                                                    "ke_iters_per_step": st[4] / a.steps}})
 
 
+def run_laspack(a):
+    """The LASPack-compatible C-ABI driven as the reference's pressure step
+    drives it (a new symmetric QMatrix filled with Q_SetLen / Q_SetEntry
+    every outer iteration, CGIter + ILUPrecond to 1e-8), on the 7-point
+    Laplacian of a 146^3 grid (3.1M rows): the PCIe-inclusive boundary rate."""
+    import subprocess
+    exe = os.path.join(ROOT, "sysu_cfd_heat_amd", "laspack_bench")
+    out = subprocess.run([exe, "146", str(max(2, a.steps + 1))], check=True,
+                         capture_output=True, text=True).stdout.strip().splitlines()[-1]
+    r = json.loads(out)
+    _line("CGIter solves/s through the LASPack boundary, 3.1M rows, new matrix per solve",
+          1000.0 / r["cgiter_new_matrix_ms"], "solves/s", a, r["cgiter_new_matrix_ms"],
+          f"7-point Laplacian 146^3 ({r['n']} rows), CG+ILU(0) 1e-8, host QMatrix per solve",
+          {"boundary": r})
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -240,6 +259,8 @@ def main():
         return run_cavity(a)
     if a.workload == "batch":
         return run_batch(a)
+    if a.workload == "laspack":
+        return run_laspack(a)
     import numpy as np
     from sysu_cfd_heat_amd import NSSolver, cases
 

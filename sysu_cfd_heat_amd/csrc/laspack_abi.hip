@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "krylov.h"
@@ -18,14 +19,29 @@
 namespace cfd {
 namespace {
 
+// Device matrix of one sparsity structure. The reference builds a new
+// QMatrix with the same pattern every outer iteration
+// (src/navier_pressure.cpp:14-18,74), so the ELL columns, the colouring and
+// each entry's ELL position are kept per structure; a new QMatrix with a
+// known pattern only sends its values (one pinned copy and a scatter kernel).
 struct DevQ {
     int n = 0, K = 0;
-    DBuf<double> diag, val;
-    DBuf<int> col;
+    bool sym = false;
+    size_t nent = 0;
+    std::vector<size_t> rowptr;  // host: first flat entry of each row
+    std::vector<int> cols;       // host: 0-based column of each flat entry (the key)
+    DBuf<double> diag, val, flat;
+    DBuf<int> col, own, mir;
+    DBuf<size_t> rp;
+    const QMatrix *owner = nullptr;  // the QMatrix whose values the device holds
     std::vector<int> hcol;
+    double *staging = nullptr;   // pinned host copy of the flat values
     Colouring colour;
     bool colour_ok = false;
     Ell ell() { return Ell{n, K, diag.p, val.p, col.p}; }
+    ~DevQ() {
+        if (staging) (void)hipHostFree(staging);
+    }
 };
 
 struct LasState {
@@ -36,6 +52,7 @@ struct LasState {
     int gmres_m = 10;
     std::unique_ptr<Krylov> kry;
     DBuf<double> x, b;
+    std::vector<std::unique_ptr<DevQ>> structs;  // most recently used last
     Krylov &K() {
         if (!kry) {
             hipStream_t s;
@@ -54,51 +71,163 @@ void set_err(LASErrIdType e) {
     if (las().err == LASOK) las().err = e;
 }
 
-// rows -> device ELL (diagonal separate; duplicate diagonal entries summed)
-DevQ *upload(QMatrix *Q) {
-    DevQ *d = (DevQ *)Q->Dev;
-    if (d && !Q->Dirty) return d;
-    if (!d) {
-        d = new DevQ;
-        Q->Dev = d;
+// values of the flat entries -> ELL: diagonal entries summed in entry order,
+// each off-diagonal entry to its slot (and, for symmetric storage, to the
+// mirrored slot of the other row); every slot is written by one entry
+__global__ void k_scatter_entries(int n, const size_t *__restrict__ rp,
+                                  const int *__restrict__ own, const int *__restrict__ mir,
+                                  const double *__restrict__ flat, double *__restrict__ diag,
+                                  double *__restrict__ val) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        double d = 0.;
+        for (size_t e = rp[r]; e < rp[r + 1]; ++e) {
+            const double v = flat[e];
+            const int o = own[e];
+            if (o < 0) {
+                d += v;
+                continue;
+            }
+            val[o] = v;
+            if (mir[e] >= 0) val[mir[e]] = v;
+        }
+        diag[r] = d;
     }
+}
+
+// f(r0, r1) over the 1-based rows [1, n], in up to 16 host threads
+template <class F>
+bool rows_parallel(int n, F f) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int T = n < (1 << 16) ? 1 : (int)std::min(16u, hw);
+    std::vector<char> ok(T, 1);
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t) {
+        const int r0 = 1 + (int)((long long)n * t / T), r1 = 1 + (int)((long long)n * (t + 1) / T);
+        if (t + 1 == T) ok[t] = f(r0, r1);
+        else th.emplace_back([&ok, &f, t, r0, r1] { ok[t] = f(r0, r1); });
+    }
+    for (auto &x : th) x.join();
+    for (char o : ok)
+        if (!o) return false;
+    return true;
+}
+
+// does Q have the pattern of d? (row lengths and columns, entry by entry)
+bool same_structure(const QMatrix *Q, const DevQ &d) {
     const int n = (int)Q->Dim;
-    std::vector<double> diag(n, 0.);
-    std::vector<std::vector<std::pair<int, double>>> off(n);
+    if (d.n != n || d.sym != (Q->Symmetry == True)) return false;
+    return rows_parallel(n, [&](int r0, int r1) {
+        for (int r = r0; r < r1; ++r) {
+            const size_t e0 = d.rowptr[r - 1];
+            if (Q->Len[r] != d.rowptr[r] - e0) return false;
+            const size_t *c = Q->Col[r];
+            for (size_t e = 0; e < Q->Len[r]; ++e)
+                if ((int)c[e] - 1 != d.cols[e0 + e]) return false;
+        }
+        return true;
+    });
+}
+
+// the structure of Q: flat entries, ELL columns (slot order: a row's
+// off-diagonals as they are met scanning the rows in order, mirrored entries
+// of symmetric storage included), each entry's slot, the device arrays
+std::unique_ptr<DevQ> build_structure(const QMatrix *Q, hipStream_t s) {
+    std::unique_ptr<DevQ> d(new DevQ);
+    const int n = (int)Q->Dim;
+    d->n = n;
+    d->sym = Q->Symmetry == True;
+    d->rowptr.assign(n + 1, 0);
+    for (int r = 1; r <= n; ++r) d->rowptr[r] = d->rowptr[r - 1] + Q->Len[r];
+    d->nent = d->rowptr[n];
+    if (d->nent >= (size_t)1 << 31) throw HipError("more than 2^31 matrix entries");
+    d->cols.resize(d->nent);
+    std::vector<int> cnt(n, 0);
     for (int r = 1; r <= n; ++r)
         for (size_t e = 0; e < Q->Len[r]; ++e) {
-            size_t c = Q->Col[r][e];
+            const size_t c = Q->Col[r][e];
             if (c < 1 || c > (size_t)n) throw HipError("Q_SetEntry column out of range");
-            double v = Q->Val[r][e];
-            if ((int)c == r) diag[r - 1] += v;
-            else {
-                off[r - 1].push_back({(int)c - 1, v});
-                if (Q->Symmetry) off[c - 1].push_back({r - 1, v});
+            d->cols[d->rowptr[r - 1] + e] = (int)c - 1;
+            if ((int)c != r) {
+                cnt[r - 1]++;
+                if (d->sym) cnt[c - 1]++;
             }
         }
     int K = 1;
-    for (auto &o : off) K = std::max<int>(K, (int)o.size());
+    for (int c : cnt) K = std::max(K, c);
     if (K > 64) throw HipError("more than 64 off-diagonal entries in a row");
-    std::vector<double> val((size_t)K * n, 0.);
-    d->hcol.assign((size_t)K * n, 0);
-    for (int i = 0; i < n; ++i)
-        for (int k = 0; k < K; ++k) {
-            bool has = k < (int)off[i].size();
-            d->hcol[(size_t)k * n + i] = has ? off[i][k].first : i;
-            val[(size_t)k * n + i] = has ? off[i][k].second : 0.;
-        }
-    hipStream_t s = las().K().stream();
-    d->n = n;
     d->K = K;
+    d->hcol.resize((size_t)K * n);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < K; ++k) d->hcol[(size_t)k * n + i] = i;  // padding: the row itself
+    std::vector<int> own(d->nent, -1), mir(d->nent, -1), fill(n, 0);
+    for (int r = 0; r < n; ++r)
+        for (size_t e = d->rowptr[r]; e < d->rowptr[r + 1]; ++e) {
+            const int c = d->cols[e];
+            if (c == r) continue;
+            const int k = fill[r]++;
+            own[e] = k * n + r;
+            d->hcol[(size_t)k * n + r] = c;
+            if (d->sym) {
+                const int kc = fill[c]++;
+                mir[e] = kc * n + c;
+                d->hcol[(size_t)kc * n + c] = r;
+            }
+        }
     d->diag.alloc(n);
     d->val.alloc((size_t)K * n);
+    d->val.zero(s);  // padding slots stay zero
     d->col.alloc((size_t)K * n);
-    d->diag.upload(diag.data(), n, s);
-    d->val.upload(val.data(), (size_t)K * n, s);
     d->col.upload(d->hcol.data(), (size_t)K * n, s);
-    d->colour_ok = false;
+    d->rp.alloc(n + 1);
+    d->rp.upload(d->rowptr.data(), n + 1, s);
+    d->own.alloc(std::max<size_t>(d->nent, 1));
+    d->mir.alloc(std::max<size_t>(d->nent, 1));
+    d->own.upload(own.data(), d->nent, s);
+    d->mir.upload(mir.data(), d->nent, s);
+    d->flat.alloc(std::max<size_t>(d->nent, 1));
+    CFD_HIP(hipHostMalloc((void **)&d->staging, sizeof(double) * std::max<size_t>(d->nent, 1),
+                          hipHostMallocDefault));
+    return d;
+}
+
+DevQ *upload(QMatrix *Q) {
+    LasState &L = las();
+    hipStream_t s = L.K().stream();
+    DevQ *d = nullptr;
+    for (size_t i = 0; i < L.structs.size(); ++i)
+        if (L.structs[i].get() == (DevQ *)Q->Dev) d = L.structs[i].get();
+    if (d && !Q->Dirty && d->owner == Q) return d;
+    d = nullptr;
+    for (size_t i = 0; i < L.structs.size(); ++i)
+        if (same_structure(Q, *L.structs[i])) {
+            std::unique_ptr<DevQ> hit = std::move(L.structs[i]);  // most recently used last
+            L.structs.erase(L.structs.begin() + i);
+            L.structs.push_back(std::move(hit));
+            d = L.structs.back().get();
+            break;
+        }
+    if (!d) {
+        if (L.structs.size() >= 8) L.structs.erase(L.structs.begin());
+        L.structs.push_back(build_structure(Q, s));
+        d = L.structs.back().get();
+    }
+    // the values: one pinned copy, one transfer, the scatter on the device
+    const int n = d->n;
+    rows_parallel(n, [&](int r0, int r1) {
+        for (int r = r0; r < r1; ++r)
+            std::memcpy(d->staging + d->rowptr[r - 1], Q->Val[r], sizeof(double) * Q->Len[r]);
+        return true;
+    });
+    CFD_HIP(hipMemcpyAsync(d->flat.p, d->staging, sizeof(double) * d->nent,
+                           hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_scatter_entries, dim3(grid_for(n)), dim3(kBlock), 0, s, n,
+                       (const size_t *)d->rp.p, (const int *)d->own.p, (const int *)d->mir.p,
+                       (const double *)d->flat.p, d->diag.p, d->val.p);
+    CFD_LAUNCH_CHECK();
+    CFD_HIP(hipStreamSynchronize(s));  // the staging buffer is reused by the next upload
+    Q->Dev = d;
     Q->Dirty = 0;
-    CFD_HIP(hipStreamSynchronize(s));
+    d->owner = Q;
     return d;
 }
 
@@ -242,8 +371,7 @@ void Q_Destr(QMatrix *Q) {
     free(Q->Cap);
     free(Q->Col);
     free(Q->Val);
-    free(Q->Name);
-    delete (DevQ *)Q->Dev;
+    free(Q->Name);  // the device matrix belongs to the structure cache
     std::memset(Q, 0, sizeof(*Q));
 }
 

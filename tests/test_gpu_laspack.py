@@ -207,3 +207,41 @@ def test_ilu_general_and_diagonal_paths_match_oracle(kind, oracle):
     assert L.LASResult() == 0 and L.GetLastAccuracy() < 1e-10
     assert np.linalg.norm(A @ _get(x) - c) <= 1e-9 * np.linalg.norm(c)
     L.Q_Destr(C.byref(Q)); L.V_Destr(C.byref(y)); L.V_Destr(C.byref(cv)); L.V_Destr(C.byref(x))
+
+
+def test_new_matrices_with_one_pattern_and_two_live_matrices():
+    """The reference's pattern: a new QMatrix with the same structure every
+    outer iteration (the device keeps the structure and receives only the
+    values), plus two live matrices of one pattern used alternately, plus a
+    matrix changed in place (Dirty): every solve uses its own values."""
+    L = _las()
+    A0 = _lap(4)
+    n = A0.shape[0]
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal(n)
+    bv = _vector(L, b)
+    L.SetRTCAccuracy(1e-12)
+    ilu = C.cast(L.ILUPrecond, C.c_void_p)
+
+    def solve(Q, A):
+        x = _vector(L, np.zeros(n))
+        L.CGIter(C.byref(Q), C.byref(x), C.byref(bv), 2000, ilu, 1.0)
+        assert L.LASResult() == 0
+        got = _get(x)
+        L.V_Destr(C.byref(x))
+        np.testing.assert_allclose(got, np.linalg.solve(A, b), rtol=1e-9, atol=1e-12)
+
+    for k in range(3):  # new matrix each time, same pattern, other values
+        A = A0 * (1.0 + 0.3 * k) + np.diag(np.full(n, 0.1 * k))
+        Q = _matrix(L, A, sym=True)
+        solve(Q, A)
+        L.Q_Destr(C.byref(Q))
+    A1, A2 = A0 * 2.0, A0 * 0.5 + np.eye(n)
+    Q1, Q2 = _matrix(L, A1, sym=True), _matrix(L, A2, sym=True)
+    for Q, A in ((Q1, A1), (Q2, A2), (Q1, A1), (Q2, A2)):
+        solve(Q, A)
+    L.Q_SetEntry(C.byref(Q1), 1, 0, 1, A1[0, 0] + 3.0)  # changed in place
+    A1 = A1.copy()
+    A1[0, 0] += 3.0
+    solve(Q1, A1)
+    L.Q_Destr(C.byref(Q1)); L.Q_Destr(C.byref(Q2)); L.V_Destr(C.byref(bv))

@@ -122,3 +122,25 @@ def test_solid_overlap_is_bit_identical(steady):
         assert np.array_equal(x, y)
     assert np.array_equal(sa[[0, 2, 3, 4, 5, 6]], sb[[0, 2, 3, 4, 5, 6]])
     assert la == lb and sa[6] > 0
+
+
+def test_solid_failure_reported_by_the_next_call():
+    """A solid solve that cannot converge while it runs beside the fluid is
+    reported by the next call (here the next Iterate), with the reference's
+    message; with the overlap off the same Iterate reports it."""
+    import sysu_cfd_heat_amd as P
+    from sysu_cfd_heat_amd import cases
+    print("[forms] solid failure", flush=True)
+    raw, p = cases.channel(nx=10, ny=4, nz=4, max_step=100)
+    p.solid = P.solve_cfg(P.CFD_CG, P.CFD_PC_ILU, 1.0, 1e-30, 2)
+    for overlap in (1, 0):
+        s = P.NSSolver(raw, p)
+        s.set_option("overlap_solid", overlap)
+        if overlap:
+            s.call("Iterate")  # the solid solve is still running (or failed) beside
+            with pytest.raises(RuntimeError, match="solid energy cannot converge"):
+                s.call("Iterate")
+        else:
+            with pytest.raises(RuntimeError, match="solid energy cannot converge"):
+                s.call("Iterate")
+        s.close()

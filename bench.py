@@ -43,10 +43,12 @@ def parse():
     ap.add_argument("--learn-nx", type=int, default=32,
                     help="training mesh nx (ny, nz = nx/2) for the learned guess")
     ap.add_argument("--workload", default="channel",
-                    choices=["channel", "cavity", "batch", "laspack"],
+                    choices=["channel", "cavity", "batch", "laspack", "poisson"],
                     help="channel: the headline CHT channel (default); cavity: the 32^3 "
                          "Re=100 lid-driven cavity; batch: 64 uneven cavity/channel cases "
-                         "packed per launch; laspack: the LASPack-compatible boundary called "
+                         "packed per launch; poisson: the pressure system alone at 7.9M tets, "
+                         "mixed precision vs FP64 to 1e-10; "
+                         "laspack: the LASPack-compatible boundary called "
                          "as the reference calls it, host matrix and vectors crossing PCIe "
                          "(extra lines, not the headline)")
     ap.add_argument("--solvers", default="config", choices=["config", "reference"],
@@ -245,6 +247,59 @@ def run_laspack(a):
           {"boundary": r})
 
 
+def run_poisson(a):
+    """The pressure-correction system alone, 160x128x64 hexahedra x 6 jittered
+    tetrahedra (7,864,320 cells), right-hand side the mass imbalance of a
+    seeded 8-mode Fourier velocity field: CG+ILU(0) to a 1e-10 relative FP64
+    residual, FP64 against mixed precision (FP32 inner CG, FP64 refinement).
+    A step is one pressure solve."""
+    import numpy as np
+    from sysu_cfd_heat_amd import NSSolver, cases
+    t0 = time.time()
+    raw, p = cases.poisson(nx=a.nx, ny=a.ny, nz=a.nz)
+    res = {}
+    sols = {}
+    for prec in (0, 1):
+        p.pres_precision = prec
+        s = NSSolver(raw, p)
+        if prec == 0:
+            print(f"[bench] poisson setup {time.time() - t0:.1f}s, {raw.counts[1]} cells",
+                  file=sys.stderr, flush=True)
+        n = s.counts()[1]
+        from sysu_cfd_heat_amd import Mesh
+        xc = Mesh(raw, p, 0).get("cell_x")
+        u, v, w = cases.fourier_velocity(xc, seed=1)
+        for nm, val in (("Un", u), ("Vn", v), ("Wn", w), ("Apr", np.ones(n))):
+            s.set(nm, val)
+        times, iters = [], []
+        for k in range(a.warmup + a.steps):
+            s.set("Pn", np.zeros(n))
+            s.sync()
+            t1 = time.perf_counter()
+            s.call("CalculatePressure")
+            s.sync()
+            if k >= a.warmup:
+                times.append(time.perf_counter() - t1)
+                iters.append(s.last_solve("p"))
+        x = s.get("xsol")
+        sols[prec] = x - x.mean()
+        res["fp64" if prec == 0 else "mixed"] = {
+            "ms_per_solve": 1000.0 * sum(times) / len(times),
+            "iterations": iters[-1][0], "true_relres": iters[-1][1]}
+        print(f"[bench] poisson precision {prec}: {res}", file=sys.stderr, flush=True)
+        s.close()
+    err = float(np.linalg.norm(sols[1] - sols[0]) / np.linalg.norm(sols[0]))
+    ms = res["mixed"]["ms_per_solve"]
+    _line("pressure solves/s at 7.9M tets, mixed precision to a 1e-10 FP64 residual",
+          1000.0 / ms, "solves/s", a, ms,
+          f"pressure system of {a.nx}x{a.ny}x{a.nz} hex x6 jittered tets ({raw.counts[1]} cells), "
+          f"8-mode Fourier velocity divergence, CG+ILU(0) to 1e-10",
+          {"dtype": "f32 inner Krylov, f64 residual and refinement",
+           "fp64": res["fp64"], "mixed": res["mixed"],
+           "mixed_over_fp64_speed": res["fp64"]["ms_per_solve"] / ms,
+           "rel_l2_mixed_vs_fp64_solution": err})
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -261,6 +316,10 @@ def main():
         return run_batch(a)
     if a.workload == "laspack":
         return run_laspack(a)
+    if a.workload == "poisson":
+        if (a.nx, a.ny, a.nz) == (128, 64, 64):  # the default mesh is the channel's
+            a.nx, a.ny, a.nz = 160, 128, 64
+        return run_poisson(a)
     import numpy as np
     from sysu_cfd_heat_amd import NSSolver, cases
 

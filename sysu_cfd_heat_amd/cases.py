@@ -100,6 +100,37 @@ def channel(nx=128, ny=64, nz=64, tets=True, jitter=0.1, seed=1, re=1e4, turbule
     return raw, p
 
 
+def poisson(nx=160, ny=128, nz=64, jitter=0.1, seed=1, tol=1e-10):
+    """The pressure-correction system alone on the channel's jittered-tet
+    generator (nx*ny*nz hexahedra x 6 tetrahedra over [0,2]x[0,1]x[0,1]),
+    closed by walls: with unit density and Apr = 1 the matrix is the
+    negative semi-definite Laplacian with face weights vol_f * area^2 / (d.n),
+    its right-hand side the mass imbalance of the velocity set in the cells
+    (`fourier_velocity`), projected to zero mean. CG+ILU(0) to `tol`."""
+    raw = box_mesh(nx, ny, nz, 2.0, 1.0, 1.0, tets=True, jitter=jitter, seed=seed,
+                   side_rid=(WALL,) * 6, fluid_rid=5)
+    p = default_params()
+    add_region(p, WALL, 1, 1, 0.0)
+    add_region(p, 5, 5, 0, 0.0, (0.0, 0.0, 0.0, 1.0, 0.0, 1e-3))
+    p.max_step = 2
+    config_solvers(p, tol)
+    p.pres = solve_cfg(CFD_CG, CFD_PC_ILU, 1.0, tol, 20000)
+    return raw, p
+
+
+def fourier_velocity(x, seed=1, modes=8):
+    """u(x) = sum over m of a_m sin(2 pi k_m . x + phi_m), a_m ~ N(0, 1)^3,
+    k_m in {1..4}^3, phi_m ~ U[0, 2 pi): a smooth seeded velocity field at the
+    points x (n x 3); returns u, v, w."""
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((modes, 3))
+    k = rng.integers(1, 5, size=(modes, 3))
+    ph = rng.uniform(0.0, 2.0 * np.pi, modes)
+    arg = 2.0 * np.pi * (x @ k.T) + ph          # n x modes
+    u = np.sin(arg) @ a                         # n x 3
+    return u[:, 0].copy(), u[:, 1].copy(), u[:, 2].copy()
+
+
 def perfect_gas(p, t0=1.0, t_wall=None):
     """Switch a case to DensityModel 1 (rho = p / (Rcpcv T), Rcpcv = cp - cp/1.4):
     gas and inflow at temperature t0, PressureReference = Rcpcv t0 so that the

@@ -60,6 +60,8 @@ def parse():
                          "inner CG, FP64 residual and update)")
     ap.add_argument("--mixed-inner-tol", type=float, default=1e-5,
                     help="mixed-precision pressure: reduction of each FP32 inner CG")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="launch-shape autotuner of the assembly and gradient kernels")
     ap.add_argument("--graphs", type=int, default=1,
                     help="replay pressure CG iterations from a captured HIP graph (cavity and "
                          "batch workloads; the channel run times its SpMV launches with events, "
@@ -329,6 +331,7 @@ def main():
     p.pres_precision = a.pres_precision
     gpu = NSSolver(raw, p)
     gpu.set_option("mixed_inner_tol", a.mixed_inner_tol)
+    gpu.set_option("autotune", a.autotune)
     setup_s = time.time() - t0
     print(f"[bench] setup {setup_s:.1f}s cells={raw.counts[1]} fluid={gpu.counts(0)[1]} "
           f"solid={gpu.counts(1)[1]}", file=sys.stderr, flush=True)
@@ -407,6 +410,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roof, "cpu_baseline": cpu, "krylov": krylov, "p_iters_learned": learned,
         "device_s_per_step": st[1] / max(st[0], 1),
+        "autotune": __import__("sysu_cfd_heat_amd").autotune_choices(),
     }
     print(json.dumps(line))
 

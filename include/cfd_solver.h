@@ -256,9 +256,15 @@ int cfd_ns_profile_read(const cfd_ns *h, double out[8]);
  *   the solver waits for it first, "Synchronize" waits for all device work)
  *   "mixed_inner_tol" in (0, 1): relative reduction of each FP32 inner CG of
  *   the mixed-precision pressure solve (default 1e-5)
+ *   "autotune" 0/1 (process-wide): the first launch of each assembly and
+ *   gradient kernel at a given size times workgroups of 64..1024 threads and
+ *   keeps the fastest (default 1; results do not depend on it)
  * cfd_ns_get_option also reads "graph_launches" (CG iterations replayed). */
 int cfd_ns_set_option(cfd_ns *h, const char *name, double value);
 double cfd_ns_get_option(const cfd_ns *h, const char *name);
+/* the autotuner's choices so far, "kernel@size=threads,..." into buf (len
+ * bytes incl. the terminator); returns the full length */
+int cfd_autotune_choices(char *buf, size_t len);
 
 /* Field output in the reference's Tecplot layout (NavierStokesSolver::Output
  * and Output2Tecplot, src/navier.cpp:264-340; HeatConductionSolver::

@@ -92,6 +92,7 @@ def lib() -> C.CDLL:
     L.cfd_ns_case_range.argtypes = [vp, C.c_int, ip]
     L.cfd_ns_write_tecplot.argtypes = [vp, C.c_char_p]
     L.cfd_ns_set_option.argtypes = [vp, C.c_char_p, C.c_double]
+    L.cfd_autotune_choices.argtypes = [C.c_char_p, C.c_size_t]
     L.cfd_ns_get_option.argtypes = [vp, C.c_char_p]
     L.cfd_ns_get_option.restype = C.c_double
     L.cfd_ns_free.argtypes = [vp]
@@ -257,6 +258,19 @@ class Mesh:
 
 
 # ---------------------------------------------------------------- params --
+def autotune_choices() -> dict:
+    """Workgroup sizes the launch-shape autotuner chose: {"kernel@size": threads}."""
+    L = lib()
+    buf = C.create_string_buffer(1 << 16)
+    L.cfd_autotune_choices(buf, len(buf))
+    out = {}
+    for item in buf.value.decode().split(","):
+        if item:
+            k, v = item.rsplit("=", 1)
+            out[k] = int(v)
+    return out
+
+
 def default_params() -> ParamsC:
     p = ParamsC()
     lib().cfd_params_default(C.byref(p))

@@ -1,5 +1,9 @@
 // Device face/cell loops of the SIMPLE step (kernels in assembly.hip).
 #pragma once
+#include <string>
+#include <utility>
+#include <vector>
+
 #include "common.h"
 #include "krylov.h"
 
@@ -155,6 +159,12 @@ void launch_fluid2solid(const DevMesh &fm, const DevMesh &sm, const double *gTf,
 void launch_solid_bc_temperature(const DevMesh &m, const double *T, const double *D,
                                  const double *q, double *BT, hipStream_t s);
 void launch_fill(double *p, size_t n, double v, hipStream_t s);
+// Launch-shape autotuner of the assembly and gradient kernels (process-wide,
+// default on): the first launch of a kernel at a given size times workgroups
+// of 64..1024 threads and keeps the fastest; results do not depend on it.
+void set_autotune(bool on);
+bool get_autotune();
+std::vector<std::pair<std::string, int>> autotune_choices();  // "kernel@size" -> threads
 // b -= mean(b): consistency projection of a pure-Neumann right-hand side
 void launch_remove_mean(double *b, int n, Red red, hipStream_t s);
 // batched cases: the mean of b is removed from each case's cells separately

@@ -2,6 +2,12 @@
 // boundary face. Every loop is a gather (no atomics): a cell visits its own
 // faces, so FP64 sums keep the reference's order and are bit-identical to it
 // when compiled with -ffp-contract=off (global reductions excepted).
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
 #include "assembly.h"
 
 namespace cfd {
@@ -986,6 +992,74 @@ __global__ void k_scale_source(int C, const double *__restrict__ vol, double q,
 }  // namespace
 
 // ------------------------------------------------------------ launchers --
+// Launch-shape autotuner for the assembly and gradient kernels. They compute
+// every cell or face on its own from their inputs and overwrite their
+// outputs (no reductions, no accumulation in place), so a workgroup size
+// changes nothing in the results and a kernel may be run again: the first
+// launch of a kernel at a given size times each candidate on its stream and
+// the fastest is kept for that kernel and size.
+namespace {
+struct ShapeTuner {
+    std::mutex mu;
+    std::map<std::pair<std::string, long long>, int> best;
+};
+ShapeTuner &tuner() {
+    static ShapeTuner t;
+    return t;
+}
+std::atomic<bool> g_autotune{true};
+
+template <class L>
+void tuned_launch(const char *name, long long n, hipStream_t s, L launch) {
+    if (!g_autotune.load() || n < 4096) {
+        launch(kBlock);
+        return;
+    }
+    const std::pair<std::string, long long> key{name, n};
+    {
+        std::lock_guard<std::mutex> g(tuner().mu);
+        auto it = tuner().best.find(key);
+        if (it != tuner().best.end()) {
+            launch(it->second);
+            return;
+        }
+    }
+    static const int cand[] = {64, 128, 256, 512, 1024};
+    hipEvent_t e0, e1;
+    CFD_HIP(hipEventCreate(&e0));
+    CFD_HIP(hipEventCreate(&e1));
+    int bestb = kBlock;
+    float bestms = 1e30f;
+    for (int rep = 0; rep < 2; ++rep)  // the first round warms caches and code
+        for (int b : cand) {
+            CFD_HIP(hipEventRecord(e0, s));
+            launch(b);
+            CFD_HIP(hipEventRecord(e1, s));
+            CFD_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            CFD_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (rep == 1 && ms < bestms) {
+                bestms = ms;
+                bestb = b;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> g(tuner().mu);
+    tuner().best[key] = bestb;
+}
+}  // namespace
+
+void set_autotune(bool on) { g_autotune.store(on); }
+bool get_autotune() { return g_autotune.load(); }
+std::vector<std::pair<std::string, int>> autotune_choices() {
+    std::lock_guard<std::mutex> g(tuner().mu);
+    std::vector<std::pair<std::string, int>> out;
+    for (auto &e : tuner().best)
+        out.push_back({e.first.first + "@" + std::to_string(e.first.second), e.second});
+    return out;
+}
+
 void launch_fill(double *p, size_t n, double v, hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill, dim3(grid_for((long long)std::min<size_t>(n, 1u << 24))),
@@ -1005,10 +1079,26 @@ void launch_gradient(const DevMesh &m, int nf, const double *const *phi,
     }
     const int g = grid_for(m.ncel);
     switch (nf) {
-    case 1: hipLaunchKernelGGL(k_gradient<1>, dim3(g), dim3(kBlock), 0, s, m, h); break;
-    case 2: hipLaunchKernelGGL(k_gradient<2>, dim3(g), dim3(kBlock), 0, s, m, h); break;
-    case 3: hipLaunchKernelGGL(k_gradient<3>, dim3(g), dim3(kBlock), 0, s, m, h); break;
-    default: hipLaunchKernelGGL(k_gradient<4>, dim3(g), dim3(kBlock), 0, s, m, h); break;
+    case 1:
+        tuned_launch("k_gradient<1>", m.ncel, s, [&](int b) {
+            hipLaunchKernelGGL(k_gradient<1>, dim3(grid_for(m.ncel, b)), dim3(b), 0, s, m, h);
+        });
+        break;
+    case 2:
+        tuned_launch("k_gradient<2>", m.ncel, s, [&](int b) {
+            hipLaunchKernelGGL(k_gradient<2>, dim3(grid_for(m.ncel, b)), dim3(b), 0, s, m, h);
+        });
+        break;
+    case 3:
+        tuned_launch("k_gradient<3>", m.ncel, s, [&](int b) {
+            hipLaunchKernelGGL(k_gradient<3>, dim3(grid_for(m.ncel, b)), dim3(b), 0, s, m, h);
+        });
+        break;
+    default:
+        tuned_launch("k_gradient<4>", m.ncel, s, [&](int b) {
+            hipLaunchKernelGGL(k_gradient<4>, dim3(grid_for(m.ncel, b)), dim3(b), 0, s, m, h);
+        });
+        break;
     }
     CFD_LAUNCH_CHECK();
     if (limiter == 0) return;
@@ -1060,7 +1150,10 @@ void launch_bc_temperature(const DevMesh &m, const double *T, double *BT, hipStr
 }
 
 void launch_build_velocity(const DevMesh &m, const VelArgs &a, int *errflag, hipStream_t s) {
-    hipLaunchKernelGGL(k_build_velocity, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m, a, errflag);
+    tuned_launch("k_build_velocity", m.ncel, s, [&](int b) {
+        hipLaunchKernelGGL(k_build_velocity, dim3(grid_for(m.ncel, b)), dim3(b), 0, s, m, a,
+                           errflag);
+    });
     CFD_LAUNCH_CHECK();
 }
 
@@ -1068,16 +1161,20 @@ void launch_rhie_chow(const DevMesh &m, const double *R, const double *U, const 
                       const double *W, const double *P, const double *gP, const double *Apr,
                       const double *BR, const double *BU, const double *BV, const double *BW,
                       double *RUFace, hipStream_t s) {
-    hipLaunchKernelGGL(k_rhie_chow, dim3(grid_for(m.nfac)), dim3(kBlock), 0, s, m, R, U, V, W, P,
-                       gP, Apr, BR, BU, BV, BW, RUFace);
+    tuned_launch("k_rhie_chow", m.nfac, s, [&](int b) {
+        hipLaunchKernelGGL(k_rhie_chow, dim3(grid_for(m.nfac, b)), dim3(b), 0, s, m, R, U, V, W,
+                           P, gP, Apr, BR, BU, BV, BW, RUFace);
+    });
     CFD_LAUNCH_CHECK();
 }
 
 void launch_build_pressure(const DevMesh &m, const double *R, const double *Apr,
                            const double *RUFace, double *diag, double *val, double *bp,
                            hipStream_t s) {
-    hipLaunchKernelGGL(k_build_pressure, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m, R, Apr,
-                       RUFace, diag, val, bp);
+    tuned_launch("k_build_pressure", m.ncel, s, [&](int b) {
+        hipLaunchKernelGGL(k_build_pressure, dim3(grid_for(m.ncel, b)), dim3(b), 0, s, m, R, Apr,
+                           RUFace, diag, val, bp);
+    });
     CFD_LAUNCH_CHECK();
 }
 
@@ -1085,8 +1182,10 @@ void launch_build_pressure_gas(const DevMesh &m, const double *R, const double *
                                const double *RUFace, const double *T, const double *BR,
                                double rcpcv, double dt, double *diag, double *val, double *bp,
                                hipStream_t s) {
-    hipLaunchKernelGGL(k_build_pressure_gas, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m, R,
-                       Apr, RUFace, T, BR, rcpcv, dt, diag, val, bp);
+    tuned_launch("k_build_pressure_gas", m.ncel, s, [&](int b) {
+        hipLaunchKernelGGL(k_build_pressure_gas, dim3(grid_for(m.ncel, b)), dim3(b), 0, s, m, R,
+                           Apr, RUFace, T, BR, rcpcv, dt, diag, val, bp);
+    });
     CFD_LAUNCH_CHECK();
 }
 
@@ -1125,7 +1224,10 @@ void launch_take_solution(const DevMesh &m, const double *x, double *fld, double
 }
 
 void launch_build_scalar(const DevMesh &m, const ScaArgs &a, int *errflag, hipStream_t s) {
-    hipLaunchKernelGGL(k_build_scalar, dim3(grid_for(m.ncel)), dim3(kBlock), 0, s, m, a, errflag);
+    tuned_launch(a.mode == 2 ? "k_build_scalar/solid" : "k_build_scalar", m.ncel, s, [&](int b) {
+        hipLaunchKernelGGL(k_build_scalar, dim3(grid_for(m.ncel, b)), dim3(b), 0, s, m, a,
+                           errflag);
+    });
     CFD_LAUNCH_CHECK();
 }
 

@@ -1162,6 +1162,10 @@ int cfd_ns_set_option(cfd_ns *h, const char *name, double value) {
         h->ns.overlap_solid = value != 0.;
         return 0;
     }
+    if (k == "autotune") {
+        set_autotune(value != 0.);
+        return 0;
+    }
     if (k == "mixed_inner_tol") {
         if (!(value > 0. && value < 1.)) {
             last_error() = "mixed_inner_tol must lie in (0, 1)";
@@ -1180,9 +1184,21 @@ double cfd_ns_get_option(const cfd_ns *h, const char *name) {
     if (k == "graphs") return h->ns.kry->use_graphs ? 1. : 0.;
     if (k == "overlap_solid") return h->ns.overlap_solid ? 1. : 0.;
     if (k == "mixed_inner_tol") return h->ns.mixed_inner_tol;
+    if (k == "autotune") return get_autotune() ? 1. : 0.;
     if (k == "graph_launches") return (double)h->ns.kry->graph_launches;
     last_error() = "unknown option " + k;
     return std::nan("");
+}
+
+int cfd_autotune_choices(char *buf, size_t len) {
+    std::string out;
+    for (auto &c : autotune_choices())
+        out += (out.empty() ? "" : ",") + c.first + "=" + std::to_string(c.second);
+    if (buf && len) {
+        std::strncpy(buf, out.c_str(), len - 1);
+        buf[len - 1] = 0;
+    }
+    return (int)out.size();
 }
 
 int cfd_ns_write_tecplot(cfd_ns *h, const char *path) {

@@ -1000,6 +1000,10 @@ Krylov::~Krylov() {
     for (auto &g : graphs_) (void)hipGraphExecDestroy(g.exec);
 }
 
+// iterations per long graph of a Krylov loop (a batch of b iterations is
+// b / kGraphIters launches of it and the rest of the one-iteration graph)
+constexpr int kGraphIters = 8;
+
 // Executable graph of the launches rec() makes on this stream, captured on
 // first use of key (every pointer and size the launches bake in).
 hipGraphExec_t Krylov::graph_for(const std::vector<uintptr_t> &key,
@@ -1526,7 +1530,6 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
     };
     // graphs of one and of kGraphIters iterations: a batch of b iterations is
     // b / kGraphIters launches of the long one and the rest of the short one
-    constexpr int kGraphIters = 8;
     hipGraphExec_t gexec = nullptr, gexec_long = nullptr;
     if (use_graphs && !prof && pc != CFD_PC_NONE) {
         std::vector<uintptr_t> key = {
@@ -1625,30 +1628,64 @@ SolveStats Krylov::bicgstab(const Ell &A, const double *b, double *x, const cfd_
     CFD_LAUNCH_CHECK();
     int batch = check_first, issued = 0, prev_it = 0;
     double prev_log = 1e300;
+    // one iteration's launches (the same every iteration)
+    auto iteration = [&]() {
+        if (jac) {
+            hipLaunchKernelGGL(k_bi_p<true>, dim3(g), dim3(kBlock), 0, st_, n, wd_.r.p, wd_.p0.p,
+                               v_.p, ph_.p, A.diag, state_.p);
+        } else {
+            hipLaunchKernelGGL(k_bi_p<false>, dim3(g), dim3(kBlock), 0, st_, n, wd_.r.p, wd_.p0.p,
+                               v_.p, ph_.p, A.diag, state_.p);
+            apply_pc(A, pc, cfg.omega, (const double *)wd_.p0.p, ph_.p, col, wd_, nullptr,
+                     state_.p);
+        }
+        CFD_REDUCE_LAUNCH(k_bi_spmv<0>, Gs, st_, A, ph_.p, v_.p, rt_.p,
+                           state_.p, part_.p, fin);
+        if (jac) {
+            CFD_REDUCE_LAUNCH(k_bi_s<true>, G, st_, n, wd_.r.p, v_.p,
+                               s_.p, sh_.p, A.diag, state_.p, part_.p, fin);
+        } else {
+            CFD_REDUCE_LAUNCH(k_bi_s<false>, G, st_, n, wd_.r.p, v_.p,
+                               s_.p, sh_.p, A.diag, state_.p, part_.p, fin);
+            apply_pc(A, pc, cfg.omega, (const double *)s_.p, sh_.p, col, wd_, nullptr, state_.p);
+        }
+        CFD_REDUCE_LAUNCH(k_bi_spmv<1>, Gs, st_, A, sh_.p, t_.p, s_.p,
+                           state_.p, part_.p, fin);
+        CFD_REDUCE_LAUNCH(k_bi_x, G, st_, n, x, wd_.r.p, ph_.p, sh_.p,
+                           s_.p, t_.p, rt_.p, state_.p, part_.p, fin);
+    };
+    // graphs of one and of kGraphIters iterations, as for CG
+    hipGraphExec_t gexec = nullptr, gexec_long = nullptr;
+    if (use_graphs && pc != CFD_PC_NONE) {
+        std::vector<uintptr_t> key = {
+            2, (uintptr_t)A.diag, (uintptr_t)A.val, (uintptr_t)A.col, (uintptr_t)x,
+            (uintptr_t)wd_.r.p, (uintptr_t)wd_.p0.p, (uintptr_t)wd_.tmp.p, (uintptr_t)wd_.ud.p,
+            (uintptr_t)wd_.lu.p, (uintptr_t)v_.p, (uintptr_t)ph_.p, (uintptr_t)s_.p,
+            (uintptr_t)sh_.p, (uintptr_t)t_.p, (uintptr_t)rt_.p, (uintptr_t)n, (uintptr_t)A.K,
+            (uintptr_t)pc, (uintptr_t)col, (uintptr_t)(col ? col->ncolour : 0),
+            (uintptr_t)(col ? col->triangle_free : 0),
+            (uintptr_t)(col ? (const void *)col->nlow.p : nullptr)};
+        uint64_t om;
+        std::memcpy(&om, &cfg.omega, sizeof om);
+        key.push_back((uintptr_t)om);
+        if (col)
+            for (int k = 0; k <= col->ncolour; ++k) key.push_back((uintptr_t)col->off[k]);
+        key.push_back(1);
+        gexec = graph_for(key, [&]() { iteration(); });
+        key.back() = kGraphIters;
+        gexec_long = graph_for(key, [&]() {
+            for (int k = 0; k < kGraphIters; ++k) iteration();
+        });
+    }
     for (;;) {
-        for (int k = 0; k < batch; ++k) {
-            if (jac) {
-                hipLaunchKernelGGL(k_bi_p<true>, dim3(g), dim3(kBlock), 0, st_, n, wd_.r.p, wd_.p0.p,
-                                   v_.p, ph_.p, A.diag, state_.p);
-            } else {
-                hipLaunchKernelGGL(k_bi_p<false>, dim3(g), dim3(kBlock), 0, st_, n, wd_.r.p, wd_.p0.p,
-                                   v_.p, ph_.p, A.diag, state_.p);
-                apply_pc(A, pc, cfg.omega, (const double *)wd_.p0.p, ph_.p, col, wd_);
-            }
-            CFD_REDUCE_LAUNCH(k_bi_spmv<0>, Gs, st_, A, ph_.p, v_.p, rt_.p,
-                               state_.p, part_.p, fin);
-            if (jac) {
-                CFD_REDUCE_LAUNCH(k_bi_s<true>, G, st_, n, wd_.r.p, v_.p,
-                                   s_.p, sh_.p, A.diag, state_.p, part_.p, fin);
-            } else {
-                CFD_REDUCE_LAUNCH(k_bi_s<false>, G, st_, n, wd_.r.p, v_.p,
-                                   s_.p, sh_.p, A.diag, state_.p, part_.p, fin);
-                apply_pc(A, pc, cfg.omega, (const double *)s_.p, sh_.p, col, wd_);
-            }
-            CFD_REDUCE_LAUNCH(k_bi_spmv<1>, Gs, st_, A, sh_.p, t_.p, s_.p,
-                               state_.p, part_.p, fin);
-            CFD_REDUCE_LAUNCH(k_bi_x, G, st_, n, x, wd_.r.p, ph_.p, sh_.p,
-                               s_.p, t_.p, rt_.p, state_.p, part_.p, fin);
+        if (gexec) {
+            int k = 0;
+            for (; k + kGraphIters <= batch; k += kGraphIters)
+                CFD_HIP(hipGraphLaunch(gexec_long, st_));
+            for (; k < batch; ++k) CFD_HIP(hipGraphLaunch(gexec, st_));
+            graph_launches += batch;
+        } else {
+            for (int k = 0; k < batch; ++k) iteration();
         }
         CFD_LAUNCH_CHECK();
         issued += batch;

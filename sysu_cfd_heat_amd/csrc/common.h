@@ -76,6 +76,7 @@ struct DBuf {
 };
 
 constexpr int kBlock = 256;          // 4 wave64 per workgroup
+constexpr int kFinBlock = 1024;      // finishing workgroup of the Krylov reductions
 constexpr int kRedGrid = 2048;       // fixed grid of reducing kernels: 8 WG/CU
 inline int grid_for(long long n, int block = kBlock) {
     long long g = (n + block - 1) / block;
@@ -96,11 +97,12 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Block-wide sum of NV values (blockDim == kBlock); result valid in thread 0.
+// Block-wide sum of NV values (blockDim a multiple of 64, at most
+// kFinBlock); result valid in thread 0. The wave sums are added in wave order.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV]) {
-    __shared__ double red[NV][kBlock / 64];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __shared__ double red[NV][kFinBlock / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
     if (lane == 0)
@@ -111,7 +113,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
             double s = 0.;
-            for (int w = 0; w < kBlock / 64; ++w) s += red[k][w];
+            for (int w = 0; w < nw; ++w) s += red[k][w];
             v[k] = s;
         }
 }
@@ -194,6 +196,9 @@ constexpr int kPartSize = 12 * 8192;
 // preconditioner writes those of rho = r.z above it (one finishing launch
 // reads both)
 constexpr int kPartRho = 2 * kRedGrid;
+// CG: the sparse product's partials of p.q sit at the top of the buffer,
+// where the update's workgroups read them while writing their own below
+constexpr int kPartPQ = kPartSize - kSpmvGrid;
 
 // Grid-stride loop of a reducing kernel in XCD-aware workgroup order.
 #define RED_LOOP_X(i, n)                                                               \
@@ -202,13 +207,21 @@ constexpr int kPartRho = 2 * kRedGrid;
 
 // Launch a reducing kernel: producer on G workgroups, then the finishing
 // workgroup. The argument list names `fin` where the kernel takes it.
-#define CFD_REDUCE_LAUNCH(kern, G, s, ...)                                     \
+#define CFD_REDUCE_LAUNCH_FB(kern, G, FB, s, ...)                              \
     do {                                                                       \
         const int g_ = (G);                                                    \
         int fin = 0;                                                           \
         hipLaunchKernelGGL(kern, dim3(g_), dim3(::cfd::kBlock), 0, s, __VA_ARGS__); \
         fin = g_;                                                              \
-        hipLaunchKernelGGL(kern, dim3(1), dim3(::cfd::kBlock), 0, s, __VA_ARGS__); \
+        hipLaunchKernelGGL(kern, dim3(1), dim3(FB), 0, s, __VA_ARGS__);        \
     } while (0)
+#define CFD_REDUCE_LAUNCH(kern, G, s, ...) \
+    CFD_REDUCE_LAUNCH_FB(kern, G, ::cfd::kBlock, s, __VA_ARGS__)
+// Krylov iterations: above kRedGrid partials a 1024-thread finishing
+// workgroup (four times the loads in flight) is faster; below, the 256-thread
+// one starts sooner
+inline int fin_block(long long cnt) { return cnt > kRedGrid ? kFinBlock : kBlock; }
+#define CFD_REDUCE_LAUNCH_K(kern, G, s, ...) \
+    CFD_REDUCE_LAUNCH_FB(kern, G, ::cfd::fin_block(G), s, __VA_ARGS__)
 
 }  // namespace cfd

@@ -164,7 +164,8 @@ __global__ void k_add_f(int n, double *__restrict__ x, const float *__restrict__
 }
 
 // ------------------------------------------------------------------- CG --
-// p = z + beta p ; q = A z + beta q ; partial p.q ; alpha = rho / p.q.
+// p = z + beta p ; q = A z + beta q ; partials of p.q (k_cg_update finishes
+// the sum and alpha = rho / p.q).
 // q = A p in exact arithmetic (A p = A z + beta A p_old), so the sparse
 // product gathers one vector (z) instead of two; p and q are updated in
 // place (each row reads and writes only its own entries). The drift of the
@@ -172,52 +173,75 @@ __global__ void k_add_f(int n, double *__restrict__ x, const float *__restrict__
 // product, orders below the stopping tolerance; the returned accuracy is the
 // true residual b - A x recomputed at the end. beta = 0 on the first
 // iteration, where p and q start at zero.
+// The previous iteration's x += alpha p is applied here, where the old p is
+// read anyway (alpha still holds that iteration's value; none on the first
+// iteration); k_cg_xlast applies the last one after the loop.
 // Each thread takes its rows two at a time (i and i + one grid stride) so
 // that two rows' gathers are in flight together; the rows a workgroup covers
 // are those of the plain grid-stride loop.
 template <class T>
 __global__ void k_cg_pq(EllT<T> A, const T *__restrict__ z, T *__restrict__ p, T *__restrict__ q,
-                        KState *st, double *part, int fin) {
+                        T *__restrict__ x, const KState *st, double *part) {
     if (st->done) return;
-    const T beta = (T)st->beta;
+    const T beta = (T)st->beta, alpha = (T)st->alpha;
+    const bool upd = st->iter > 0;
     const int n = A.n, stride = gridDim.x * blockDim.x;
     double acc[1] = {0.};
-    for (int i = fin ? n : xcd_block() * blockDim.x + threadIdx.x; i < n; i += 2 * stride) {
+    for (int i = xcd_block() * blockDim.x + threadIdx.x; i < n; i += 2 * stride) {
         const int j = i + stride;
         const bool hj = j < n;
         const T zi = z[i], zj = hj ? z[j] : T(0);
         T si = row_sum(A, i, z, A.diag[i] * zi);
         T sj = hj ? row_sum(A, j, z, A.diag[j] * zj) : T(0);
-        const T pi = zi + beta * p[i];
+        const T poi = p[i];
+        if (upd) x[i] += alpha * poi;
+        const T pi = zi + beta * poi;
         si += beta * q[i];
         p[i] = pi;
         q[i] = si;
         acc[0] += (double)pi * si;
         if (hj) {
-            const T pj = zj + beta * p[j];
+            const T poj = p[j];
+            if (upd) x[j] += alpha * poj;
+            const T pj = zj + beta * poj;
             sj += beta * q[j];
             p[j] = pj;
             q[j] = sj;
             acc[0] += (double)pj * sj;
         }
     }
-    if (grid_reduce<1>(acc, part, fin)) {
-        st->pq = acc[0];
-        st->alpha = st->rho / acc[0];
-    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
 }
 
-// x += alpha p ; r -= alpha q ; partial ||r||^2. Jacobi: z = r / d and the
-// partial rho = r.z, then beta = rho / rho_old.
+// r -= alpha q ; partial ||r||^2 (x += alpha p is deferred to the next
+// k_cg_pq). Jacobi: z = r / d and the partial rho = r.z, then
+// beta = rho / rho_old.
+// alpha = rho / p.q: every producer workgroup sums the sparse product's npq
+// partials itself (same order in each, so the same alpha everywhere; the
+// first workgroup records it), which saves a finishing launch. The
+// finishing launch (fin > 0) takes alpha from the state.
 template <bool JAC, class T>
-__global__ void k_cg_update(int n, T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
-                            const T *__restrict__ q, const T *__restrict__ d, T *__restrict__ z,
-                            KState *st, double *part, int fin) {
+__global__ void k_cg_update(int n, T *__restrict__ r, const T *__restrict__ q,
+                            const T *__restrict__ d, T *__restrict__ z, KState *st, double *part,
+                            const double *pqpart, int npq, int fin) {
     if (st->done) return;
-    const T alpha = (T)st->alpha;
+    __shared__ double s_alpha;
+    if (!fin) {
+        double v[1] = {fin_share(pqpart, npq)};
+        block_sum<1>(v);
+        if (threadIdx.x == 0) {
+            s_alpha = st->rho / v[0];
+            if (blockIdx.x == 0) {
+                st->pq = v[0];
+                st->alpha = s_alpha;
+            }
+        }
+        __syncthreads();
+    }
+    const T alpha = fin ? T(0) : (T)s_alpha;
     double acc[2] = {0., 0.};
     RED_LOOP(i, n) {
-        x[i] += alpha * p[i];
         T ri = r[i] - alpha * q[i];
         r[i] = ri;
         acc[0] += (double)ri * ri;
@@ -238,6 +262,16 @@ __global__ void k_cg_update(int n, T *__restrict__ x, T *__restrict__ r, const T
         }
         st->done = rtc(st->rnorm, st->bnorm, st->eps) || st->iter >= st->maxit;
     }
+}
+
+// After the iterations: the last completed iteration's x += alpha p, which
+// no k_cg_pq applied (the solve stopped after its update).
+template <class T>
+__global__ void k_cg_xlast(int n, T *__restrict__ x, const T *__restrict__ p,
+                           const KState *st) {
+    if (st->iter == 0) return;
+    const T alpha = (T)st->alpha;
+    GRID_LOOP(i, n) x[i] += alpha * p[i];
 }
 
 // The finishing launches of the update and of rho = r.z as one workgroup:
@@ -671,12 +705,15 @@ __global__ void k_ilu_factor(int n, int K, const int *__restrict__ col, int r0, 
 }
 // ILU(0) of a matrix whose graph has no triangles (no two neighbours of a
 // row are neighbours of each other): no fill position of ILU(0) is then in
-// the pattern except the diagonal, so the factor keeps A's off-diagonals
-// (L = A_lower D^-1, U = A_upper) and only the diagonal changes. Same
+// the pattern except the diagonal, so U keeps A's upper off-diagonals, the
+// lower factor is l_pc = a_pc / u_cc, and only the diagonal is updated. Same
 // operations, in the same order, as k_ilu_factor restricted to those updates.
+// lu holds a copy of A's off-diagonals; the l_pc are written over its lower
+// slots, so the forward sweep reads them instead of gathering u_cc.
 template <class T>
 __global__ void k_dilu_factor(int n, int K, const int *__restrict__ col,
-                              const T *__restrict__ val, int r0, int r1, T *__restrict__ ud) {
+                              const T *__restrict__ val, int r0, int r1, T *__restrict__ ud,
+                              T *__restrict__ lu) {
     int p = r0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= r1) return;
     int lc[kIluMaxK], ls[kIluMaxK], nl = 0;
@@ -696,6 +733,7 @@ __global__ void k_dilu_factor(int n, int K, const int *__restrict__ col,
     for (int a = 0; a < nl; ++a) {
         const int c = lc[a];
         const T l = val[(size_t)ls[a] * n + p] / ud[c];
+        lu[(size_t)ls[a] * n + p] = l;
         for (int q = 0; q < K; ++q) {
             const size_t oc = (size_t)q * n + c;
             if (col[oc] == p) {
@@ -715,8 +753,6 @@ __global__ void k_dilu_factor(int n, int K, const int *__restrict__ col,
 // rpart[blockIdx.x]; the launches of one application use consecutive slices.
 // With st, the sweeps do nothing once the solve has stopped (iterations
 // issued past convergence).
-// D (triangle-free graph): lu holds A's off-diagonals; the lower factor
-// entry is formed as a_pc / u_cc, as k_ilu_factor would have stored it.
 // rows per thread of the ILU(0) sweep launches: one keeps two rounds of
 // waves per launch, which overlap better than two rows per thread
 constexpr int kSweepRows = 1;  // 2 measured slower (82.7 vs 78.5 us per application)
@@ -725,12 +761,11 @@ constexpr int kSweepRows = 1;  // 2 measured slower (82.7 vs 78.5 us per applica
 // so that the column loads of every slot, then the gathers, are issued
 // together instead of one slot's chain after the other; KK = 0: any K.
 // The products are summed in slot order either way.
-template <class T, bool D, int KK>
+template <class T, int KK>
 __device__ __forceinline__ T ilu_fwd_row(int n, int K, const int *__restrict__ col,
                                          const unsigned char *__restrict__ nlow,
-                                         const T *__restrict__ lu, const T *__restrict__ ud,
-                                         int p, int c0end, const T *__restrict__ c,
-                                         const T *__restrict__ t) {
+                                         const T *__restrict__ lu, int p, int c0end,
+                                         const T *__restrict__ c, const T *__restrict__ t) {
     T acc = c[p];
     const int nl = nlow[p];
     if (KK > 0) {
@@ -741,8 +776,7 @@ __device__ __forceinline__ T ilu_fwd_row(int n, int K, const int *__restrict__ c
 #pragma unroll
         for (int k = 0; k < KK; ++k) {
             if (k < nl) {
-                const T l = lu[(size_t)k * n + p];
-                a[k] = D ? l / ud[cl[k]] : l;
+                a[k] = lu[(size_t)k * n + p];
                 v[k] = cl[k] < c0end ? c[cl[k]] : t[cl[k]];
             }
         }
@@ -753,7 +787,7 @@ __device__ __forceinline__ T ilu_fwd_row(int n, int K, const int *__restrict__ c
         for (int k = 0; k < nl; ++k) {
             const size_t o = (size_t)k * n + p;
             const int cl = col[o];
-            acc -= (D ? lu[o] / ud[cl] : lu[o]) * (cl < c0end ? c[cl] : t[cl]);
+            acc -= lu[o] * (cl < c0end ? c[cl] : t[cl]);
         }
     }
     return acc;
@@ -795,7 +829,7 @@ __device__ __forceinline__ T ilu_bwd_row(int n, int K, const int *__restrict__ c
 // One sweep launch covers rows [r0, r1) as R interleaved parts of h rows:
 // thread q takes rows r0 + q, r0 + h + q, ... so that R rows' loads are in
 // flight together and the launch needs R times fewer waves.
-template <class T, bool D, int KK, int R>
+template <class T, int KK, int R>
 __global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col,
                           const unsigned char *__restrict__ nlow, const T *__restrict__ lu,
                           const T *__restrict__ ud, int r0, int r1, int h, int c0end,
@@ -808,7 +842,7 @@ __global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col,
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int p = r0 + r * h + q;
-        if (q < h && p < r1) acc[r] = ilu_fwd_row<T, D, KK>(n, K, col, nlow, lu, ud, p, c0end, c, t);
+        if (q < h && p < r1) acc[r] = ilu_fwd_row<T, KK>(n, K, col, nlow, lu, p, c0end, c, t);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -827,7 +861,7 @@ __global__ void k_ilu_fwd(int n, int K, const int *__restrict__ col,
         if (threadIdx.x == 0) rpart[blockIdx.x] = rz[0];
     }
 }
-template <class T, bool D, int KK, int R>
+template <class T, int KK, int R>
 __global__ void k_ilu_bwd(int n, int K, const int *__restrict__ col,
                           const unsigned char *__restrict__ nlow, const T *__restrict__ lu,
                           const T *__restrict__ ud, int r0, int r1, int h, int c0end,
@@ -1193,18 +1227,18 @@ void Krylov::setup_pc(const EllT<T> &A, int pc, Colouring *col, KWork<T> &w) {
     const int n = A.n;
     if (A.K > kIluMaxK) throw HipError("ILU(0): more than 16 off-diagonals per row");
     CFD_HIP(hipMemcpyAsync(w.ud.p, A.diag, sizeof(T) * n, hipMemcpyDeviceToDevice, st_));
-    if (col->triangle_free) {  // only the diagonal changes
+    CFD_HIP(hipMemcpyAsync(w.lu.p, A.val, sizeof(T) * (size_t)A.K * n, hipMemcpyDeviceToDevice,
+                           st_));
+    if (col->triangle_free) {  // the diagonal and the lower factor l = a / u
         for (int c = 0; c < col->ncolour; ++c) {
             int r0 = col->off[c], r1 = col->off[c + 1];
             if (r1 > r0)
                 hipLaunchKernelGGL(k_dilu_factor<T>, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_,
-                                   n, A.K, A.col, (const T *)A.val, r0, r1, w.ud.p);
+                                   n, A.K, A.col, (const T *)A.val, r0, r1, w.ud.p, w.lu.p);
         }
         CFD_LAUNCH_CHECK();
         return;
     }
-    CFD_HIP(hipMemcpyAsync(w.lu.p, A.val, sizeof(T) * (size_t)A.K * n, hipMemcpyDeviceToDevice,
-                           st_));
     for (int c = 0; c < col->ncolour; ++c) {
         int r0 = col->off[c], r1 = col->off[c + 1];
         if (r1 > r0)
@@ -1233,7 +1267,7 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
             const int r0 = col->off[k], r1 = col->off[k + 1];
             if (r1 > r0) nbk += grid8(r1 - r0);  // last forward + every backward colour
         }
-        double *rp = (rz_blocks && nbk <= kPartSize - kPartRho) ? part_.p + kPartRho : nullptr;
+        double *rp = (rz_blocks && nbk <= kPartPQ - kPartRho) ? part_.p + kPartRho : nullptr;
         const T om = (T)omega, sc = (T)(omega * (2. - omega));
         auto sweep = [&](auto kk, bool fwd, int r0, int r1, int g, int last, double *rpp) {
             constexpr int KK = decltype(kk)::value;
@@ -1277,9 +1311,8 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
             const int r0 = col->off[k], r1 = col->off[k + 1];
             if (r1 > r0) nbk += grid8((r1 - r0 + kSweepRows - 1) / kSweepRows);  // last fwd + every bwd
         }
-        double *rp = (rz_blocks && nbk <= kPartSize - kPartRho) ? part_.p + kPartRho : nullptr;
-        const bool dilu = col->triangle_free;  // the factor's off-diagonals are A's
-        const T *lu = dilu ? (const T *)A.val : (const T *)w.lu.p;
+        double *rp = (rz_blocks && nbk <= kPartPQ - kPartRho) ? part_.p + kPartRho : nullptr;
+        const T *lu = w.lu.p;
         int used = 0;
         // one sweep launch for slot count KK (see k_ilu_fwd), kSweepRows rows
         // per thread over h = ceil(rows / kSweepRows)
@@ -1288,25 +1321,14 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
             constexpr int R = kSweepRows;
             const int h = (r1 - r0 + R - 1) / R;
             const unsigned char *nl = col->nlow.p;
-            if (fwd) {
-                if (dilu)
-                    hipLaunchKernelGGL((k_ilu_fwd<T, true, KK, R>), dim3(g), dim3(kBlock), 0, st_,
-                                       n, A.K, A.col, nl, lu, (const T *)w.ud.p, r0, r1, h, c0end,
-                                       c, w.tmp.p, y, last, rpp, st);
-                else
-                    hipLaunchKernelGGL((k_ilu_fwd<T, false, KK, R>), dim3(g), dim3(kBlock), 0,
-                                       st_, n, A.K, A.col, nl, lu, (const T *)w.ud.p, r0, r1, h,
-                                       c0end, c, w.tmp.p, y, last, rpp, st);
-            } else {
-                if (dilu)
-                    hipLaunchKernelGGL((k_ilu_bwd<T, true, KK, R>), dim3(g), dim3(kBlock), 0, st_,
-                                       n, A.K, A.col, nl, lu, (const T *)w.ud.p, r0, r1, h, c0end,
-                                       c, (const T *)w.tmp.p, y, rpp, st);
-                else
-                    hipLaunchKernelGGL((k_ilu_bwd<T, false, KK, R>), dim3(g), dim3(kBlock), 0,
-                                       st_, n, A.K, A.col, nl, lu, (const T *)w.ud.p, r0, r1, h,
-                                       c0end, c, (const T *)w.tmp.p, y, rpp, st);
-            }
+            if (fwd)
+                hipLaunchKernelGGL((k_ilu_fwd<T, KK, R>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
+                                   A.col, nl, lu, (const T *)w.ud.p, r0, r1, h, c0end, c,
+                                   w.tmp.p, y, last, rpp, st);
+            else
+                hipLaunchKernelGGL((k_ilu_bwd<T, KK, R>), dim3(g), dim3(kBlock), 0, st_, n, A.K,
+                                   A.col, nl, lu, (const T *)w.ud.p, r0, r1, h, c0end, c,
+                                   (const T *)w.tmp.p, y, rpp, st);
         };
         auto sweep_k = [&](bool fwd, int r0, int r1, int g, int last, double *rpp) {
             if (A.K == 4) sweep(std::integral_constant<int, 4>(), fwd, r0, r1, g, last, rpp);
@@ -1498,7 +1520,7 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
     };
     auto precond_rho = [&]() {
         const int nbk = precond_rho_partials(nullptr);
-        hipLaunchKernelGGL(k_cg_rho<T>, dim3(1), dim3(kBlock), 0, st_, n, (const T *)w.r.p,
+        hipLaunchKernelGGL(k_cg_rho<T>, dim3(1), dim3(fin_block(nbk)), 0, st_, n, (const T *)w.r.p,
                            (const T *)w.z.p, state_.p, rpart, nbk);
     };
     CFD_REDUCE_LAUNCH(k_residual<T>, Gs, st_, A, b, (const T *)x, w.r.p, w.p0.p, w.q.p,
@@ -1511,20 +1533,19 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
     size_t nev = 0;
     // one CG iteration after the first pq launch (the same launches every
     // iteration: p and q are updated in place)
+    double *pqpart = part_.p + kPartPQ;
     auto rest_of_iteration = [&]() {
-        hipLaunchKernelGGL(k_cg_pq<T>, dim3(1), dim3(kBlock), 0, st_, A, (const T *)w.z.p,
-                           w.p0.p, w.q.p, state_.p, part_.p, Gs);
         if (jac) {
-            CFD_REDUCE_LAUNCH((k_cg_update<true, T>), G, st_, n, x, w.r.p, (const T *)w.p0.p,
-                              (const T *)w.q.p, (const T *)A.diag, w.z.p, state_.p, part_.p,
-                              fin);
+            CFD_REDUCE_LAUNCH_K((k_cg_update<true, T>), G, st_, n, w.r.p, (const T *)w.q.p,
+                                (const T *)A.diag, w.z.p, state_.p, part_.p,
+                                (const double *)pqpart, Gs, fin);
         } else {
             // update producer, preconditioner, then one finishing launch for both
-            hipLaunchKernelGGL((k_cg_update<false, T>), dim3(G), dim3(kBlock), 0, st_, n, x,
-                               w.r.p, (const T *)w.p0.p, (const T *)w.q.p, (const T *)A.diag,
-                               (T *)nullptr, state_.p, part_.p, 0);
+            hipLaunchKernelGGL((k_cg_update<false, T>), dim3(G), dim3(kBlock), 0, st_, n, w.r.p,
+                               (const T *)w.q.p, (const T *)A.diag, (T *)nullptr, state_.p,
+                               part_.p, (const double *)pqpart, Gs, 0);
             const int nbk = precond_rho_partials(state_.p);
-            hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(kBlock), 0, st_, state_.p,
+            hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(fin_block(G + nbk)), 0, st_, state_.p,
                                (const double *)part_.p, G, (const double *)rpart, nbk);
         }
     };
@@ -1546,7 +1567,7 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
         auto record = [&](int iters) {
             for (int k = 0; k < iters; ++k) {
                 hipLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, A,
-                                   (const T *)w.z.p, w.p0.p, w.q.p, state_.p, part_.p, 0);
+                                   (const T *)w.z.p, w.p0.p, w.q.p, x, state_.p, pqpart);
                 rest_of_iteration();
             }
         };
@@ -1566,7 +1587,7 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
         for (int k = 0; k < (gexec ? 0 : batch); ++k) {
             if (prof) CFD_HIP(hipEventRecord(ev(2 * nev), st_));
             hipLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, A, (const T *)w.z.p,
-                               w.p0.p, w.q.p, state_.p, part_.p, 0);
+                               w.p0.p, w.q.p, x, state_.p, pqpart);
             if (prof) {
                 CFD_HIP(hipEventRecord(ev(2 * nev + 1), st_));
                 ++nev;
@@ -1588,10 +1609,16 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
         if (hstate_->done || issued > cfg.maxit + 2) break;
         batch = next_batch(batch, prev_it, prev_log, cfg.maxit);
     }
+    hipLaunchKernelGGL(k_cg_xlast<T>, dim3(grid_for(n)), dim3(kBlock), 0, st_, n, x,
+                       (const T *)w.p0.p, (const KState *)state_.p);
+    CFD_LAUNCH_CHECK();
     if (prof) {
+        // launches that did work, less the first (which has no x update to
+        // apply), so every counted launch moves the same bytes
         size_t worked = std::min<size_t>(nev, (size_t)hstate_->iter);
         double ms_sum = 0.;
-        for (size_t k = 0; k < worked; ++k) {
+        worked = worked > 0 ? worked - 1 : 0;
+        for (size_t k = 1; k <= worked; ++k) {
             float ms = 0.f;
             CFD_HIP(hipEventElapsedTime(&ms, ev(2 * k), ev(2 * k + 1)));
             ms_sum += ms;
@@ -1639,19 +1666,19 @@ SolveStats Krylov::bicgstab(const Ell &A, const double *b, double *x, const cfd_
             apply_pc(A, pc, cfg.omega, (const double *)wd_.p0.p, ph_.p, col, wd_, nullptr,
                      state_.p);
         }
-        CFD_REDUCE_LAUNCH(k_bi_spmv<0>, Gs, st_, A, ph_.p, v_.p, rt_.p,
+        CFD_REDUCE_LAUNCH_K(k_bi_spmv<0>, Gs, st_, A, ph_.p, v_.p, rt_.p,
                            state_.p, part_.p, fin);
         if (jac) {
-            CFD_REDUCE_LAUNCH(k_bi_s<true>, G, st_, n, wd_.r.p, v_.p,
+            CFD_REDUCE_LAUNCH_K(k_bi_s<true>, G, st_, n, wd_.r.p, v_.p,
                                s_.p, sh_.p, A.diag, state_.p, part_.p, fin);
         } else {
-            CFD_REDUCE_LAUNCH(k_bi_s<false>, G, st_, n, wd_.r.p, v_.p,
+            CFD_REDUCE_LAUNCH_K(k_bi_s<false>, G, st_, n, wd_.r.p, v_.p,
                                s_.p, sh_.p, A.diag, state_.p, part_.p, fin);
             apply_pc(A, pc, cfg.omega, (const double *)s_.p, sh_.p, col, wd_, nullptr, state_.p);
         }
-        CFD_REDUCE_LAUNCH(k_bi_spmv<1>, Gs, st_, A, sh_.p, t_.p, s_.p,
+        CFD_REDUCE_LAUNCH_K(k_bi_spmv<1>, Gs, st_, A, sh_.p, t_.p, s_.p,
                            state_.p, part_.p, fin);
-        CFD_REDUCE_LAUNCH(k_bi_x, G, st_, n, x, wd_.r.p, ph_.p, sh_.p,
+        CFD_REDUCE_LAUNCH_K(k_bi_x, G, st_, n, x, wd_.r.p, ph_.p, sh_.p,
                            s_.p, t_.p, rt_.p, state_.p, part_.p, fin);
     };
     // graphs of one and of kGraphIters iterations, as for CG
@@ -1739,11 +1766,11 @@ SolveStats Krylov::gmres(const Ell &A, const double *b, double *x, const cfd_sol
                 hipLaunchKernelGGL(k_gm_spmv, dim3(g8), dim3(kBlock), 0, st_, A, wd_.z.p, Wn,
                                    state_.p);
             }
-            CFD_REDUCE_LAUNCH(k_gm_dot, G, st_, n, Wn, V_.p, state_.p,
+            CFD_REDUCE_LAUNCH_K(k_gm_dot, G, st_, n, Wn, V_.p, state_.p,
                                part_.p, fin);
-            CFD_REDUCE_LAUNCH(k_gm_sub<1>, G, st_, n, Wn, V_.p,
+            CFD_REDUCE_LAUNCH_K(k_gm_sub<1>, G, st_, n, Wn, V_.p,
                                state_.p, part_.p, fin);
-            CFD_REDUCE_LAUNCH(k_gm_sub<2>, G, st_, n, Wn, V_.p,
+            CFD_REDUCE_LAUNCH_K(k_gm_sub<2>, G, st_, n, Wn, V_.p,
                                state_.p, part_.p, fin);
             std::swap(W, Wn);
         }

@@ -239,13 +239,13 @@ void Krylov::solve_multi(const Ell &A, int nb, const double *const *b, double *c
         for (int k = 0; k < batch; ++k) {
             hipLaunchKernelGGL(k_m_p, dim3(g), dim3(kBlock), 0, st_, n, V, nb,
                                (const double *)A.diag, (const KState *)mstate_.p);
-            CFD_REDUCE_LAUNCH(k_m_spmv<0>, Gs, st_, A, V, nb, mstate_.p,
+            CFD_REDUCE_LAUNCH_K(k_m_spmv<0>, Gs, st_, A, V, nb, mstate_.p,
                                part_.p, fin);
-            CFD_REDUCE_LAUNCH(k_m_s, G, st_, n, V, nb,
+            CFD_REDUCE_LAUNCH_K(k_m_s, G, st_, n, V, nb,
                                (const double *)A.diag, mstate_.p, part_.p, fin);
-            CFD_REDUCE_LAUNCH(k_m_spmv<1>, Gs, st_, A, V, nb, mstate_.p,
+            CFD_REDUCE_LAUNCH_K(k_m_spmv<1>, Gs, st_, A, V, nb, mstate_.p,
                                part_.p, fin);
-            CFD_REDUCE_LAUNCH(k_m_x, G, st_, n, V, nb, mstate_.p,
+            CFD_REDUCE_LAUNCH_K(k_m_x, G, st_, n, V, nb, mstate_.p,
                                part_.p, fin);
         }
         CFD_LAUNCH_CHECK();

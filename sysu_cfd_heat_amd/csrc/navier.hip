@@ -1135,17 +1135,18 @@ int cfd_ns_profile_read(const cfd_ns *h, double out[8]) {
     long long nnz = 0;
     for (size_t k = 0; k < n.dmo.ell_col_host.size(); ++k)
         if (n.dmo.ell_col_host[k] != (int)(k % n.hm.ncel)) ++nnz;
-    // k_cg_pq per launch: reads z, diag, p, q (4 words per row), writes p, q
-    // (2 words per row); one value word + a 4 B column per stored
+    // k_cg_pq per launch (each timed launch applies the previous iteration's
+    // x update): reads z, diag, p, q, x (5 words per row), writes p, q, x
+    // (3 words per row); one value word + a 4 B column per stored
     // off-diagonal; neighbour gathers of z are not counted (each vector is
     // read once). Words are 8 B (FP64) or 4 B (FP32).
     out[0] = (double)n.kry->prof_launches;
     out[1] = n.kry->prof_ms;
-    out[2] = 48.0 * n.hm.ncel + 12.0 * (double)nnz;
+    out[2] = 64.0 * n.hm.ncel + 12.0 * (double)nnz;
     out[3] = (double)nnz;
     out[4] = (double)n.kry->prof_launches_f32();
     out[5] = n.kry->prof_ms_f32();
-    out[6] = 24.0 * n.hm.ncel + 8.0 * (double)nnz;
+    out[6] = 32.0 * n.hm.ncel + 8.0 * (double)nnz;
     out[7] = 0.;
     return 0;
 }

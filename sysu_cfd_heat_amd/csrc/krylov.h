@@ -58,7 +58,7 @@ struct KState {
 // Work vectors of one precision.
 template <class T>
 struct KWork {
-    DBuf<T> r, z, p0, p1, q, tmp, lu, ud, pdiag, pval, pb, px;
+    DBuf<T> r, z, p0, az, q, tmp, lu, ud, pdiag, pval, pb, px;
     int n = 0, K = 0;
     void ensure(int n, int K);
 };
@@ -151,9 +151,11 @@ class Krylov {
     // rz_blocks: ILU and SSOR; also write the workgroup partials of c.y to the
     // partials buffer from offset kPartRho and return their count (-1 when
     // they do not fit: the caller reduces c.y itself). st: the sweeps do
-    // nothing once that solve has stopped.
+    // nothing once that solve has stopped. az: ILU(0) also leaves (A y)_p
+    // there for the rows below az_rows (colour 0).
     void apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, Colouring *col,
-                  KWork<T> &w, int *rz_blocks = nullptr, const KState *st = nullptr);
+                  KWork<T> &w, int *rz_blocks = nullptr, const KState *st = nullptr,
+                  T *az = nullptr);
     template <class T>
     SolveStats cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &cfg, Colouring *col,
                   int pc, KWork<T> &w, bool per_case = false);

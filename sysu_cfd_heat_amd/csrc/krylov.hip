@@ -179,9 +179,13 @@ __global__ void k_add_f(int n, double *__restrict__ x, const float *__restrict__
 // Each thread takes its rows two at a time (i and i + one grid stride) so
 // that two rows' gathers are in flight together; the rows a workgroup covers
 // are those of the plain grid-stride loop.
+// Rows below c0 take (A z)_i from az, where the preconditioner's last sweep
+// left it (same sum, same order: see k_ilu_bwd0_az), instead of reading the
+// row and gathering z again.
 template <class T>
 __global__ void k_cg_pq(EllT<T> A, const T *__restrict__ z, T *__restrict__ p, T *__restrict__ q,
-                        T *__restrict__ x, const KState *st, double *part) {
+                        T *__restrict__ x, const T *__restrict__ az, int c0, const KState *st,
+                        double *part) {
     if (st->done) return;
     const T beta = (T)st->beta, alpha = (T)st->alpha;
     const bool upd = st->iter > 0;
@@ -191,8 +195,8 @@ __global__ void k_cg_pq(EllT<T> A, const T *__restrict__ z, T *__restrict__ p, T
         const int j = i + stride;
         const bool hj = j < n;
         const T zi = z[i], zj = hj ? z[j] : T(0);
-        T si = row_sum(A, i, z, A.diag[i] * zi);
-        T sj = hj ? row_sum(A, j, z, A.diag[j] * zj) : T(0);
+        T si = i < c0 ? az[i] : row_sum(A, i, z, A.diag[i] * zi);
+        T sj = hj ? (j < c0 ? az[j] : row_sum(A, j, z, A.diag[j] * zj)) : T(0);
         const T poi = p[i];
         if (upd) x[i] += alpha * poi;
         const T pi = zi + beta * poi;
@@ -890,6 +894,52 @@ __global__ void k_ilu_bwd(int n, int K, const int *__restrict__ col,
     }
 }
 
+// Colour 0's backward launch in a CG iteration. Every neighbour of a colour-0
+// row has a later colour, so all its slots are upper ones and hold A's own
+// values (U's rows without lower neighbours are A's). Besides y_p the
+// launch forms az_p = (A y)_p = a_pp y_p + sum_k a_pk y_k from the same
+// loads, in row_sum's order (diagonal term, then the slots in order; the
+// padding slots, which would add a_pk = 0, are skipped).
+template <class T, int KK>
+__global__ void k_ilu_bwd0_az(int n, const int *__restrict__ col, const T *__restrict__ lu,
+                              const T *__restrict__ ud, const T *__restrict__ diag, int r1,
+                              const T *__restrict__ c, T *__restrict__ y, T *__restrict__ az,
+                              double *rpart, const KState *st) {
+    if (st && st->done) return;
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
+    double rz[1] = {0.};
+    if (p < r1) {
+        int cl[KK];
+        T a[KK], v[KK];
+#pragma unroll
+        for (int k = 0; k < KK; ++k) cl[k] = col[(size_t)k * n + p];
+#pragma unroll
+        for (int k = 0; k < KK; ++k) {
+            if (cl[k] > p) {
+                a[k] = lu[(size_t)k * n + p];
+                v[k] = y[cl[k]];
+            }
+        }
+        const T cp = c[p];
+        T acc = cp;
+#pragma unroll
+        for (int k = 0; k < KK; ++k)
+            if (cl[k] > p) acc -= a[k] * v[k];
+        const T yv = acc / ud[p];
+        y[p] = yv;
+        rz[0] = (double)cp * yv;
+        T sp = diag[p] * yv;
+#pragma unroll
+        for (int k = 0; k < KK; ++k)
+            if (cl[k] > p) sp += a[k] * v[k];
+        az[p] = sp;
+    }
+    if (rpart) {
+        block_sum<1>(rz);
+        if (threadIdx.x == 0) rpart[blockIdx.x] = rz[0];
+    }
+}
+
 // permuted copies: A'[p] = A[perm[p]], slot k of row p taken from slot
 // smap[k][p] of the source row (columns via the cached pcol)
 __global__ void k_perm_matrix(int n, int K, const int *__restrict__ perm,
@@ -1015,7 +1065,7 @@ void KWork<T>::ensure(int n_, int K_) {
     if (n_ <= n && K_ <= K) return;
     n = std::max(n, n_);
     K = std::max(K, K_);
-    for (auto *b : {&r, &z, &p0, &p1, &q, &tmp, &ud, &pdiag, &pb, &px}) b->alloc(n);
+    for (auto *b : {&r, &z, &p0, &az, &q, &tmp, &ud, &pdiag, &pb, &px}) b->alloc(n);
     lu.alloc((size_t)K * n);
     pval.alloc((size_t)K * n);
 }
@@ -1221,6 +1271,15 @@ double Krylov::relres(const Ell &A, const double *x, const double *b) {
     return bn > 0 ? rn / bn : rn;
 }
 
+// Rows [0, az_rows) of the permuted order for which ILU(0)'s last backward
+// launch can also leave (A y)_p (k_ilu_bwd0_az): colour 0, when there is a
+// later colour and the slot count is a constant; 0 otherwise.
+template <class T>
+static int az_rows(const EllT<T> &A, int pc, const Colouring *col) {
+    if (pc != CFD_PC_ILU || !col || col->ncolour < 2 || (A.K != 4 && A.K != 6)) return 0;
+    return col->off[1];
+}
+
 template <class T>
 void Krylov::setup_pc(const EllT<T> &A, int pc, Colouring *col, KWork<T> &w) {
     if (pc != CFD_PC_ILU) return;
@@ -1251,7 +1310,7 @@ void Krylov::setup_pc(const EllT<T> &A, int pc, Colouring *col, KWork<T> &w) {
 // A is the permuted matrix for SSOR / ILU
 template <class T>
 void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, Colouring *col,
-                      KWork<T> &w, int *rz_blocks, const KState *st) {
+                      KWork<T> &w, int *rz_blocks, const KState *st, T *az) {
     if (rz_blocks) *rz_blocks = -1;
     const int n = A.n;
     if (pc == CFD_PC_NONE) {
@@ -1347,7 +1406,19 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
             const int r0 = col->off[k], r1 = col->off[k + 1];
             if (r1 <= r0) continue;
             const int g = grid8((r1 - r0 + kSweepRows - 1) / kSweepRows);
-            sweep_k(false, r0, r1, g, 0, rp ? rp + used : (double *)nullptr);
+            double *rpp = rp ? rp + used : (double *)nullptr;
+            if (k == 0 && az && az_rows(A, pc, col) > 0) {
+                if (A.K == 4)
+                    hipLaunchKernelGGL((k_ilu_bwd0_az<T, 4>), dim3(grid8(r1)), dim3(kBlock), 0,
+                                       st_, n, A.col, lu, (const T *)w.ud.p, (const T *)A.diag,
+                                       r1, c, y, az, rpp, st);
+                else
+                    hipLaunchKernelGGL((k_ilu_bwd0_az<T, 6>), dim3(grid8(r1)), dim3(kBlock), 0,
+                                       st_, n, A.col, lu, (const T *)w.ud.p, (const T *)A.diag,
+                                       r1, c, y, az, rpp, st);
+            } else {
+                sweep_k(false, r0, r1, g, 0, rpp);
+            }
             if (rp) used += g;
         }
         if (rz_blocks && rp) *rz_blocks = used;
@@ -1504,13 +1575,14 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
     const double dev_tol = per_case ? 0. : cfg.tol;  // batched: the host decides
     const int n = A.n, G = red_grid(n), Gs = spmv_grid(n);
     const bool jac = pc == CFD_PC_JACOBI;
+    const int c0 = az_rows(A, pc, col);  // rows whose (A z)_i the preconditioner leaves in az
     // z = M^-1 r and the partials of rho = r.z above kPartRho (the ILU sweeps
     // write them themselves); returns their count
     double *rpart = part_.p + kPartRho;
     auto precond_rho_partials = [&](const KState *stp) -> int {
         int nbk = -1;
         apply_pc(A, pc, cfg.omega, (const T *)w.r.p, w.z.p, col, w,
-                 (pc == CFD_PC_ILU || pc == CFD_PC_SSOR) ? &nbk : nullptr, stp);
+                 (pc == CFD_PC_ILU || pc == CFD_PC_SSOR) ? &nbk : nullptr, stp, w.az.p);
         if (nbk < 0) {
             nbk = G;
             hipLaunchKernelGGL(k_cg_rho<T>, dim3(G), dim3(kBlock), 0, st_, n, (const T *)w.r.p,
@@ -1557,6 +1629,7 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
             sizeof(T), (uintptr_t)A.diag, (uintptr_t)A.val, (uintptr_t)A.col, (uintptr_t)x,
             (uintptr_t)b, (uintptr_t)w.r.p, (uintptr_t)w.z.p, (uintptr_t)w.p0.p,
             (uintptr_t)w.q.p, (uintptr_t)w.tmp.p, (uintptr_t)w.ud.p, (uintptr_t)w.lu.p,
+            (uintptr_t)w.az.p,
             (uintptr_t)n, (uintptr_t)A.K, (uintptr_t)pc, (uintptr_t)col,
             (uintptr_t)(col ? col->ncolour : 0), (uintptr_t)(col ? col->triangle_free : 0)};
         uint64_t om;
@@ -1567,7 +1640,8 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
         auto record = [&](int iters) {
             for (int k = 0; k < iters; ++k) {
                 hipLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, A,
-                                   (const T *)w.z.p, w.p0.p, w.q.p, x, state_.p, pqpart);
+                                   (const T *)w.z.p, w.p0.p, w.q.p, x, (const T *)w.az.p, c0,
+                                   state_.p, pqpart);
                 rest_of_iteration();
             }
         };
@@ -1587,7 +1661,7 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
         for (int k = 0; k < (gexec ? 0 : batch); ++k) {
             if (prof) CFD_HIP(hipEventRecord(ev(2 * nev), st_));
             hipLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, A, (const T *)w.z.p,
-                               w.p0.p, w.q.p, x, state_.p, pqpart);
+                               w.p0.p, w.q.p, x, (const T *)w.az.p, c0, state_.p, pqpart);
             if (prof) {
                 CFD_HIP(hipEventRecord(ev(2 * nev + 1), st_));
                 ++nev;

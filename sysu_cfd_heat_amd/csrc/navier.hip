@@ -1132,21 +1132,30 @@ int cfd_ns_profile(cfd_ns *h, int enable) {
 
 int cfd_ns_profile_read(const cfd_ns *h, double out[8]) {
     const NSSolver &n = h->ns;
-    long long nnz = 0;
-    for (size_t k = 0; k < n.dmo.ell_col_host.size(); ++k)
-        if (n.dmo.ell_col_host[k] != (int)(k % n.hm.ncel)) ++nnz;
+    // with ILU(0) over two or more colours (4 or 6 slots), colour-0 rows take
+    // (A z)_i from the preconditioner's last sweep instead of their row
+    const int K = n.dmo.dm.K;
+    const bool az = n.P.pres.precond == CFD_PC_ILU && n.colour_ready &&
+                    n.colour.ncolour > 1 && (K == 4 || K == 6);
+    long long nnz = 0, nnz_read = 0;
+    for (size_t k = 0; k < n.dmo.ell_col_host.size(); ++k) {
+        const int row = (int)(k % n.hm.ncel);
+        if (n.dmo.ell_col_host[k] == row) continue;
+        ++nnz;
+        if (!(az && n.colour.colour_host[row] == 0)) ++nnz_read;
+    }
     // k_cg_pq per launch (each timed launch applies the previous iteration's
-    // x update): reads z, diag, p, q, x (5 words per row), writes p, q, x
-    // (3 words per row); one value word + a 4 B column per stored
-    // off-diagonal; neighbour gathers of z are not counted (each vector is
-    // read once). Words are 8 B (FP64) or 4 B (FP32).
+    // x update): reads z, p, q, x and the diagonal or az_i (5 words per row),
+    // writes p, q, x (3 words per row); one value word + a 4 B column per
+    // off-diagonal of the rows it computes; neighbour gathers of z are not
+    // counted (each vector is read once). Words are 8 B (FP64) or 4 B (FP32).
     out[0] = (double)n.kry->prof_launches;
     out[1] = n.kry->prof_ms;
-    out[2] = 64.0 * n.hm.ncel + 12.0 * (double)nnz;
+    out[2] = 64.0 * n.hm.ncel + 12.0 * (double)nnz_read;
     out[3] = (double)nnz;
     out[4] = (double)n.kry->prof_launches_f32();
     out[5] = n.kry->prof_ms_f32();
-    out[6] = 32.0 * n.hm.ncel + 8.0 * (double)nnz;
+    out[6] = 32.0 * n.hm.ncel + 8.0 * (double)nnz_read;
     out[7] = 0.;
     return 0;
 }

@@ -64,8 +64,11 @@ def parse():
                     help="launch-shape autotuner of the assembly and gradient kernels")
     ap.add_argument("--graphs", type=int, default=1,
                     help="replay pressure CG iterations from a captured HIP graph (cavity and "
-                         "batch workloads; the channel run times its SpMV launches with events, "
-                         "which keeps them eager)")
+                         "batch workloads; the channel run times one SpMV launch in eight with "
+                         "events, which keeps its launches eager)")
+    ap.add_argument("--live-timing", type=int, default=1,
+                    help="channel: time the pressure SpMV launches with HIP events inside the "
+                         "timed region (the roofline figure); 0 leaves them untimed")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many MiB device-to-device (5 times) and exit: the "
                          "known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE "
@@ -332,6 +335,7 @@ def main():
     gpu = NSSolver(raw, p)
     gpu.set_option("mixed_inner_tol", a.mixed_inner_tol)
     gpu.set_option("autotune", a.autotune)
+    gpu.set_option("graphs", a.graphs)
     setup_s = time.time() - t0
     print(f"[bench] setup {setup_s:.1f}s cells={raw.counts[1]} fluid={gpu.counts(0)[1]} "
           f"solid={gpu.counts(1)[1]}", file=sys.stderr, flush=True)
@@ -340,7 +344,7 @@ def main():
         print(f"[bench] warmup {k + 1}/{a.warmup} done", file=sys.stderr, flush=True)
     gpu.sync()
     gpu._fn("reset_stats")(gpu.h)
-    gpu.profile(True)
+    gpu.profile(bool(a.live_timing))
     t1 = time.perf_counter()
     for k in range(a.steps):
         # returns after the fluid work of the iteration is done; the solid
@@ -360,7 +364,7 @@ def main():
     roof = {"bound": "hbm", "kernel": "k_cg_pq (pressure CG: p update + SpMV + p.q)",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-            "launches": int(launches), "avg_us": avg_ms * 1e3,
+            "launches": int(launches), "avg_us": avg_ms * 1e3 if launches else None,
             "bytes_per_launch": bytes_per}
     krylov = {"mom_iters_per_step": st[2] / a.steps, "p_iters_per_step": st[3] / a.steps,
               "ke_iters_per_step": st[4] / a.steps, "T_iters_per_step": st[5] / a.steps,

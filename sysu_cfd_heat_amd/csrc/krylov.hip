@@ -1087,6 +1087,9 @@ Krylov::~Krylov() {
 // iterations per long graph of a Krylov loop (a batch of b iterations is
 // b / kGraphIters launches of it and the rest of the one-iteration graph)
 constexpr int kGraphIters = 8;
+// live timing of k_cg_pq: one launch in kProfEvery between HIP events (every
+// launch moves the same bytes; timing each would cost 3 % of the run)
+constexpr int kProfEvery = 8;
 
 // Executable graph of the launches rec() makes on this stream, captured on
 // first use of key (every pointer and size the launches bake in).
@@ -1576,6 +1579,9 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
     const int n = A.n, G = red_grid(n), Gs = spmv_grid(n);
     const bool jac = pc == CFD_PC_JACOBI;
     const int c0 = az_rows(A, pc, col);  // rows whose (A z)_i the preconditioner leaves in az
+    // the update at full grid: 1024-thread workgroups, a quarter as many,
+    // each summing the p.q partials once (the same threads stream the rows)
+    const int ub = G >= kRedGrid ? kFinBlock : kBlock, Gu = G * kBlock / ub;
     // z = M^-1 r and the partials of rho = r.z above kPartRho (the ILU sweeps
     // write them themselves); returns their count
     double *rpart = part_.p + kPartRho;
@@ -1603,6 +1609,7 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
     int batch = check_first, issued = 0, prev_it = 0;
     double prev_log = 1e300;
     size_t nev = 0;
+    std::vector<int> ev_iter;  // the iteration of each timed launch
     // one CG iteration after the first pq launch (the same launches every
     // iteration: p and q are updated in place)
     double *pqpart = part_.p + kPartPQ;
@@ -1613,12 +1620,12 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
                                 (const double *)pqpart, Gs, fin);
         } else {
             // update producer, preconditioner, then one finishing launch for both
-            hipLaunchKernelGGL((k_cg_update<false, T>), dim3(G), dim3(kBlock), 0, st_, n, w.r.p,
+            hipLaunchKernelGGL((k_cg_update<false, T>), dim3(Gu), dim3(ub), 0, st_, n, w.r.p,
                                (const T *)w.q.p, (const T *)A.diag, (T *)nullptr, state_.p,
                                part_.p, (const double *)pqpart, Gs, 0);
             const int nbk = precond_rho_partials(state_.p);
-            hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(fin_block(G + nbk)), 0, st_, state_.p,
-                               (const double *)part_.p, G, (const double *)rpart, nbk);
+            hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(fin_block(Gu + nbk)), 0, st_, state_.p,
+                               (const double *)part_.p, Gu, (const double *)rpart, nbk);
         }
     };
     // graphs of one and of kGraphIters iterations: a batch of b iterations is
@@ -1659,11 +1666,15 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
             graph_launches += batch;
         }
         for (int k = 0; k < (gexec ? 0 : batch); ++k) {
-            if (prof) CFD_HIP(hipEventRecord(ev(2 * nev), st_));
+            // live timing samples iterations 1, 1 + kProfEvery, ... (iteration
+            // 0's launch has no x update to apply)
+            const bool timed = prof && (issued + k) % kProfEvery == 1;
+            if (timed) CFD_HIP(hipEventRecord(ev(2 * nev), st_));
             hipLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, A, (const T *)w.z.p,
                                w.p0.p, w.q.p, x, (const T *)w.az.p, c0, state_.p, pqpart);
-            if (prof) {
+            if (timed) {
                 CFD_HIP(hipEventRecord(ev(2 * nev + 1), st_));
+                ev_iter.push_back(issued + k);
                 ++nev;
             }
             rest_of_iteration();
@@ -1687,15 +1698,15 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
                        (const T *)w.p0.p, (const KState *)state_.p);
     CFD_LAUNCH_CHECK();
     if (prof) {
-        // launches that did work, less the first (which has no x update to
-        // apply), so every counted launch moves the same bytes
-        size_t worked = std::min<size_t>(nev, (size_t)hstate_->iter);
+        // the timed launches that did work (iterations before the last)
+        size_t worked = 0;
         double ms_sum = 0.;
-        worked = worked > 0 ? worked - 1 : 0;
-        for (size_t k = 1; k <= worked; ++k) {
+        for (size_t k = 0; k < nev; ++k) {
+            if (ev_iter[k] >= hstate_->iter) continue;
             float ms = 0.f;
             CFD_HIP(hipEventElapsedTime(&ms, ev(2 * k), ev(2 * k + 1)));
             ms_sum += ms;
+            ++worked;
         }
         if (sizeof(T) == 8) {
             prof_ms += ms_sum;

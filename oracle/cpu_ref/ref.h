@@ -67,6 +67,11 @@ int cell_face_info(Mesh &m);               // src/BasicSolver.cpp:317-512
 int split_region(const Mesh &full, const std::vector<int> &new2old,
                  int coupled_id, Mesh &out);  // src/Preprocessor.cpp:308-438
 std::vector<int> greedy_colour(int n, const std::vector<std::vector<int>> &adj);
+// The product's ILU(0) block colouring (blocks of B consecutive cells):
+// colour per cell, or empty when the mesh has none (not every cell has 4
+// faces, n not a multiple of B, or a triangle in the adjacency).
+std::vector<int> block_colour(int n, const std::vector<int> &nface,
+                              const std::vector<std::vector<int>> &adj, int B);
 
 // LASPack restatement (rows with 0-based columns; entry 0 is whatever the
 // caller set first, the reference always sets the diagonal there)

@@ -317,4 +317,16 @@ int oracle_colour(int n, const int *ptr, const int *adj, int *colour) {
     return 0;
 }
 
+// The product's ILU(0) block colouring (block_colour): out[i] = colour of
+// cell i, or -1 everywhere when the mesh has none.
+int oracle_block_colour(int n, const int *nface, const int *ptr, const int *adj, int block,
+                        int *out) {
+    std::vector<std::vector<int>> a(n);
+    for (int i = 0; i < n; ++i) a[i].assign(adj + ptr[i], adj + ptr[i + 1]);
+    std::vector<int> nf(nface, nface + n);
+    std::vector<int> c = block_colour(n, nf, a, block);
+    for (int i = 0; i < n; ++i) out[i] = c.empty() ? -1 : c[i];
+    return 0;
+}
+
 }  // extern "C"

@@ -1,5 +1,6 @@
 // ORACLE -- TEST INFRASTRUCTURE ONLY (see ref.h).
 // Restates face construction, geometry and the conjugate-heat-transfer split.
+#include <deque>
 #include <cmath>
 #include <cstdio>
 #include <map>
@@ -339,6 +340,58 @@ std::vector<int> greedy_colour(int n, const std::vector<std::vector<int>> &adj) 
         col[i] = c;
     }
     return col;
+}
+
+// Restates the product's block colouring rule for its ILU(0) order (the
+// reference has none; this is the ordering the GPU factors in). Eligible
+// meshes: every cell has 4 faces, n is a multiple of B, no triangle i-j-k in
+// the adjacency. Blocks are cells [bB, (b+1)B); the block graph joins blocks
+// with a face between them. Bipartite: colour by breadth-first parity from
+// the lowest uncoloured block of each component (colour 0); otherwise
+// greedy first-fit over blocks in order.
+std::vector<int> block_colour(int n, const std::vector<int> &nface,
+                              const std::vector<std::vector<int>> &adj, int B) {
+    if (n == 0 || n % B) return {};
+    for (int i = 0; i < n; ++i)
+        if (nface[i] != 4) return {};
+    std::vector<char> mark(n, 0);
+    for (int i = 0; i < n; ++i) {  // triangle test through a neighbour mark
+        for (int j : adj[i]) mark[j] = 1;
+        bool tri = false;
+        for (int j : adj[i])
+            for (int k : adj[j])
+                if (k != i && mark[k]) tri = true;
+        for (int j : adj[i]) mark[j] = 0;
+        if (tri) return {};
+    }
+    const int nb = n / B;
+    std::vector<std::vector<int>> badj(nb);
+    for (int i = 0; i < n; ++i)
+        for (int j : adj[i])
+            if (j / B != i / B) badj[i / B].push_back(j / B);
+    std::vector<int> bc(nb, -1);
+    bool bip = true;
+    for (int s0 = 0; s0 < nb && bip; ++s0) {
+        if (bc[s0] >= 0) continue;
+        std::deque<int> q{s0};
+        bc[s0] = 0;
+        while (!q.empty() && bip) {
+            int u = q.front();
+            q.pop_front();
+            for (int v : badj[u]) {
+                if (bc[v] < 0) {
+                    bc[v] = 1 - bc[u];
+                    q.push_back(v);
+                } else if (bc[v] == bc[u]) {
+                    bip = false;
+                }
+            }
+        }
+    }
+    if (!bip) bc = greedy_colour(nb, badj);
+    std::vector<int> out(n);
+    for (int i = 0; i < n; ++i) out[i] = bc[i / B];
+    return out;
 }
 
 }  // namespace oref

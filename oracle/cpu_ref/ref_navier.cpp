@@ -192,6 +192,11 @@ int RefNS::setup(const cfd_raw_mesh &raw, const cfd_params &p) {
         for (int j = 0; j < M.cell[i].nface; ++j)
             if (M.cell[i].cell[j] >= 0) adj[i].push_back(M.cell[i].cell[j]);
     colour = greedy_colour(n, adj);
+    {
+        std::vector<int> nf(n);
+        for (int i = 0; i < n; ++i) nf[i] = M.cell[i].nface;
+        bcolour = block_colour(n, nf, adj, 6);
+    }
     if (ht) {
         RefHT &h = *ht;
         int m = h.M.ncel;
@@ -203,6 +208,9 @@ int RefNS::setup(const cfd_raw_mesh &raw, const cfd_params &p) {
             for (int j = 0; j < h.M.cell[i].nface; ++j)
                 if (h.M.cell[i].cell[j] >= 0) a2[i].push_back(h.M.cell[i].cell[j]);
         h.colour = greedy_colour(m, a2);
+        std::vector<int> nf(m);
+        for (int i = 0; i < m; ++i) nf[i] = h.M.cell[i].nface;
+        h.bcolour = block_colour(m, nf, a2, 6);
     }
     return InitFlowField();
 }
@@ -543,7 +551,8 @@ SolveCfg RefNS::cfg_of(const cfd_solve_cfg &c) const {
     s.tol = c.tol;
     s.maxit = c.maxit;
     s.mc = mc;
-    s.colour = &colour;
+    // ILU(0) factors in the product's block-colour order where the mesh has one
+    s.colour = (c.precond == CFD_PC_ILU && !bcolour.empty()) ? &bcolour : &colour;
     return s;
 }
 
@@ -1229,7 +1238,7 @@ int RefHT::UpdateEnergy() {
     c.tol = ns->P.solid.tol;
     c.maxit = ns->P.solid.maxit;
     c.mc = ns->mc;
-    c.colour = &colour;
+    c.colour = (c.precond == CFD_PC_ILU && !bcolour.empty()) ? &bcolour : &colour;
     last = las_solve(As, xsol, bs, c);
     if (last.iters >= c.maxit && last.relres > c.tol) return -1;
     Tn = xsol;

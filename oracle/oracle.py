@@ -56,6 +56,7 @@ def lib():
         L.oracle_las_precond.argtypes = ([C.c_int, C.c_int, ip, ip, dp, C.c_int, C.c_double,
                                           C.c_int, dp, dp])
         L.oracle_colour.argtypes = [C.c_int, ip, ip, ip]
+        L.oracle_block_colour.argtypes = [C.c_int, ip, ip, ip, C.c_int, ip]
         _lib = L
     return _lib
 
@@ -135,6 +136,20 @@ class NSSolver(P.NSSolver):
         if t < 0:
             raise RuntimeError(_err())
         return t, r.value
+
+
+def block_colour(nface, adj_lists, block=6):
+    """The product's ILU(0) block colouring (cells in blocks of `block`
+    consecutive cells); -1 everywhere when the mesh has none."""
+    n = len(adj_lists)
+    ptr = np.zeros(n + 1, np.int32)
+    for i, a in enumerate(adj_lists):
+        ptr[i + 1] = ptr[i] + len(a)
+    adj = np.array([j for a in adj_lists for j in a] or [0], np.int32)
+    nf = np.ascontiguousarray(nface, np.int32)
+    out = np.zeros(n, np.int32)
+    lib().oracle_block_colour(n, P._ip(nf), P._ip(ptr), P._ip(adj), block, P._ip(out))
+    return out
 
 
 def rows_from_dense(A: np.ndarray, sym: bool = False):

@@ -202,7 +202,8 @@ def box_mesh(nx: int, ny: int, nz: int, lx=1.0, ly=1.0, lz=1.0, tets=False,
 
 INT_ARRAYS = {"face_vert": 4, "face_cell1": 1, "face_cell2": 1, "face_bnd": 1,
               "cell_nface": 1, "cell_face": 6, "cell_cell": 6, "cell_vert": 8,
-              "cell_rid": 1, "bnd_face": 1, "bnd_rid": 1, "cell_colour": 1}
+              "cell_rid": 1, "bnd_face": 1, "bnd_rid": 1, "cell_colour": 1,
+              "cell_bcolour": 1}
 DBL_ARRAYS = {"vert": 3, "face_x": 3, "face_n": 3, "face_area": 1, "face_lambda": 1,
               "face_rlencos": 1, "face_xpac": 3, "face_xnac": 3, "cell_x": 3,
               "cell_vol": 1, "bnd_distance": 1}

@@ -26,19 +26,32 @@ using EllF = EllT<float>;
 // blocks (greedy first-fit over the symmetrised graph in natural order,
 // stable within a colour), so each triangular sweep of SSOR / ILU(0) is one
 // contiguous launch per colour.
+// Block-colour form (block > 1, ILU(0) only): rows come in blocks of `block`
+// consecutive natural rows, given_colour colours the blocks (no two blocks
+// of a colour adjacent), and row j of colour c's t-th block sits at
+// off[c] + j * nbk[c] + t. One thread then takes a block (its rows in
+// natural order), and a wave reads row j of 64 blocks coalesced.
 struct Colouring {
     int n = 0, K = 0, ncolour = 0;
+    int block = 1;                 // rows per block (1: plain multicolour order)
     bool triangle_free = false;    // ILU(0) keeps A's off-diagonals
     const int *src_col = nullptr;  // device structure this was built for
     std::vector<int> off;          // colour block offsets, size ncolour+1
+    std::vector<int> nbk;          // blocks per colour (block > 1)
     std::vector<int> colour_host;
     DBuf<int> perm, iperm, pcol;
-    // pcol lists each row's earlier-colour neighbours first: nlow[p] of them;
-    // slot k of permuted row p is slot smap[k][p] of the natural row
+    // pcol lists each row's lower (earlier in the order) neighbours first:
+    // nlow[p] of them; slot k of permuted row p is slot smap[k][p] of the
+    // natural row
     DBuf<unsigned char> smap, nlow;
+    // block > 1: per slot (K x n, slot-major), the neighbour's row within the
+    // row's own block, kNotInBlock for other colours and padding
+    DBuf<unsigned char> bin;
+    static constexpr unsigned char kNotInBlock = 255;
     // host_col: the K x n slot-major column array; given_colour: optional
+    // for block == 1, the block colour of every row for block > 1
     void build(const int *host_col, int n, int K, const int *given_colour, const int *dev_col,
-               hipStream_t s);
+               hipStream_t s, int block = 1);
 };
 
 struct SolveStats {
@@ -156,6 +169,9 @@ class Krylov {
     void apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, Colouring *col,
                   KWork<T> &w, int *rz_blocks = nullptr, const KState *st = nullptr,
                   T *az = nullptr);
+    template <class T>
+    void apply_block_ilu(const EllT<T> &A, const T *c, T *y, Colouring *col, KWork<T> &w,
+                         int *rz_blocks, const KState *st, T *az);
     template <class T>
     SolveStats cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &cfg, Colouring *col,
                   int pc, KWork<T> &w, bool per_case = false);

@@ -20,6 +20,7 @@
 #include <type_traits>
 
 #include "krylov.h"
+#include "mesh.h"
 
 namespace cfd {
 
@@ -940,6 +941,198 @@ __global__ void k_ilu_bwd0_az(int n, const int *__restrict__ col, const T *__res
     }
 }
 
+// ---- ILU(0) sweeps in block-colour order (Colouring::block > 1) ----------
+// Colour c's rows sit at r0 + j * nb + t (row j of its t-th block); one
+// thread takes block t and walks its rows in order, so a wave reads row j of
+// 64 blocks coalesced. A row's lower neighbours are those of earlier colours
+// and the earlier rows of its own block; no two blocks of a colour are
+// adjacent, so a same-colour neighbour is in the row's own block (bin gives
+// its row) and its value is taken from the thread's column of an LDS array
+// [row][thread]; only other colours' values are gathered from memory.
+// Sums run in slot order, as in the one-row-per-thread sweeps.
+
+// One block's rows, every load issued before the recursion through the
+// block (which then waits on registers and LDS only): per row j and slot k
+// the column, the same-block row (bin), the factor value and, for a
+// neighbour of another colour, its gathered value. FWD selects the lower
+// slots, BWD the upper ones (padding slots, which hold the row itself, are
+// neither).
+template <class T, int KK, int B>
+struct BlockLoad {
+    unsigned char bj[B][KK];
+    bool use[B][KK];
+    int nl[B];
+    T a[B][KK], g[B][KK];
+    // aval (backward only): A's values into the lower slots' a (not used by
+    // the recursion; for (A y)_p)
+    template <bool FWD>
+    __device__ __forceinline__ void load(int n, int K, const int *__restrict__ col,
+                                         const unsigned char *__restrict__ bin,
+                                         const unsigned char *__restrict__ nlow,
+                                         const T *__restrict__ lu, int base, int nb,
+                                         const T *__restrict__ other,
+                                         const T *__restrict__ aval = nullptr) {
+        int cl[B][KK];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const int p = base + j * nb;
+            nl[j] = nlow[p];
+#pragma unroll
+            for (int k = 0; k < KK; ++k) {
+                cl[j][k] = k < K ? col[(size_t)k * n + p] : p;
+                bj[j][k] = k < K ? bin[(size_t)k * n + p] : Colouring::kNotInBlock;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const int p = base + j * nb;
+#pragma unroll
+            for (int k = 0; k < KK; ++k) {
+                use[j][k] = FWD ? k < nl[j] : (k >= nl[j] && cl[j][k] > p);
+                a[j][k] = use[j][k] ? lu[(size_t)k * n + p]
+                          : (!FWD && aval && k < nl[j]) ? aval[(size_t)k * n + p] : T(0);
+                g[j][k] = (use[j][k] && bj[j][k] == Colouring::kNotInBlock) ? other[cl[j][k]]
+                                                                            : T(0);
+            }
+        }
+    }
+    // acc minus a * value over the selected slots of row j, in slot order
+    // (same-block values from the LDS column sv[.][tid])
+    __device__ __forceinline__ T sub(int j, T acc, const T (*sv)[kBlock], int tid) const {
+#pragma unroll
+        for (int k = 0; k < KK; ++k)
+            if (use[j][k])
+                acc -= a[j][k] *
+                       (bj[j][k] != Colouring::kNotInBlock ? sv[bj[j][k]][tid] : g[j][k]);
+        return acc;
+    }
+};
+
+// forward sweep of a colour that is not the last: t of its rows,
+// t_p = c_p - sum_lower l_pk t_k
+template <class T, int KK, int B>
+__global__ __launch_bounds__(256) void k_bilu_fwd(int n, int K, const int *__restrict__ col,
+                                                  const unsigned char *__restrict__ bin,
+                                                  const unsigned char *__restrict__ nlow,
+                                                  const T *__restrict__ lu, int r0, int nb,
+                                                  const T *__restrict__ c,
+                                                  const T *__restrict__ tin, T *__restrict__ tout,
+                                                  const KState *st) {
+    __shared__ T sv[B][kBlock];
+    if (st && st->done) return;
+    const int tid = threadIdx.x, t = xcd_block() * blockDim.x + tid;
+    if (t >= nb) return;
+    const int base = r0 + t;
+    BlockLoad<T, KK, B> L;
+    L.template load<true>(n, K, col, bin, nlow, lu, base, nb, tin);
+    T cv[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j) cv[j] = c[base + j * nb];
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+        const T tv = L.sub(j, cv[j], sv, tid);
+        sv[j][tid] = tv;
+        tout[base + j * nb] = tv;
+    }
+}
+// the last colour: forward and backward in one pass (its upper neighbours
+// are in its own block); y of its rows and the partials of c.y
+template <class T, int KK, int B>
+__global__ __launch_bounds__(256) void k_bilu_last(int n, int K, const int *__restrict__ col,
+                                                   const unsigned char *__restrict__ bin,
+                                                   const unsigned char *__restrict__ nlow,
+                                                   const T *__restrict__ lu,
+                                                   const T *__restrict__ ud, int r0, int nb,
+                                                   const T *__restrict__ c,
+                                                   const T *__restrict__ tin, T *__restrict__ y,
+                                                   double *rpart, const KState *st) {
+    __shared__ T sv[B][kBlock], sy[B][kBlock];
+    if (st && st->done) return;
+    const int tid = threadIdx.x, t = xcd_block() * blockDim.x + tid;
+    double rz[1] = {0.};
+    if (t < nb) {
+        const int base = r0 + t;
+        T cv[B], dv[B];
+        {
+            BlockLoad<T, KK, B> L;
+            L.template load<true>(n, K, col, bin, nlow, lu, base, nb, tin);
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                cv[j] = c[base + j * nb];
+                dv[j] = ud[base + j * nb];
+            }
+#pragma unroll
+            for (int j = 0; j < B; ++j) sv[j][tid] = L.sub(j, cv[j], sv, tid);
+        }
+        BlockLoad<T, KK, B> U;
+        U.template load<false>(n, K, col, bin, nlow, lu, base, nb, y);
+#pragma unroll
+        for (int j = B - 1; j >= 0; --j) {
+            const T yv = U.sub(j, sv[j][tid], sy, tid) / dv[j];
+            sy[j][tid] = yv;
+            y[base + j * nb] = yv;
+            rz[0] += (double)cv[j] * yv;
+        }
+    }
+    if (rpart) {
+        block_sum<1>(rz);
+        if (threadIdx.x == 0) rpart[blockIdx.x] = rz[0];
+    }
+}
+// backward sweep of a colour that is not the last: y of its rows from their
+// t and later colours' y (y_p = (t_p - sum_upper u_pk y_k) / u_pp), the
+// partials of c.y. AZ (colour 0, whose lower neighbours are all in its own
+// block): also az_p = (A y)_p for its rows, a_pp y_p + sum_k a_pk y_k in slot
+// order (lower slots from A, whose values the factor replaced; padding slots
+// skipped), as k_cg_pq would sum it.
+template <class T, int KK, int B, bool AZ>
+__global__ __launch_bounds__(256) void k_bilu_bwd(
+    int n, int K, const int *__restrict__ col, const unsigned char *__restrict__ bin,
+    const unsigned char *__restrict__ nlow, const T *__restrict__ lu, const T *__restrict__ ud,
+    const T *__restrict__ aval, const T *__restrict__ adiag, int r0, int nb,
+    const T *__restrict__ c, const T *__restrict__ tin, const T *__restrict__ yin,
+    T *__restrict__ yout, T *__restrict__ az, double *rpart, const KState *st) {
+    __shared__ T sy[B][kBlock];
+    if (st && st->done) return;
+    const int tid = threadIdx.x, t = xcd_block() * blockDim.x + tid;
+    double rz[1] = {0.};
+    if (t < nb) {
+        const int base = r0 + t;
+        BlockLoad<T, KK, B> U;
+        U.template load<false>(n, K, col, bin, nlow, lu, base, nb, yin, AZ ? aval : nullptr);
+        T tv[B], dv[B], dg[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            tv[j] = tin[base + j * nb];
+            dv[j] = ud[base + j * nb];
+            if (AZ) dg[j] = adiag[base + j * nb];
+        }
+#pragma unroll
+        for (int j = B - 1; j >= 0; --j) {
+            const T yv = U.sub(j, tv[j], sy, tid) / dv[j];
+            sy[j][tid] = yv;
+            yout[base + j * nb] = yv;
+            rz[0] += (double)c[base + j * nb] * yv;
+        }
+        if (AZ) {
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                T sp = dg[j] * sy[j][tid];
+#pragma unroll
+                for (int k = 0; k < KK; ++k)
+                    if (k < U.nl[j] || U.use[j][k])  // every slot but padding
+                        sp += U.a[j][k] * (U.bj[j][k] != Colouring::kNotInBlock
+                                               ? sy[U.bj[j][k]][tid] : U.g[j][k]);
+                az[base + j * nb] = sp;
+            }
+        }
+    }
+    if (rpart) {
+        block_sum<1>(rz);
+        if (threadIdx.x == 0) rpart[blockIdx.x] = rz[0];
+    }
+}
+
 // permuted copies: A'[p] = A[perm[p]], slot k of row p taken from slot
 // smap[k][p] of the source row (columns via the cached pcol)
 __global__ void k_perm_matrix(int n, int K, const int *__restrict__ perm,
@@ -966,11 +1159,13 @@ __global__ void k_perm_scatter(int n, const int *__restrict__ perm, const double
 
 // ------------------------------------------------------------ Colouring --
 void Colouring::build(const int *hc, int n_, int K_, const int *given, const int *dev_col,
-                      hipStream_t s) {
+                      hipStream_t s, int block_) {
     n = n_;
     K = K_;
+    block = block_;
     src_col = dev_col;
     colour_host.assign(n, 0);
+    if (block > 1 && (!given || n % block)) throw HipError("block colouring: bad input");
     if (given) {
         std::copy(given, given + n, colour_host.begin());
     } else {
@@ -1023,10 +1218,28 @@ void Colouring::build(const int *hc, int n_, int K_, const int *given, const int
     for (int c : colour_host) off[c + 1]++;
     for (int c = 0; c < ncolour; ++c) off[c + 1] += off[c];
     std::vector<int> hp(n), hip_(n), fillc(off.begin(), off.end() - 1);
-    for (int i = 0; i < n; ++i) {
-        int p = fillc[colour_host[i]]++;
-        hp[p] = i;
-        hip_[i] = p;
+    if (block == 1) {
+        for (int i = 0; i < n; ++i) {
+            int p = fillc[colour_host[i]]++;
+            hp[p] = i;
+            hip_[i] = p;
+        }
+    } else {
+        nbk.assign(ncolour, 0);
+        for (int c = 0; c < ncolour; ++c) {
+            if ((off[c + 1] - off[c]) % block) throw HipError("block colouring: split block");
+            nbk[c] = (off[c + 1] - off[c]) / block;
+        }
+        std::vector<int> tnext(ncolour, 0);  // blocks of each colour met so far
+        for (int i0 = 0; i0 < n; i0 += block) {
+            const int c = colour_host[i0], t = tnext[c]++;
+            for (int j = 0; j < block; ++j) {
+                if (colour_host[i0 + j] != c) throw HipError("block colouring: split block");
+                const int p = off[c] + j * nbk[c] + t;
+                hp[p] = i0 + j;
+                hip_[i0 + j] = p;
+            }
+        }
     }
     // permuted columns with each row's earlier-colour (lower) entries first,
     // in slot order, then the rest in slot order: a forward sweep reads the
@@ -1057,6 +1270,25 @@ void Colouring::build(const int *hc, int n_, int K_, const int *given, const int
     nlow.alloc(n);
     smap.upload(hsm.data(), (size_t)K * n, s);
     nlow.upload(hnl.data(), n, s);
+    if (block > 1) {
+        // per slot: the row of the neighbour within the row's own block, or
+        // kNotInBlock (another colour; padding slots point at the row itself)
+        std::vector<unsigned char> hb((size_t)K * n, kNotInBlock);
+        for (int c = 0; c < ncolour; ++c)
+            for (int j = 0; j < block; ++j)
+                for (int t = 0; t < nbk[c]; ++t) {
+                    const int p = off[c] + j * nbk[c] + t;
+                    for (int k = 0; k < K; ++k) {
+                        const int q = hpc[(size_t)k * n + p];
+                        if (q == p || q < off[c] || q >= off[c + 1]) continue;
+                        const int d = q - off[c] - t;
+                        if (d % nbk[c] || d / nbk[c] >= block) throw HipError("block colouring: adjacent blocks");
+                        hb[(size_t)k * n + p] = (unsigned char)(d / nbk[c]);
+                    }
+                }
+        bin.alloc((size_t)K * n);
+        bin.upload(hb.data(), (size_t)K * n, s);
+    }
 }
 
 // --------------------------------------------------------------- Krylov --
@@ -1183,6 +1415,7 @@ void Krylov::set_cases(const std::vector<int> &off) {
 // rows of case k in colour c are one contiguous range inside colour block c.
 void Krylov::build_segments(const Colouring *col, int n) {
     if (seg_built_ && seg_col_ == col && seg_n_ == n) return;
+    if (col && col->block > 1) throw HipError("batched cases: block-colour order not supported");
     const int nc = (int)case_off_.size() - 1;
     if (case_off_.back() != n) throw HipError("batched cases do not cover the system");
     seg_host_.clear();
@@ -1279,8 +1512,60 @@ double Krylov::relres(const Ell &A, const double *x, const double *b) {
 // later colour and the slot count is a constant; 0 otherwise.
 template <class T>
 static int az_rows(const EllT<T> &A, int pc, const Colouring *col) {
-    if (pc != CFD_PC_ILU || !col || col->ncolour < 2 || (A.K != 4 && A.K != 6)) return 0;
-    return col->off[1];
+    if (pc != CFD_PC_ILU || !col || col->ncolour < 2) return 0;
+    if (col->block > 1) return A.K <= 4 ? col->off[1] : 0;
+    return (A.K == 4 || A.K == 6) ? col->off[1] : 0;
+}
+
+// ILU(0) in block-colour order: a forward launch per colour but the last,
+// the last colour's forward and backward in one launch, then a backward
+// launch per colour from the second last down to colour 0 (which also
+// leaves az when asked). The launches that finish rows of y write the
+// partials of c.y.
+template <class T>
+void Krylov::apply_block_ilu(const EllT<T> &A, const T *c, T *y, Colouring *col, KWork<T> &w,
+                             int *rz_blocks, const KState *st, T *az) {
+    constexpr int B = kIluBlock;
+    if (col->block != B || A.K > 4) throw HipError("block-colour ILU(0): <= 4 slots, 6-row blocks");
+    const int n = A.n, nc = col->ncolour;
+    const unsigned char *nl = col->nlow.p, *bi = col->bin.p;
+    const T *lu = w.lu.p, *ud = w.ud.p;
+    int nbk = 0;
+    for (int k = 0; k < nc; ++k)
+        if (col->nbk[k]) nbk += grid8(col->nbk[k]);
+    double *rp = (rz_blocks && nbk <= kPartPQ - kPartRho) ? part_.p + kPartRho : nullptr;
+    int used = 0;
+    for (int k = 0; k < nc - 1; ++k) {
+        if (!col->nbk[k]) continue;
+        hipLaunchKernelGGL((k_bilu_fwd<T, 4, B>), dim3(grid8(col->nbk[k])), dim3(kBlock), 0, st_,
+                           n, A.K, A.col, bi, nl, lu, col->off[k], col->nbk[k], c,
+                           (const T *)w.tmp.p, w.tmp.p, st);
+    }
+    if (col->nbk[nc - 1]) {
+        const int g = grid8(col->nbk[nc - 1]);
+        hipLaunchKernelGGL((k_bilu_last<T, 4, B>), dim3(g), dim3(kBlock), 0, st_, n, A.K, A.col,
+                           bi, nl, lu, ud, col->off[nc - 1], col->nbk[nc - 1], c,
+                           (const T *)w.tmp.p, y, rp ? rp + used : (double *)nullptr, st);
+        if (rp) used += g;
+    }
+    for (int k = nc - 2; k >= 0; --k) {
+        if (!col->nbk[k]) continue;
+        const int g = grid8(col->nbk[k]);
+        double *rpp = rp ? rp + used : (double *)nullptr;
+        if (k == 0 && az && az_rows(A, CFD_PC_ILU, col) > 0)
+            hipLaunchKernelGGL((k_bilu_bwd<T, 4, B, true>), dim3(g), dim3(kBlock), 0, st_, n,
+                               A.K, A.col, bi, nl, lu, ud, (const T *)A.val, (const T *)A.diag,
+                               col->off[k], col->nbk[k], c, (const T *)w.tmp.p, (const T *)y, y,
+                               az, rpp, st);
+        else
+            hipLaunchKernelGGL((k_bilu_bwd<T, 4, B, false>), dim3(g), dim3(kBlock), 0, st_, n,
+                               A.K, A.col, bi, nl, lu, ud, (const T *)A.val, (const T *)A.diag,
+                               col->off[k], col->nbk[k], c, (const T *)w.tmp.p, (const T *)y, y,
+                               (T *)nullptr, rpp, st);
+        if (rp) used += g;
+    }
+    if (rz_blocks && rp) *rz_blocks = used;
+    CFD_LAUNCH_CHECK();
 }
 
 template <class T>
@@ -1292,15 +1577,22 @@ void Krylov::setup_pc(const EllT<T> &A, int pc, Colouring *col, KWork<T> &w) {
     CFD_HIP(hipMemcpyAsync(w.lu.p, A.val, sizeof(T) * (size_t)A.K * n, hipMemcpyDeviceToDevice,
                            st_));
     if (col->triangle_free) {  // the diagonal and the lower factor l = a / u
-        for (int c = 0; c < col->ncolour; ++c) {
-            int r0 = col->off[c], r1 = col->off[c + 1];
-            if (r1 > r0)
-                hipLaunchKernelGGL(k_dilu_factor<T>, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, st_,
-                                   n, A.K, A.col, (const T *)A.val, r0, r1, w.ud.p, w.lu.p);
-        }
+        // rows that do not depend on each other: one colour, or in block-colour
+        // order row j of every block of one colour
+        const int parts = col->block;
+        for (int c = 0; c < col->ncolour; ++c)
+            for (int j = 0; j < parts; ++j) {
+                const int len = (col->off[c + 1] - col->off[c]) / parts;
+                const int r0 = col->off[c] + j * len, r1 = r0 + len;
+                if (r1 > r0)
+                    hipLaunchKernelGGL(k_dilu_factor<T>, dim3(grid_for(r1 - r0)), dim3(kBlock), 0,
+                                       st_, n, A.K, A.col, (const T *)A.val, r0, r1, w.ud.p,
+                                       w.lu.p);
+            }
         CFD_LAUNCH_CHECK();
         return;
     }
+    if (col->block > 1) throw HipError("block-colour ILU(0) needs a triangle-free graph");
     for (int c = 0; c < col->ncolour; ++c) {
         int r0 = col->off[c], r1 = col->off[c + 1];
         if (r1 > r0)
@@ -1322,6 +1614,7 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
         hipLaunchKernelGGL(k_jacobi<T>, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
                            (const T *)A.diag, c, y);
     } else if (pc == CFD_PC_SSOR) {
+        if (col->block > 1) throw HipError("SSOR sweeps need the plain multicolour order");
         const int nc = col->ncolour;
         const int c0end = nc > 1 ? col->off[1] : n;
         int nbk = 0;
@@ -1365,6 +1658,8 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
             if (rp) used += g;
         }
         if (rz_blocks && rp) *rz_blocks = used;
+    } else if (col->block > 1) {
+        apply_block_ilu(A, c, y, col, w, rz_blocks, st, az);
     } else {
         const int nc = col->ncolour;
         const int c0end = nc > 1 ? col->off[1] : n;
@@ -1638,7 +1933,8 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
             (uintptr_t)w.q.p, (uintptr_t)w.tmp.p, (uintptr_t)w.ud.p, (uintptr_t)w.lu.p,
             (uintptr_t)w.az.p,
             (uintptr_t)n, (uintptr_t)A.K, (uintptr_t)pc, (uintptr_t)col,
-            (uintptr_t)(col ? col->ncolour : 0), (uintptr_t)(col ? col->triangle_free : 0)};
+            (uintptr_t)(col ? col->ncolour : 0), (uintptr_t)(col ? col->triangle_free : 0),
+            (uintptr_t)(col ? col->block : 0)};
         uint64_t om;
         std::memcpy(&om, &cfg.omega, sizeof om);
         key.push_back((uintptr_t)om);
@@ -1775,7 +2071,7 @@ SolveStats Krylov::bicgstab(const Ell &A, const double *b, double *x, const cfd_
             (uintptr_t)wd_.lu.p, (uintptr_t)v_.p, (uintptr_t)ph_.p, (uintptr_t)s_.p,
             (uintptr_t)sh_.p, (uintptr_t)t_.p, (uintptr_t)rt_.p, (uintptr_t)n, (uintptr_t)A.K,
             (uintptr_t)pc, (uintptr_t)col, (uintptr_t)(col ? col->ncolour : 0),
-            (uintptr_t)(col ? col->triangle_free : 0),
+            (uintptr_t)(col ? col->triangle_free : 0), (uintptr_t)(col ? col->block : 0),
             (uintptr_t)(col ? (const void *)col->nlow.p : nullptr)};
         uint64_t om;
         std::memcpy(&om, &cfg.omega, sizeof om);

@@ -465,11 +465,88 @@ int concat_meshes(const std::vector<HostMesh> &parts, HostMesh &o, std::vector<i
         off[2].push_back(o.nfac);
         off[3].push_back(o.nbnd);
     }
+    // the union's cells need not come in whole blocks: greedy colouring only
+    o.bcolour.assign(o.ncel, -1);
+    o.nbcolour = 0;
     return 0;
 }
 
+namespace {
+// Block colouring for the ILU(0) order (the product's own ordering; the
+// reference factors in natural order). Defined for tetrahedral meshes (every
+// cell has 4 faces) whose cells come in whole blocks of kIluBlock
+// consecutive cells and whose adjacency has no triangles (no two neighbours
+// of a cell are neighbours of each other). Blocks are coloured so that no
+// two blocks of one colour share a face: by breadth-first parity when the
+// block graph is bipartite (each connected component from its lowest block,
+// colour 0), otherwise greedy first-fit in block order. Otherwise -1.
+void colour_blocks(HostMesh &m) {
+    const int n = m.ncel, B = kIluBlock;
+    m.bcolour.assign(n, -1);
+    m.nbcolour = 0;
+    if (n == 0 || n % B) return;
+    auto nbr = [&](int i, int a) { return m.cell_cell[6 * (size_t)i + a]; };
+    for (int i = 0; i < n; ++i)
+        if (m.cell_nface[i] != 4) return;
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 4; ++a) {
+            const int j = nbr(i, a);
+            if (j < 0) continue;
+            for (int b = 0; b < 4; ++b) {
+                const int k = nbr(j, b);
+                if (k < 0 || k == i) continue;
+                for (int c = 0; c < 4; ++c)
+                    if (nbr(i, c) == k) return;  // triangle i, j, k
+            }
+        }
+    const int nb = n / B;
+    std::vector<std::vector<int>> badj(nb);
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 4; ++a) {
+            const int j = nbr(i, a);
+            if (j >= 0 && j / B != i / B) badj[i / B].push_back(j / B);
+        }
+    std::vector<int> bc(nb, -1), queue;
+    bool bipartite = true;
+    for (int s = 0; s < nb && bipartite; ++s) {
+        if (bc[s] >= 0) continue;
+        bc[s] = 0;
+        queue.assign(1, s);
+        for (size_t h = 0; h < queue.size() && bipartite; ++h) {
+            const int u = queue[h];
+            for (int v : badj[u]) {
+                if (bc[v] < 0) {
+                    bc[v] = 1 - bc[u];
+                    queue.push_back(v);
+                } else if (bc[v] == bc[u]) {
+                    bipartite = false;
+                }
+            }
+        }
+    }
+    if (!bipartite) {
+        std::vector<char> used;
+        for (int b = 0; b < nb; ++b) {
+            used.assign(badj[b].size() + 2, 0);
+            for (int v : badj[b])
+                if (v < b && bc[v] < (int)used.size()) used[bc[v]] = 1;
+            int c = 0;
+            while (used[c]) ++c;
+            bc[b] = c;
+        }
+    }
+    int maxc = 0;
+    for (int i = 0; i < n; ++i) {
+        m.bcolour[i] = bc[i / B];
+        maxc = std::max(maxc, m.bcolour[i]);
+    }
+    m.nbcolour = maxc + 1;
+}
+}  // namespace
+
 // Greedy first-fit colouring over the interior-face adjacency, natural
-// order: colour(i) = least colour not used by a neighbour j < i.
+// order: colour(i) = least colour not used by a neighbour j < i. Then the
+// block colouring of the ILU(0) order (colour_blocks).
 void colour_cells(HostMesh &m) {
     m.colour.assign(m.ncel, 0);
     int maxc = 0;
@@ -485,6 +562,7 @@ void colour_cells(HostMesh &m) {
         maxc = std::max(maxc, c);
     }
     m.ncolour = maxc + 1;
+    colour_blocks(m);
 }
 
 }  // namespace cfd
@@ -781,6 +859,7 @@ int mesh_get_int(const HostMesh &m, const char *name, int *out) {
     else if (s == "bnd_face") v = &m.bnd_face;
     else if (s == "bnd_rid") v = &m.bnd_rid;
     else if (s == "cell_colour") v = &m.colour;
+    else if (s == "cell_bcolour") v = &m.bcolour;
     if (!v) { last_error() = "unknown int array " + s; return -1; }
     std::memcpy(out, v->data(), sizeof(int) * v->size());
     return 0;

@@ -26,7 +26,15 @@ struct HostMesh {
     std::vector<double> bnd_dist;
     std::vector<int> colour;                  // greedy first-fit over cells
     int ncolour = 0;
+    // colour of each cell's block of kIluBlock consecutive cells for the
+    // ILU(0) order (colour_cells), -1 everywhere when the mesh has none
+    std::vector<int> bcolour;
+    int nbcolour = 0;
 };
+
+// cells per block of the ILU(0) block-colour order (the 6 tetrahedra of a
+// split hexahedron)
+constexpr int kIluBlock = 6;
 
 int load_raw(const cfd_raw_mesh &raw, HostMesh &m, std::string &err);
 int build_faces(HostMesh &m, std::string &err);

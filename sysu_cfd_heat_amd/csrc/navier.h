@@ -43,8 +43,8 @@ class HTSolver {
     HostMesh hm;
     DevMeshOwner dmo;
     std::vector<int> breg;
-    Colouring colour;
-    bool colour_ready = false;
+    Colouring colour, bcolour;  // bcolour: ILU(0) block-colour order
+    bool colour_ready = false, bcolour_ready = false;
     double URF = 1.0;  // src/Preprocessor.cpp:506
     DBuf<double> T, R, D, g, BT, BndQ, xsol, bs, src, diag, val, gtmp;
     DBuf<double> Tnp, ApE;  // transient runs
@@ -82,8 +82,8 @@ class NSSolver {
     std::unique_ptr<HTSolver> ht;
     hipStream_t s = nullptr;
     std::unique_ptr<Krylov> kry;
-    Colouring colour;
-    bool colour_ready = false;
+    Colouring colour, bcolour;  // bcolour: ILU(0) block-colour order
+    bool colour_ready = false, bcolour_ready = false;
 
     DBuf<double> R, U, V, W, P_, T, TE, ED, VisLam, VisTur, Apr, RUFace;
     DBuf<double> gU, gV, gW, gP, gPhi;  // [3][C]

@@ -422,13 +422,33 @@ int NSSolver::InitFlowField() {
     return 0;
 }
 
-Colouring *NSSolver::colouring_for(const cfd_solve_cfg &c) {
-    if (c.precond != CFD_PC_SSOR && c.precond != CFD_PC_ILU) return nullptr;
+namespace {
+// The ordering a solve with preconditioner pc sweeps in: ILU(0) takes the
+// block-colour order where the mesh has one (hm.bcolour), SSOR and the
+// other meshes the greedy cell colouring; each is built once.
+Colouring *colouring_of(const HostMesh &hm, const DevMeshOwner &dmo, int pc, Colouring &colour,
+                        bool &colour_ready, Colouring &bcolour, bool &bcolour_ready,
+                        hipStream_t s) {
+    if (pc != CFD_PC_SSOR && pc != CFD_PC_ILU) return nullptr;
+    if (pc == CFD_PC_ILU && !hm.bcolour.empty() && hm.bcolour[0] >= 0 && dmo.dm.K <= 4) {
+        if (!bcolour_ready) {
+            bcolour.build(dmo.ell_col_host.data(), hm.ncel, dmo.dm.K, hm.bcolour.data(),
+                          dmo.dm.ell_col, s, kIluBlock);
+            bcolour_ready = true;
+        }
+        return &bcolour;
+    }
     if (!colour_ready) {
-        colour.build(dmo.ell_col_host.data(), hm.ncel, dmo.dm.K, hm.colour.data(), dmo.dm.ell_col, s);
+        colour.build(dmo.ell_col_host.data(), hm.ncel, dmo.dm.K, hm.colour.data(), dmo.dm.ell_col,
+                     s);
         colour_ready = true;
     }
     return &colour;
+}
+}  // namespace
+
+Colouring *NSSolver::colouring_for(const cfd_solve_cfg &c) {
+    return colouring_of(hm, dmo, c.precond, colour, colour_ready, bcolour, bcolour_ready, s);
 }
 
 SolveStats NSSolver::solve(const Ell &A, const double *b, double *x, const cfd_solve_cfg &c,
@@ -822,15 +842,8 @@ int HTSolver::UpdateEnergy() {
     }
     BuildScalarMatrix();
     CFD_HIP(hipMemcpyAsync(xsol, T, sizeof(double) * hm.ncel, hipMemcpyDeviceToDevice, s));
-    Colouring *col = nullptr;
-    if (ns->P.solid.precond == CFD_PC_SSOR || ns->P.solid.precond == CFD_PC_ILU) {
-        if (!colour_ready) {
-            colour.build(dmo.ell_col_host.data(), hm.ncel, dmo.dm.K, hm.colour.data(),
-                         dmo.dm.ell_col, s);
-            colour_ready = true;
-        }
-        col = &colour;
-    }
+    Colouring *col = colouring_of(hm, dmo, ns->P.solid.precond, colour, colour_ready, bcolour,
+                                  bcolour_ready, s);
     Ell A{hm.ncel, dmo.dm.K, diag.p, val.p, dmo.dm.ell_col};
     last = kry->solve(A, bs, xsol, ns->P.solid, col);
     ns->stats[6] += last.iters;
@@ -1135,14 +1148,15 @@ int cfd_ns_profile_read(const cfd_ns *h, double out[8]) {
     // with ILU(0) over two or more colours (4 or 6 slots), colour-0 rows take
     // (A z)_i from the preconditioner's last sweep instead of their row
     const int K = n.dmo.dm.K;
-    const bool az = n.P.pres.precond == CFD_PC_ILU && n.colour_ready &&
-                    n.colour.ncolour > 1 && (K == 4 || K == 6);
+    const Colouring &pc = n.bcolour_ready ? n.bcolour : n.colour;  // ILU's order
+    const bool az = n.P.pres.precond == CFD_PC_ILU && (n.colour_ready || n.bcolour_ready) &&
+                    pc.ncolour > 1 && (pc.block > 1 ? K <= 4 : (K == 4 || K == 6));
     long long nnz = 0, nnz_read = 0;
     for (size_t k = 0; k < n.dmo.ell_col_host.size(); ++k) {
         const int row = (int)(k % n.hm.ncel);
         if (n.dmo.ell_col_host[k] == row) continue;
         ++nnz;
-        if (!(az && n.colour.colour_host[row] == 0)) ++nnz_read;
+        if (!(az && pc.colour_host[row] == 0)) ++nnz_read;
     }
     // k_cg_pq per launch (each timed launch applies the previous iteration's
     // x update): reads z, p, q, x and the diagonal or az_i (5 words per row),

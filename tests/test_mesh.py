@@ -25,7 +25,7 @@ def test_product_matches_oracle_bit_exact(name, oracle):
     a, b = P.Mesh(raw), oracle.Mesh(raw)
     assert a.counts == b.counts
     for nm in list(P.INT_ARRAYS) + list(P.DBL_ARRAYS):
-        if nm == "cell_colour":
+        if nm in ("cell_colour", "cell_bcolour"):
             continue
         x, y = a.get(nm), b.get(nm)
         if nm in ("face_xpac", "face_xnac"):  # undefined on boundary faces
@@ -152,6 +152,48 @@ def test_colouring_is_proper_first_fit(oracle):
     assert np.array_equal(col, oracle.colour(adj))
 
 
+@pytest.mark.parametrize("dims", [(4, 3, 3), (5, 2, 3)])
+def test_block_colouring_of_split_hexahedra(oracle, dims):
+    """The ILU(0) block-colour order: the 6 tetrahedra of each split
+    hexahedron form one block, and the hexahedra's face graph is bipartite,
+    so the blocks take the parity (i + j + k) mod 2 of their hexahedron
+    (the component is rooted at hexahedron 0 with colour 0). The oracle's
+    restatement agrees bit for bit; no two adjacent cells of different
+    blocks share a colour."""
+    nx, ny, nz = dims
+    m = P.Mesh(P.box_mesh(nx, ny, nz, tets=True, jitter=0.1, seed=3))
+    bc, cc, nf = m.get("cell_bcolour"), m.get("cell_cell"), m.get("cell_nface")
+    n = m.counts[1]
+    assert n == 6 * nx * ny * nz
+    x = m.get("cell_x")
+    h = np.array([1.0 / nx, 1.0 / ny, 1.0 / nz])
+    ijk = np.floor(x / h).astype(int)  # the hexahedron of each tetrahedron
+    assert np.array_equal(ijk[::6].repeat(6, axis=0), ijk)  # blocks are hexahedra
+    assert np.array_equal(bc, ijk.sum(axis=1) % 2)
+    adj = [[j for j in cc[i] if j >= 0] for i in range(n)]
+    for i, a in enumerate(adj):
+        for j in a:
+            if j // 6 != i // 6:
+                assert bc[i] != bc[j]
+    assert np.array_equal(bc, oracle.block_colour(nf, adj))
+
+
+def test_block_colouring_absent_where_undefined(oracle):
+    """Hexahedra (6 faces), or a triangle in the adjacency, or cells not in
+    whole blocks: no block order (-1), in the product and the oracle alike."""
+    m = P.Mesh(P.box_mesh(3, 3, 3))
+    assert (m.get("cell_bcolour") == -1).all()
+    # a planted triangle: three 4-face cells, pairwise adjacent, six per block
+    nf = np.full(6, 4)
+    adj = [[1, 2], [0, 2], [0, 1], [4], [3], []]
+    assert (oracle.block_colour(nf, adj) == -1).all()
+    # no triangle: blocks 0 and 1 adjacent through cells 5 and 6
+    nf = np.full(12, 4)
+    adj = [[1], [0], [], [], [], [6], [5], [], [], [], [], []]
+    assert np.array_equal(oracle.block_colour(nf, adj), [0] * 6 + [1] * 6)
+    assert (oracle.block_colour(nf[:11], adj[:11]) == -1).all()  # 11 cells
+
+
 def test_cht_split_matches_oracle(oracle):
     raw, p = cases.channel(nx=6, ny=4, nz=3, tets=True, jitter=0.1, seed=4)
     for part in (0, 1):
@@ -159,7 +201,7 @@ def test_cht_split_matches_oracle(oracle):
         ns = oracle.NSSolver(raw, p)
         assert a.counts == ns.counts(part)
         for nm in list(P.INT_ARRAYS) + list(P.DBL_ARRAYS):
-            if nm in ("cell_colour",):
+            if nm in ("cell_colour", "cell_bcolour"):
                 continue
             x, y = a.get(nm), ns.mesh_get(nm, part)
             if nm in ("face_xpac", "face_xnac"):

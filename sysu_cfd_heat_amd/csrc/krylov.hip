@@ -963,9 +963,11 @@ struct BlockLoad {
     bool use[B][KK];
     int nl[B];
     T a[B][KK], g[B][KK];
-    // aval (backward only): A's values into the lower slots' a (not used by
-    // the recursion; for (A y)_p)
-    template <bool FWD>
+    // MODE 0: the lower slots (forward), 1: the upper ones (backward; aval:
+    // A's values into the lower slots' a, not used by the recursion, for
+    // (A y)_p), 2: both (the last colour, which has no later colour: only
+    // lower slots gather, from other; padding slots keep a = 0)
+    template <int MODE>
     __device__ __forceinline__ void load(int n, int K, const int *__restrict__ col,
                                          const unsigned char *__restrict__ bin,
                                          const unsigned char *__restrict__ nlow,
@@ -974,13 +976,18 @@ struct BlockLoad {
                                          const T *__restrict__ aval = nullptr) {
         int cl[B][KK];
 #pragma unroll
+        for (int j = 0; j < B; ++j) nl[j] = nlow[base + j * nb];
+        // MODE 0 reads the lower slots' columns only (nlow is nearly uniform
+        // over row j of a colour's blocks, so the other slots' lines are not
+        // fetched)
+#pragma unroll
         for (int j = 0; j < B; ++j) {
             const int p = base + j * nb;
-            nl[j] = nlow[p];
 #pragma unroll
             for (int k = 0; k < KK; ++k) {
-                cl[j][k] = k < K ? col[(size_t)k * n + p] : p;
-                bj[j][k] = k < K ? bin[(size_t)k * n + p] : Colouring::kNotInBlock;
+                const bool rd = MODE == 0 ? k < nl[j] : k < K;
+                cl[j][k] = rd ? col[(size_t)k * n + p] : p;
+                bj[j][k] = rd ? bin[(size_t)k * n + p] : Colouring::kNotInBlock;
             }
         }
 #pragma unroll
@@ -988,20 +995,24 @@ struct BlockLoad {
             const int p = base + j * nb;
 #pragma unroll
             for (int k = 0; k < KK; ++k) {
-                use[j][k] = FWD ? k < nl[j] : (k >= nl[j] && cl[j][k] > p);
+                use[j][k] = MODE == 0 ? k < nl[j]
+                            : MODE == 1 ? (k >= nl[j] && cl[j][k] > p) : k < K;
                 a[j][k] = use[j][k] ? lu[(size_t)k * n + p]
-                          : (!FWD && aval && k < nl[j]) ? aval[(size_t)k * n + p] : T(0);
-                g[j][k] = (use[j][k] && bj[j][k] == Colouring::kNotInBlock) ? other[cl[j][k]]
-                                                                            : T(0);
+                          : (MODE == 1 && aval && k < nl[j]) ? aval[(size_t)k * n + p] : T(0);
+                const bool gather = MODE == 2 ? k < nl[j] : use[j][k];
+                g[j][k] = (gather && bj[j][k] == Colouring::kNotInBlock) ? other[cl[j][k]]
+                                                                         : T(0);
             }
         }
     }
     // acc minus a * value over the selected slots of row j, in slot order
-    // (same-block values from the LDS column sv[.][tid])
+    // (same-block values from the LDS column sv[.][tid]); MODE 2: the lower
+    // slots (LOWER) or the upper ones (padding slots subtract 0 * 0)
+    template <int SEL = 0>
     __device__ __forceinline__ T sub(int j, T acc, const T (*sv)[kBlock], int tid) const {
 #pragma unroll
         for (int k = 0; k < KK; ++k)
-            if (use[j][k])
+            if (SEL == 0 ? use[j][k] : SEL == 1 ? k < nl[j] : (k >= nl[j] && use[j][k]))
                 acc -= a[j][k] *
                        (bj[j][k] != Colouring::kNotInBlock ? sv[bj[j][k]][tid] : g[j][k]);
         return acc;
@@ -1024,7 +1035,7 @@ __global__ __launch_bounds__(256) void k_bilu_fwd(int n, int K, const int *__res
     if (t >= nb) return;
     const int base = r0 + t;
     BlockLoad<T, KK, B> L;
-    L.template load<true>(n, K, col, bin, nlow, lu, base, nb, tin);
+    L.template load<0>(n, K, col, bin, nlow, lu, base, nb, tin);
     T cv[B];
 #pragma unroll
     for (int j = 0; j < B; ++j) cv[j] = c[base + j * nb];
@@ -1053,22 +1064,18 @@ __global__ __launch_bounds__(256) void k_bilu_last(int n, int K, const int *__re
     if (t < nb) {
         const int base = r0 + t;
         T cv[B], dv[B];
-        {
-            BlockLoad<T, KK, B> L;
-            L.template load<true>(n, K, col, bin, nlow, lu, base, nb, tin);
+        BlockLoad<T, KK, B> L;  // every slot: the lower ones forward, the upper backward
+        L.template load<2>(n, K, col, bin, nlow, lu, base, nb, tin);
 #pragma unroll
-            for (int j = 0; j < B; ++j) {
-                cv[j] = c[base + j * nb];
-                dv[j] = ud[base + j * nb];
-            }
-#pragma unroll
-            for (int j = 0; j < B; ++j) sv[j][tid] = L.sub(j, cv[j], sv, tid);
+        for (int j = 0; j < B; ++j) {
+            cv[j] = c[base + j * nb];
+            dv[j] = ud[base + j * nb];
         }
-        BlockLoad<T, KK, B> U;
-        U.template load<false>(n, K, col, bin, nlow, lu, base, nb, y);
+#pragma unroll
+        for (int j = 0; j < B; ++j) sv[j][tid] = L.template sub<1>(j, cv[j], sv, tid);
 #pragma unroll
         for (int j = B - 1; j >= 0; --j) {
-            const T yv = U.sub(j, sv[j][tid], sy, tid) / dv[j];
+            const T yv = L.template sub<2>(j, sv[j][tid], sy, tid) / dv[j];
             sy[j][tid] = yv;
             y[base + j * nb] = yv;
             rz[0] += (double)cv[j] * yv;
@@ -1099,7 +1106,7 @@ __global__ __launch_bounds__(256) void k_bilu_bwd(
     if (t < nb) {
         const int base = r0 + t;
         BlockLoad<T, KK, B> U;
-        U.template load<false>(n, K, col, bin, nlow, lu, base, nb, yin, AZ ? aval : nullptr);
+        U.template load<1>(n, K, col, bin, nlow, lu, base, nb, yin, AZ ? aval : nullptr);
         T tv[B], dv[B], dg[B];
 #pragma unroll
         for (int j = 0; j < B; ++j) {

@@ -426,11 +426,12 @@ namespace {
 // The ordering a solve with preconditioner pc sweeps in: ILU(0) takes the
 // block-colour order where the mesh has one (hm.bcolour), SSOR and the
 // other meshes the greedy cell colouring; each is built once.
-Colouring *colouring_of(const HostMesh &hm, const DevMeshOwner &dmo, int pc, Colouring &colour,
-                        bool &colour_ready, Colouring &bcolour, bool &bcolour_ready,
-                        hipStream_t s) {
+Colouring *colouring_of(const HostMesh &hm, const DevMeshOwner &dmo, int pc, bool block,
+                        Colouring &colour, bool &colour_ready, Colouring &bcolour,
+                        bool &bcolour_ready, hipStream_t s) {
     if (pc != CFD_PC_SSOR && pc != CFD_PC_ILU) return nullptr;
-    if (pc == CFD_PC_ILU && !hm.bcolour.empty() && hm.bcolour[0] >= 0 && dmo.dm.K <= 4) {
+    if (block && pc == CFD_PC_ILU && !hm.bcolour.empty() && hm.bcolour[0] >= 0 &&
+        dmo.dm.K <= 4) {
         if (!bcolour_ready) {
             bcolour.build(dmo.ell_col_host.data(), hm.ncel, dmo.dm.K, hm.bcolour.data(),
                           dmo.dm.ell_col, s, kIluBlock);
@@ -448,7 +449,8 @@ Colouring *colouring_of(const HostMesh &hm, const DevMeshOwner &dmo, int pc, Col
 }  // namespace
 
 Colouring *NSSolver::colouring_for(const cfd_solve_cfg &c) {
-    return colouring_of(hm, dmo, c.precond, colour, colour_ready, bcolour, bcolour_ready, s);
+    return colouring_of(hm, dmo, c.precond, ilu_block, colour, colour_ready, bcolour,
+                        bcolour_ready, s);
 }
 
 SolveStats NSSolver::solve(const Ell &A, const double *b, double *x, const cfd_solve_cfg &c,
@@ -842,8 +844,8 @@ int HTSolver::UpdateEnergy() {
     }
     BuildScalarMatrix();
     CFD_HIP(hipMemcpyAsync(xsol, T, sizeof(double) * hm.ncel, hipMemcpyDeviceToDevice, s));
-    Colouring *col = colouring_of(hm, dmo, ns->P.solid.precond, colour, colour_ready, bcolour,
-                                  bcolour_ready, s);
+    Colouring *col = colouring_of(hm, dmo, ns->P.solid.precond, ns->ilu_block, colour,
+                                  colour_ready, bcolour, bcolour_ready, s);
     Ell A{hm.ncel, dmo.dm.K, diag.p, val.p, dmo.dm.ell_col};
     last = kry->solve(A, bs, xsol, ns->P.solid, col);
     ns->stats[6] += last.iters;
@@ -1148,8 +1150,9 @@ int cfd_ns_profile_read(const cfd_ns *h, double out[8]) {
     // with ILU(0) over two or more colours (4 or 6 slots), colour-0 rows take
     // (A z)_i from the preconditioner's last sweep instead of their row
     const int K = n.dmo.dm.K;
-    const Colouring &pc = n.bcolour_ready ? n.bcolour : n.colour;  // ILU's order
-    const bool az = n.P.pres.precond == CFD_PC_ILU && (n.colour_ready || n.bcolour_ready) &&
+    const bool blk = n.ilu_block && n.bcolour_ready;
+    const Colouring &pc = blk ? n.bcolour : n.colour;  // ILU's order
+    const bool az = n.P.pres.precond == CFD_PC_ILU && (blk || n.colour_ready) &&
                     pc.ncolour > 1 && (pc.block > 1 ? K <= 4 : (K == 4 || K == 6));
     long long nnz = 0, nnz_read = 0;
     for (size_t k = 0; k < n.dmo.ell_col_host.size(); ++k) {
@@ -1190,6 +1193,10 @@ int cfd_ns_set_option(cfd_ns *h, const char *name, double value) {
         set_autotune(value != 0.);
         return 0;
     }
+    if (k == "ilu_block") {
+        h->ns.ilu_block = value != 0.;
+        return 0;
+    }
     if (k == "mixed_inner_tol") {
         if (!(value > 0. && value < 1.)) {
             last_error() = "mixed_inner_tol must lie in (0, 1)";
@@ -1209,6 +1216,7 @@ double cfd_ns_get_option(const cfd_ns *h, const char *name) {
     if (k == "overlap_solid") return h->ns.overlap_solid ? 1. : 0.;
     if (k == "mixed_inner_tol") return h->ns.mixed_inner_tol;
     if (k == "autotune") return get_autotune() ? 1. : 0.;
+    if (k == "ilu_block") return h->ns.ilu_block ? 1. : 0.;
     if (k == "graph_launches") return (double)h->ns.kry->graph_launches;
     last_error() = "unknown option " + k;
     return std::nan("");

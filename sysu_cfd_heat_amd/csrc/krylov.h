@@ -115,9 +115,10 @@ class Krylov {
     bool use_graphs = true;
     long long graph_launches = 0;
 
-    // Live timing of the CG SpMV kernel (k_cg_pq): HIP events around one
-    // launch in eight (iterations 1, 9, 17, ...) on this solver's stream;
-    // only launches that did work count.
+    // Live timing of the CG SpMV kernel (k_cg_pq): HIP events attached to
+    // one launch in eight (iterations 1, 9, 17, ...; hipExtLaunchKernelGGL
+    // records the kernel's start and end) on this solver's stream; only
+    // launches that did work count.
     bool prof = false;
     double prof_ms = 0.;
     long long prof_launches = 0;

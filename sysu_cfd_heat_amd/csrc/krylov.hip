@@ -19,6 +19,8 @@
 #include <numeric>
 #include <type_traits>
 
+#include <hip/hip_ext.h>
+
 #include "krylov.h"
 #include "mesh.h"
 
@@ -1326,8 +1328,9 @@ Krylov::~Krylov() {
 // iterations per long graph of a Krylov loop (a batch of b iterations is
 // b / kGraphIters launches of it and the rest of the one-iteration graph)
 constexpr int kGraphIters = 8;
-// live timing of k_cg_pq: one launch in kProfEvery between HIP events (every
-// launch moves the same bytes; timing each would cost 3 % of the run)
+// live timing of k_cg_pq: HIP events attached to one launch in kProfEvery
+// (every launch moves the same bytes; separate event records around every
+// launch cost 3 % of the run)
 constexpr int kProfEvery = 8;
 
 // Executable graph of the launches rec() makes on this stream, captured on
@@ -1972,13 +1975,17 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
             // live timing samples iterations 1, 1 + kProfEvery, ... (iteration
             // 0's launch has no x update to apply)
             const bool timed = prof && (issued + k) % kProfEvery == 1;
-            if (timed) CFD_HIP(hipEventRecord(ev(2 * nev), st_));
-            hipLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, A, (const T *)w.z.p,
-                               w.p0.p, w.q.p, x, (const T *)w.az.p, c0, state_.p, pqpart);
-            if (timed) {
-                CFD_HIP(hipEventRecord(ev(2 * nev + 1), st_));
+            if (timed) {  // the events are part of the launch: kernel start to end
+                hipExtLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, ev(2 * nev),
+                                      ev(2 * nev + 1), 0u, A, (const T *)w.z.p, w.p0.p, w.q.p,
+                                      x, (const T *)w.az.p, c0, (const KState *)state_.p,
+                                      pqpart);
                 ev_iter.push_back(issued + k);
                 ++nev;
+            } else {
+                hipLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, A,
+                                   (const T *)w.z.p, w.p0.p, w.q.p, x, (const T *)w.az.p, c0,
+                                   state_.p, pqpart);
             }
             rest_of_iteration();
         }

@@ -956,9 +956,9 @@ __global__ void k_ilu_bwd0_az(int n, const int *__restrict__ col, const T *__res
 // One block's rows, every load issued before the recursion through the
 // block (which then waits on registers and LDS only): per row j and slot k
 // the column, the same-block row (bin), the factor value and, for a
-// neighbour of another colour, its gathered value. FWD selects the lower
-// slots, BWD the upper ones (padding slots, which hold the row itself, are
-// neither).
+// neighbour of another colour, its gathered value. load<MODE> selects the
+// slots (padding slots, which hold the row itself, are neither lower nor
+// upper).
 template <class T, int KK, int B>
 struct BlockLoad {
     unsigned char bj[B][KK];
@@ -1007,9 +1007,10 @@ struct BlockLoad {
             }
         }
     }
-    // acc minus a * value over the selected slots of row j, in slot order
-    // (same-block values from the LDS column sv[.][tid]); MODE 2: the lower
-    // slots (LOWER) or the upper ones (padding slots subtract 0 * 0)
+    // acc minus a * value over slots of row j, in slot order (same-block
+    // values from the LDS column sv[.][tid]). SEL 0: the slots load
+    // selected; after a MODE 2 load, SEL 1: the lower slots, SEL 2: the
+    // upper ones (its padding slots subtract 0 * 0)
     template <int SEL = 0>
     __device__ __forceinline__ T sub(int j, T acc, const T (*sv)[kBlock], int tid) const {
 #pragma unroll

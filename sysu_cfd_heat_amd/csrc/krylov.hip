@@ -185,18 +185,52 @@ __global__ void k_add_f(int n, double *__restrict__ x, const float *__restrict__
 // Rows below c0 take (A z)_i from az, where the preconditioner's last sweep
 // left it (same sum, same order: see k_ilu_bwd0_az), instead of reading the
 // row and gathering z again.
+// Block-colour order (RowMap with colours, every colour's block count a
+// multiple of 8): XCD x's workgroups take the x-th eighth of every colour's
+// blocks, all their rows, so that the z they gather (rows of the blocks'
+// neighbours) is mostly what the same XCD's workgroups wrote in the last
+// sweep (whose launches give XCD x the x-th eighth of the blocks too).
+struct RowMap {
+    int ncol = 0;               // 0: the plain XCD-aware grid-stride loop
+    int off[5] = {0, 0, 0, 0, 0};
+    int nb[4] = {0, 0, 0, 0};   // blocks of each colour
+};
+// row of the XCD's lx-th row (x: the XCD)
+__device__ __forceinline__ int map_row(const RowMap &m, int x, int lx) {
+    int c = 0, base = 0, end = 0;  // end: where the XCD's rows of colour k end
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        end += kIluBlock * (m.nb[k] >> 3);
+        if (k + 1 < m.ncol && lx >= end) {
+            base = end;
+            c = k + 1;
+        }
+    }
+    const int nbx = m.nb[c] >> 3, rem = lx - base, j = rem / nbx;
+    return m.off[c] + j * m.nb[c] + x * nbx + (rem - j * nbx);
+}
+
 template <class T>
 __global__ void k_cg_pq(EllT<T> A, const T *__restrict__ z, T *__restrict__ p, T *__restrict__ q,
-                        T *__restrict__ x, const T *__restrict__ az, int c0, const KState *st,
-                        double *part) {
+                        T *__restrict__ x, const T *__restrict__ az, int c0, RowMap rm,
+                        const KState *st, double *part) {
     if (st->done) return;
     const T beta = (T)st->beta, alpha = (T)st->alpha;
     const bool upd = st->iter > 0;
-    const int n = A.n, stride = gridDim.x * blockDim.x;
+    const int n = A.n;
+    // plain: rows l of the XCD-aware grid stride; mapped: XCD (blockIdx % 8)
+    // strides over its own rows
+    const bool mapped = rm.ncol > 0;
+    const int xc = blockIdx.x & 7;
+    const int nl = mapped ? n >> 3 : n;  // every colour's rows split evenly over 8 XCDs
+    const int stride = mapped ? (gridDim.x >> 3) * blockDim.x : gridDim.x * blockDim.x;
+    const int l0 = mapped ? (blockIdx.x >> 3) * blockDim.x + threadIdx.x
+                          : xcd_block() * blockDim.x + threadIdx.x;
     double acc[1] = {0.};
-    for (int i = xcd_block() * blockDim.x + threadIdx.x; i < n; i += 2 * stride) {
-        const int j = i + stride;
-        const bool hj = j < n;
+    for (int li = l0; li < nl; li += 2 * stride) {
+        const int i = mapped ? map_row(rm, xc, li) : li, lj = li + stride;
+        const bool hj = lj < nl;
+        const int j = !hj ? 0 : mapped ? map_row(rm, xc, lj) : lj;
         const T zi = z[i], zj = hj ? z[j] : T(0);
         T si = i < c0 ? az[i] : row_sum(A, i, z, A.diag[i] * zi);
         T sj = hj ? (j < c0 ? az[j] : row_sum(A, j, z, A.diag[j] * zj)) : T(0);
@@ -1885,6 +1919,16 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
     const int n = A.n, G = red_grid(n), Gs = spmv_grid(n);
     const bool jac = pc == CFD_PC_JACOBI;
     const int c0 = az_rows(A, pc, col);  // rows whose (A z)_i the preconditioner leaves in az
+    RowMap rm;  // block-colour order: each XCD takes an eighth of every colour's blocks
+    if (pc == CFD_PC_ILU && col && col->block > 1 && col->ncolour <= 4 && Gs % 8 == 0) {
+        bool ok = true;
+        for (int k = 0; k < col->ncolour; ++k) ok = ok && col->nbk[k] % 8 == 0;
+        if (ok) {
+            rm.ncol = col->ncolour;
+            for (int k = 0; k <= col->ncolour; ++k) rm.off[k] = col->off[k];
+            for (int k = 0; k < col->ncolour; ++k) rm.nb[k] = col->nbk[k];
+        }
+    }
     // the update at full grid: 1024-thread workgroups, a quarter as many,
     // each summing the p.q partials once (the same threads stream the rows)
     const int ub = G >= kRedGrid ? kFinBlock : kBlock, Gu = G * kBlock / ub;
@@ -1954,7 +1998,7 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
         auto record = [&](int iters) {
             for (int k = 0; k < iters; ++k) {
                 hipLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, A,
-                                   (const T *)w.z.p, w.p0.p, w.q.p, x, (const T *)w.az.p, c0,
+                                   (const T *)w.z.p, w.p0.p, w.q.p, x, (const T *)w.az.p, c0, rm,
                                    state_.p, pqpart);
                 rest_of_iteration();
             }
@@ -1979,13 +2023,13 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
             if (timed) {  // the events are part of the launch: kernel start to end
                 hipExtLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, ev(2 * nev),
                                       ev(2 * nev + 1), 0u, A, (const T *)w.z.p, w.p0.p, w.q.p,
-                                      x, (const T *)w.az.p, c0, (const KState *)state_.p,
+                                      x, (const T *)w.az.p, c0, rm, (const KState *)state_.p,
                                       pqpart);
                 ev_iter.push_back(issued + k);
                 ++nev;
             } else {
                 hipLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, A,
-                                   (const T *)w.z.p, w.p0.p, w.q.p, x, (const T *)w.az.p, c0,
+                                   (const T *)w.z.p, w.p0.p, w.q.p, x, (const T *)w.az.p, c0, rm,
                                    state_.p, pqpart);
             }
             rest_of_iteration();

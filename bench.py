@@ -66,9 +66,9 @@ def parse():
                     help="replay pressure CG iterations from a captured HIP graph (cavity and "
                          "batch workloads; the channel run times one SpMV launch in eight with "
                          "events, which keeps its launches eager)")
-    ap.add_argument("--ilu-block", type=int, default=1,
-                    help="ILU(0) in block-colour order (the 6 tetrahedra of a split hexahedron "
-                         "as one block); 0: the point multicolour order")
+    ap.add_argument("--block-order", type=int, default=1,
+                    help="ILU(0) and SSOR in block-colour order (the 6 tetrahedra of a split "
+                         "hexahedron as one block); 0: the point multicolour order")
     ap.add_argument("--live-timing", type=int, default=1,
                     help="channel: time the pressure SpMV launches with HIP events inside the "
                          "timed region (the roofline figure); 0 leaves them untimed")
@@ -339,7 +339,7 @@ def main():
     gpu.set_option("mixed_inner_tol", a.mixed_inner_tol)
     gpu.set_option("autotune", a.autotune)
     gpu.set_option("graphs", a.graphs)
-    gpu.set_option("ilu_block", a.ilu_block)
+    gpu.set_option("block_order", a.block_order)
     setup_s = time.time() - t0
     print(f"[bench] setup {setup_s:.1f}s cells={raw.counts[1]} fluid={gpu.counts(0)[1]} "
           f"solid={gpu.counts(1)[1]}", file=sys.stderr, flush=True)
@@ -415,7 +415,7 @@ def main():
                    "cells": raw.counts[1],
                    "solvers": "BiCGSTAB+Jacobi / CG+ILU(0), 1e-8" if a.solvers == "config" else
                    "reference: GMRES+SSOR / CG+SSOR(1.3) 1e-6",
-                   "ilu_order": "block-colour (6-tet blocks)" if a.ilu_block else "point colour",
+                   "sweep_order": "block-colour (6-tet blocks)" if a.block_order else "point colour",
                    "parallelism": "single GPU"},
         "roofline": roof, "cpu_baseline": cpu, "krylov": krylov, "p_iters_learned": learned,
         "device_s_per_step": st[1] / max(st[0], 1),

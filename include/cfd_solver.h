@@ -259,12 +259,12 @@ int cfd_ns_profile_read(const cfd_ns *h, double out[8]);
  *   "autotune" 0/1 (process-wide): the first launch of each assembly and
  *   gradient kernel at a given size times workgroups of 64..1024 threads and
  *   keeps the fastest (default 1; results do not depend on it)
- *   "ilu_block" 0/1: ILU(0) preconditioners of the fluid and the solid in
- *   block-colour order (the 6 tetrahedra of a split hexahedron as one block,
- *   factored in natural order, the blocks coloured) where the mesh has one;
- *   0: the point multicolour order (default 1; the order changes the
- *   preconditioner, so iteration counts differ, every solve still meets its
- *   tolerance on the true FP64 residual)
+ *   "block_order" 0/1: ILU(0) and SSOR preconditioners of the fluid and the
+ *   solid in block-colour order (the 6 tetrahedra of a split hexahedron as
+ *   one block, swept in natural order, the blocks coloured) where the mesh
+ *   has one; 0: the point multicolour order (default 1; the order changes
+ *   the preconditioner, so iteration counts differ, every solve still meets
+ *   its tolerance on the true FP64 residual)
  * cfd_ns_get_option also reads "graph_launches" (CG iterations replayed). */
 int cfd_ns_set_option(cfd_ns *h, const char *name, double value);
 double cfd_ns_get_option(const cfd_ns *h, const char *name);

@@ -551,8 +551,10 @@ SolveCfg RefNS::cfg_of(const cfd_solve_cfg &c) const {
     s.tol = c.tol;
     s.maxit = c.maxit;
     s.mc = mc;
-    // ILU(0) factors in the product's block-colour order where the mesh has one
-    s.colour = (c.precond == CFD_PC_ILU && !bcolour.empty()) ? &bcolour : &colour;
+    // ILU(0) and SSOR sweep in the product's block-colour order where the
+    // mesh has one
+    const bool blk = (c.precond == CFD_PC_ILU || c.precond == CFD_PC_SSOR) && !bcolour.empty();
+    s.colour = blk ? &bcolour : &colour;
     return s;
 }
 
@@ -1238,7 +1240,8 @@ int RefHT::UpdateEnergy() {
     c.tol = ns->P.solid.tol;
     c.maxit = ns->P.solid.maxit;
     c.mc = ns->mc;
-    c.colour = (c.precond == CFD_PC_ILU && !bcolour.empty()) ? &bcolour : &colour;
+    const bool blk = (c.precond == CFD_PC_ILU || c.precond == CFD_PC_SSOR) && !bcolour.empty();
+    c.colour = blk ? &bcolour : &colour;
     last = las_solve(As, xsol, bs, c);
     if (last.iters >= c.maxit && last.relres > c.tol) return -1;
     Tn = xsol;

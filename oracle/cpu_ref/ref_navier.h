@@ -20,7 +20,7 @@ struct RefHT {
     double URF = 1.0;  // src/Preprocessor.cpp:506
     std::vector<double> Tn, Rn, diffCoef, dPhidX, BTem, xsol, bs;
     std::vector<double> Tnp, Tnp2;  // transient history
-    std::vector<int> colour, bcolour;  // bcolour: the ILU(0) block order (may be empty)
+    std::vector<int> colour, bcolour;  // bcolour: the ILU(0)/SSOR block order (may be empty)
     QMat As;
     SolveResult last;
     int InitFlowField();
@@ -40,7 +40,7 @@ struct RefNS {
     std::unique_ptr<RefHT> ht;
     double URF[8];
     int mc = 0;  // 1: multicolour SSOR/ILU order (the GPU order)
-    std::vector<int> colour, bcolour;  // bcolour: the ILU(0) block order (may be empty)
+    std::vector<int> colour, bcolour;  // bcolour: the ILU(0)/SSOR block order (may be empty)
     std::vector<double> Rn, Un, Vn, Wn, Pn, Tn, TE, ED, VisLam, VisTur, Apr;
     std::vector<double> RUFace, dPdX, dUdX, dVdX, dWdX, dPhidX;
     std::vector<double> BRo, BU, BV, BW, BPre, BTem, BTE, BED, kcond;

@@ -160,7 +160,7 @@ class Krylov {
     void ensure(int n, int K);
     void read_state();
     template <class T>
-    void setup_pc(const EllT<T> &A, int pc, Colouring *col, KWork<T> &w);
+    void setup_pc(const EllT<T> &A, int pc, Colouring *col, KWork<T> &w, double omega = 1.);
     template <class T>
     // rz_blocks: ILU and SSOR; also write the workgroup partials of c.y to the
     // partials buffer from offset kPartRho and return their count (-1 when
@@ -171,8 +171,9 @@ class Krylov {
                   KWork<T> &w, int *rz_blocks = nullptr, const KState *st = nullptr,
                   T *az = nullptr);
     template <class T>
+    // cs: the forward sweeps read cs * c (SSOR: 2 - omega; ILU: 1)
     void apply_block_ilu(const EllT<T> &A, const T *c, T *y, Colouring *col, KWork<T> &w,
-                         int *rz_blocks, const KState *st, T *az);
+                         int *rz_blocks, const KState *st, T *az, T cs);
     template <class T>
     SolveStats cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &cfg, Colouring *col,
                   int pc, KWork<T> &w, bool per_case = false);

@@ -1063,7 +1063,7 @@ __global__ __launch_bounds__(256) void k_bilu_fwd(int n, int K, const int *__res
                                                   const unsigned char *__restrict__ bin,
                                                   const unsigned char *__restrict__ nlow,
                                                   const T *__restrict__ lu, int r0, int nb,
-                                                  const T *__restrict__ c,
+                                                  const T *__restrict__ c, T cs,
                                                   const T *__restrict__ tin, T *__restrict__ tout,
                                                   const KState *st) {
     __shared__ T sv[B][kBlock];
@@ -1075,7 +1075,7 @@ __global__ __launch_bounds__(256) void k_bilu_fwd(int n, int K, const int *__res
     L.template load<0>(n, K, col, bin, nlow, lu, base, nb, tin);
     T cv[B];
 #pragma unroll
-    for (int j = 0; j < B; ++j) cv[j] = c[base + j * nb];
+    for (int j = 0; j < B; ++j) cv[j] = cs * c[base + j * nb];
 #pragma unroll
     for (int j = 0; j < B; ++j) {
         const T tv = L.sub(j, cv[j], sv, tid);
@@ -1091,7 +1091,7 @@ __global__ __launch_bounds__(256) void k_bilu_last(int n, int K, const int *__re
                                                    const unsigned char *__restrict__ nlow,
                                                    const T *__restrict__ lu,
                                                    const T *__restrict__ ud, int r0, int nb,
-                                                   const T *__restrict__ c,
+                                                   const T *__restrict__ c, T cs,
                                                    const T *__restrict__ tin, T *__restrict__ y,
                                                    double *rpart, const KState *st) {
     __shared__ T sv[B][kBlock], sy[B][kBlock];
@@ -1109,7 +1109,7 @@ __global__ __launch_bounds__(256) void k_bilu_last(int n, int K, const int *__re
             dv[j] = ud[base + j * nb];
         }
 #pragma unroll
-        for (int j = 0; j < B; ++j) sv[j][tid] = L.template sub<1>(j, cv[j], sv, tid);
+        for (int j = 0; j < B; ++j) sv[j][tid] = L.template sub<1>(j, cs * cv[j], sv, tid);
 #pragma unroll
         for (int j = B - 1; j >= 0; --j) {
             const T yv = L.template sub<2>(j, sv[j][tid], sy, tid) / dv[j];
@@ -1174,6 +1174,26 @@ __global__ __launch_bounds__(256) void k_bilu_bwd(
     if (rpart) {
         block_sum<1>(rz);
         if (threadIdx.x == 0) rpart[blockIdx.x] = rz[0];
+    }
+}
+
+// SSOR(omega) in the form of the block ILU(0) sweeps: with l_pk =
+// a_pk omega / d_k (lower slots), u_pk = a_pk (upper) and u_pp = d_p / omega,
+// the forward sweep of (2 - omega) c gives (D/omega) t' for
+// (D/omega + L) t' = (2 - omega) c, and the backward sweep
+// (D/omega + U)^-1 of that, i.e. the SSOR result
+// omega (2 - omega) (D + omega U)^-1 D (D + omega L)^-1 c.
+template <class T>
+__global__ void k_ssor_block_factor(int n, int K, const int *__restrict__ col,
+                                    const T *__restrict__ val, const T *__restrict__ diag, T om,
+                                    T *__restrict__ lu, T *__restrict__ ud) {
+    GRID_LOOP(p, n) {
+        ud[p] = diag[p] / om;
+        for (int k = 0; k < K; ++k) {
+            const size_t o = (size_t)k * n + p;
+            const int c = col[o];
+            lu[o] = c < p ? val[o] * om / diag[c] : val[o];
+        }
     }
 }
 
@@ -1557,8 +1577,10 @@ double Krylov::relres(const Ell &A, const double *x, const double *b) {
 // later colour and the slot count is a constant; 0 otherwise.
 template <class T>
 static int az_rows(const EllT<T> &A, int pc, const Colouring *col) {
-    if (pc != CFD_PC_ILU || !col || col->ncolour < 2) return 0;
-    if (col->block > 1) return A.K <= 4 ? col->off[1] : 0;
+    if (!col || col->ncolour < 2) return 0;
+    if (col->block > 1)  // ILU(0) or SSOR in block-colour order
+        return (pc == CFD_PC_ILU || pc == CFD_PC_SSOR) && A.K <= 4 ? col->off[1] : 0;
+    if (pc != CFD_PC_ILU) return 0;
     return (A.K == 4 || A.K == 6) ? col->off[1] : 0;
 }
 
@@ -1569,7 +1591,7 @@ static int az_rows(const EllT<T> &A, int pc, const Colouring *col) {
 // partials of c.y.
 template <class T>
 void Krylov::apply_block_ilu(const EllT<T> &A, const T *c, T *y, Colouring *col, KWork<T> &w,
-                             int *rz_blocks, const KState *st, T *az) {
+                             int *rz_blocks, const KState *st, T *az, T cs) {
     constexpr int B = kIluBlock;
     if (col->block != B || A.K > 4) throw HipError("block-colour ILU(0): <= 4 slots, 6-row blocks");
     const int n = A.n, nc = col->ncolour;
@@ -1583,13 +1605,13 @@ void Krylov::apply_block_ilu(const EllT<T> &A, const T *c, T *y, Colouring *col,
     for (int k = 0; k < nc - 1; ++k) {
         if (!col->nbk[k]) continue;
         hipLaunchKernelGGL((k_bilu_fwd<T, 4, B>), dim3(grid8(col->nbk[k])), dim3(kBlock), 0, st_,
-                           n, A.K, A.col, bi, nl, lu, col->off[k], col->nbk[k], c,
+                           n, A.K, A.col, bi, nl, lu, col->off[k], col->nbk[k], c, cs,
                            (const T *)w.tmp.p, w.tmp.p, st);
     }
     if (col->nbk[nc - 1]) {
         const int g = grid8(col->nbk[nc - 1]);
         hipLaunchKernelGGL((k_bilu_last<T, 4, B>), dim3(g), dim3(kBlock), 0, st_, n, A.K, A.col,
-                           bi, nl, lu, ud, col->off[nc - 1], col->nbk[nc - 1], c,
+                           bi, nl, lu, ud, col->off[nc - 1], col->nbk[nc - 1], c, cs,
                            (const T *)w.tmp.p, y, rp ? rp + used : (double *)nullptr, st);
         if (rp) used += g;
     }
@@ -1597,7 +1619,7 @@ void Krylov::apply_block_ilu(const EllT<T> &A, const T *c, T *y, Colouring *col,
         if (!col->nbk[k]) continue;
         const int g = grid8(col->nbk[k]);
         double *rpp = rp ? rp + used : (double *)nullptr;
-        if (k == 0 && az && az_rows(A, CFD_PC_ILU, col) > 0)
+        if (k == 0 && az && az_rows(A, CFD_PC_ILU, col) > 0)  // any block-order preconditioner
             hipLaunchKernelGGL((k_bilu_bwd<T, 4, B, true>), dim3(g), dim3(kBlock), 0, st_, n,
                                A.K, A.col, bi, nl, lu, ud, (const T *)A.val, (const T *)A.diag,
                                col->off[k], col->nbk[k], c, (const T *)w.tmp.p, (const T *)y, y,
@@ -1614,7 +1636,16 @@ void Krylov::apply_block_ilu(const EllT<T> &A, const T *c, T *y, Colouring *col,
 }
 
 template <class T>
-void Krylov::setup_pc(const EllT<T> &A, int pc, Colouring *col, KWork<T> &w) {
+void Krylov::setup_pc(const EllT<T> &A, int pc, Colouring *col, KWork<T> &w, double omega) {
+    if (pc == CFD_PC_SSOR && col && col->block > 1) {
+        // SSOR in the block sweeps' form: lower factor a_pk omega / d_k, upper
+        // a_pk, diagonal d_p / omega (see apply_block_ilu)
+        hipLaunchKernelGGL(k_ssor_block_factor<T>, dim3(grid_for(A.n)), dim3(kBlock), 0, st_,
+                           A.n, A.K, A.col, (const T *)A.val, (const T *)A.diag, (T)omega,
+                           w.lu.p, w.ud.p);
+        CFD_LAUNCH_CHECK();
+        return;
+    }
     if (pc != CFD_PC_ILU) return;
     const int n = A.n;
     if (A.K > kIluMaxK) throw HipError("ILU(0): more than 16 off-diagonals per row");
@@ -1658,8 +1689,9 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
     } else if (pc == CFD_PC_JACOBI) {
         hipLaunchKernelGGL(k_jacobi<T>, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
                            (const T *)A.diag, c, y);
+    } else if (pc == CFD_PC_SSOR && col->block > 1) {
+        apply_block_ilu(A, c, y, col, w, rz_blocks, st, az, (T)(2. - omega));
     } else if (pc == CFD_PC_SSOR) {
-        if (col->block > 1) throw HipError("SSOR sweeps need the plain multicolour order");
         const int nc = col->ncolour;
         const int c0end = nc > 1 ? col->off[1] : n;
         int nbk = 0;
@@ -1704,7 +1736,7 @@ void Krylov::apply_pc(const EllT<T> &A, int pc, double omega, const T *c, T *y, 
         }
         if (rz_blocks && rp) *rz_blocks = used;
     } else if (col->block > 1) {
-        apply_block_ilu(A, c, y, col, w, rz_blocks, st, az);
+        apply_block_ilu(A, c, y, col, w, rz_blocks, st, az, T(1));
     } else {
         const int nc = col->ncolour;
         const int c0end = nc > 1 ? col->off[1] : n;
@@ -1778,7 +1810,7 @@ void Krylov::precond(const Ell &A, int pc, double omega, const double *c, double
         hipLaunchKernelGGL(k_perm_matrix, dim3(grid_for(n)), dim3(kBlock), 0, st_, n, A.K,
                            col->perm.p, (const unsigned char *)col->smap.p, A.diag, A.val, wd_.pdiag.p, wd_.pval.p);
         Ell P{n, A.K, wd_.pdiag.p, wd_.pval.p, col->pcol.p};
-        setup_pc(P, pc, col, wd_);
+        setup_pc(P, pc, col, wd_, omega);
         hipLaunchKernelGGL(k_perm_gather, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
                            col->perm.p, c, wd_.pb.p);
         apply_pc(P, pc, omega, (const double *)wd_.pb.p, wd_.px.p, col, wd_);
@@ -1812,7 +1844,7 @@ SolveStats Krylov::solve(const Ell &A0, const double *b0, double *x0, const cfd_
         A = Ell{n, A0.K, wd_.pdiag.p, wd_.pval.p, col->pcol.p};
         b = wd_.pb.p;
         x = wd_.px.p;
-        setup_pc(A, pc, col, wd_);
+        setup_pc(A, pc, col, wd_, cfg.omega);
     }
     const bool per_case = batched();
     if (per_case) {
@@ -1862,7 +1894,7 @@ SolveStats Krylov::solve_mixed(const Ell &A0, const double *b0, double *x0,
                        (const double *)A.diag, (const double *)A.val, wf_.pdiag.p, wf_.pval.p);
     CFD_LAUNCH_CHECK();
     EllF Af{n, A.K, wf_.pdiag.p, wf_.pval.p, A.col};
-    setup_pc(Af, pc, col, wf_);
+    setup_pc(Af, pc, col, wf_, cfg.omega);
     SolveStats S;
     int total = 0;
     double rel = 0.;

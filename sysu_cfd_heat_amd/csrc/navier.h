@@ -109,9 +109,9 @@ class NSSolver {
     // mixed-precision pressure solves: the reduction each FP32 inner CG
     // makes before the FP64 residual is recomputed
     double mixed_inner_tol = 1e-5;
-    // ILU(0) of the fluid and the solid in block-colour order where the mesh
-    // has one (else the point colouring)
-    bool ilu_block = true;
+    // ILU(0) and SSOR of the fluid and the solid in block-colour order where
+    // the mesh has one (else the point colouring)
+    bool block_order = true;
     int run_solid(const std::function<int()> &f, bool async);
     int solid_join();
     // perfect gas (DensityModel 1): rho = p / (Rcpcv T), Rcpcv = cp - cp / gama

@@ -423,15 +423,14 @@ int NSSolver::InitFlowField() {
 }
 
 namespace {
-// The ordering a solve with preconditioner pc sweeps in: ILU(0) takes the
-// block-colour order where the mesh has one (hm.bcolour), SSOR and the
-// other meshes the greedy cell colouring; each is built once.
+// The ordering a solve with preconditioner pc sweeps in: the block-colour
+// order where the mesh has one (hm.bcolour) and block is set, else the
+// greedy cell colouring; each is built once.
 Colouring *colouring_of(const HostMesh &hm, const DevMeshOwner &dmo, int pc, bool block,
                         Colouring &colour, bool &colour_ready, Colouring &bcolour,
                         bool &bcolour_ready, hipStream_t s) {
     if (pc != CFD_PC_SSOR && pc != CFD_PC_ILU) return nullptr;
-    if (block && pc == CFD_PC_ILU && !hm.bcolour.empty() && hm.bcolour[0] >= 0 &&
-        dmo.dm.K <= 4) {
+    if (block && !hm.bcolour.empty() && hm.bcolour[0] >= 0 && dmo.dm.K <= 4) {
         if (!bcolour_ready) {
             bcolour.build(dmo.ell_col_host.data(), hm.ncel, dmo.dm.K, hm.bcolour.data(),
                           dmo.dm.ell_col, s, kIluBlock);
@@ -449,7 +448,7 @@ Colouring *colouring_of(const HostMesh &hm, const DevMeshOwner &dmo, int pc, boo
 }  // namespace
 
 Colouring *NSSolver::colouring_for(const cfd_solve_cfg &c) {
-    return colouring_of(hm, dmo, c.precond, ilu_block, colour, colour_ready, bcolour,
+    return colouring_of(hm, dmo, c.precond, block_order, colour, colour_ready, bcolour,
                         bcolour_ready, s);
 }
 
@@ -844,7 +843,7 @@ int HTSolver::UpdateEnergy() {
     }
     BuildScalarMatrix();
     CFD_HIP(hipMemcpyAsync(xsol, T, sizeof(double) * hm.ncel, hipMemcpyDeviceToDevice, s));
-    Colouring *col = colouring_of(hm, dmo, ns->P.solid.precond, ns->ilu_block, colour,
+    Colouring *col = colouring_of(hm, dmo, ns->P.solid.precond, ns->block_order, colour,
                                   colour_ready, bcolour, bcolour_ready, s);
     Ell A{hm.ncel, dmo.dm.K, diag.p, val.p, dmo.dm.ell_col};
     last = kry->solve(A, bs, xsol, ns->P.solid, col);
@@ -1150,10 +1149,12 @@ int cfd_ns_profile_read(const cfd_ns *h, double out[8]) {
     // with ILU(0) over two or more colours (4 or 6 slots), colour-0 rows take
     // (A z)_i from the preconditioner's last sweep instead of their row
     const int K = n.dmo.dm.K;
-    const bool blk = n.ilu_block && n.bcolour_ready;
+    const bool blk = n.block_order && n.bcolour_ready;
     const Colouring &pc = blk ? n.bcolour : n.colour;  // ILU's order
-    const bool az = n.P.pres.precond == CFD_PC_ILU && (blk || n.colour_ready) &&
-                    pc.ncolour > 1 && (pc.block > 1 ? K <= 4 : (K == 4 || K == 6));
+    const int ppc = n.P.pres.precond;
+    const bool az = (blk ? (ppc == CFD_PC_ILU || ppc == CFD_PC_SSOR) && K <= 4
+                         : ppc == CFD_PC_ILU && n.colour_ready && (K == 4 || K == 6)) &&
+                    pc.ncolour > 1;
     long long nnz = 0, nnz_read = 0;
     for (size_t k = 0; k < n.dmo.ell_col_host.size(); ++k) {
         const int row = (int)(k % n.hm.ncel);
@@ -1193,8 +1194,8 @@ int cfd_ns_set_option(cfd_ns *h, const char *name, double value) {
         set_autotune(value != 0.);
         return 0;
     }
-    if (k == "ilu_block") {
-        h->ns.ilu_block = value != 0.;
+    if (k == "block_order") {
+        h->ns.block_order = value != 0.;
         return 0;
     }
     if (k == "mixed_inner_tol") {
@@ -1216,7 +1217,7 @@ double cfd_ns_get_option(const cfd_ns *h, const char *name) {
     if (k == "overlap_solid") return h->ns.overlap_solid ? 1. : 0.;
     if (k == "mixed_inner_tol") return h->ns.mixed_inner_tol;
     if (k == "autotune") return get_autotune() ? 1. : 0.;
-    if (k == "ilu_block") return h->ns.ilu_block ? 1. : 0.;
+    if (k == "block_order") return h->ns.block_order ? 1. : 0.;
     if (k == "graph_launches") return (double)h->ns.kry->graph_launches;
     last_error() = "unknown option " + k;
     return std::nan("");

@@ -141,16 +141,21 @@ def test_simple_iterations_match_oracle(case):
         assert rr < 1e-7, (eq, it, rr)
 
 
+@pytest.mark.parametrize("precond", ["ilu", "ssor"])
 @pytest.mark.parametrize("case", ["cavity_tet", "channel_cht"])
-def test_ilu_order_matches_oracle(case):
+def test_ilu_order_matches_oracle(case, precond):
     """Tetrahedral meshes of split hexahedra: the pressure and solid ILU(0)
-    run in the block-colour order (the 6 tetrahedra of a hexahedron as one
-    block, two block colours) on the device and in the oracle, so the CG
-    solves take the same iteration counts (to rounding: +-1)."""
-    P, _ = _solvers()
+    or SSOR(1.3) sweep in the block-colour order (the 6 tetrahedra of a
+    hexahedron as one block, two block colours) on the device and in the
+    oracle, so the CG solves take the same iteration counts (to rounding:
+    +-1)."""
+    P, O = _solvers()
     raw, p = _cases()[case]
     assert (P.Mesh(raw, p, 0).get("cell_bcolour") >= 0).all()
-    gpu, ref = _pair(case)
+    if precond == "ssor":
+        p.pres = P.solve_cfg(P.CFD_CG, P.CFD_PC_SSOR, 1.3, 1e-8, 2000)
+        p.solid = P.solve_cfg(P.CFD_CG, P.CFD_PC_SSOR, 1.3, 1e-8, 2000)
+    gpu, ref = P.NSSolver(raw, p), O.NSSolver(raw, p, mc=1)
     for _ in range(2):
         gpu.call("Iterate")
         ref.call("Iterate")

@@ -183,8 +183,9 @@ __global__ void k_add_f(int n, double *__restrict__ x, const float *__restrict__
 // that two rows' gathers are in flight together; the rows a workgroup covers
 // are those of the plain grid-stride loop.
 // Rows below c0 take (A z)_i from az, where the preconditioner's last sweep
-// left it (same sum, same order: see k_ilu_bwd0_az), instead of reading the
-// row and gathering z again.
+// left it (same sum, same order: see k_ilu_bwd0_az and k_bilu_bwd), instead
+// of reading the row and gathering z again.
+
 // Block-colour order (RowMap with colours, every colour's block count a
 // multiple of 8): XCD x's workgroups take the x-th eighth of every colour's
 // blocks, all their rows, so that the z they gather (rows of the blocks'
@@ -1572,9 +1573,11 @@ double Krylov::relres(const Ell &A, const double *x, const double *b) {
     return bn > 0 ? rn / bn : rn;
 }
 
-// Rows [0, az_rows) of the permuted order for which ILU(0)'s last backward
-// launch can also leave (A y)_p (k_ilu_bwd0_az): colour 0, when there is a
-// later colour and the slot count is a constant; 0 otherwise.
+// Rows [0, az_rows) of the permuted order for which the preconditioner's
+// last backward launch can also leave (A y)_p (k_ilu_bwd0_az for ILU(0) in
+// the point order, k_bilu_bwd for ILU(0) or SSOR in block order): colour 0,
+// when there is a later colour and the slot count is 4 or 6 (block order:
+// at most 4); 0 otherwise.
 template <class T>
 static int az_rows(const EllT<T> &A, int pc, const Colouring *col) {
     if (!col || col->ncolour < 2) return 0;

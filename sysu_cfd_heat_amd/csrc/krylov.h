@@ -62,6 +62,10 @@ struct SolveStats {
     int refinements = 0; // mixed precision: FP64 outer corrections
 };
 
+// iterations per long graph of a Krylov loop (a batch of b iterations is
+// b / kGraphIters launches of it and the rest of the one-iteration graph)
+constexpr int kGraphIters = 8;
+
 struct KState {
     double rho, rho_old, alpha, beta, omega, pq, rr, bb, rnorm, bnorm, eps, hscale;
     double h[12], h2[12], H[11 * 10], cs[10], sn[10], g[11], y[10];
@@ -109,9 +113,10 @@ class Krylov {
     int check_first = 4;  // iterations before the first host check
     int check_max = 64;   // batch cap
 
-    // CG iterations replayed from HIP graphs of 1 and 8 iterations, captured
-    // once per matrix / vector / preconditioner configuration (results are
-    // identical to the eager launches). Not used while live timing is on.
+    // Krylov iterations (CG, BiCGSTAB, multi-RHS BiCGSTAB) replayed from HIP
+    // graphs of 1 and 8 iterations, captured once per matrix / vector /
+    // preconditioner configuration (results are identical to the eager
+    // launches). CG does not use them while live timing is on.
     bool use_graphs = true;
     long long graph_launches = 0;
 

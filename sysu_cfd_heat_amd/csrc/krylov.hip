@@ -1381,9 +1381,6 @@ Krylov::~Krylov() {
     for (auto &g : graphs_) (void)hipGraphExecDestroy(g.exec);
 }
 
-// iterations per long graph of a Krylov loop (a batch of b iterations is
-// b / kGraphIters launches of it and the rest of the one-iteration graph)
-constexpr int kGraphIters = 8;
 // live timing of k_cg_pq: HIP events attached to one launch in kProfEvery
 // (every launch moves the same bytes; separate event records around every
 // launch cost 3 % of the run)

@@ -235,18 +235,42 @@ void Krylov::solve_multi(const Ell &A, int nb, const double *const *b, double *c
                        fin, per_case ? 0. : cfg.tol, cfg.maxit);
     CFD_LAUNCH_CHECK();
     int batch = check_first, issued = 0;
+    // one iteration's launches (the same every iteration)
+    auto iteration = [&]() {
+        hipLaunchKernelGGL(k_m_p, dim3(g), dim3(kBlock), 0, st_, n, V, nb,
+                           (const double *)A.diag, (const KState *)mstate_.p);
+        CFD_REDUCE_LAUNCH_K(k_m_spmv<0>, Gs, st_, A, V, nb, mstate_.p, part_.p, fin);
+        CFD_REDUCE_LAUNCH_K(k_m_s, G, st_, n, V, nb, (const double *)A.diag, mstate_.p,
+                            part_.p, fin);
+        CFD_REDUCE_LAUNCH_K(k_m_spmv<1>, Gs, st_, A, V, nb, mstate_.p, part_.p, fin);
+        CFD_REDUCE_LAUNCH_K(k_m_x, G, st_, n, V, nb, mstate_.p, part_.p, fin);
+    };
+    // graphs of one and of kGraphIters iterations, as for CG
+    hipGraphExec_t gexec = nullptr, gexec_long = nullptr;
+    if (use_graphs) {
+        std::vector<uintptr_t> key = {3, (uintptr_t)A.diag, (uintptr_t)A.val, (uintptr_t)A.col,
+                                      (uintptr_t)n, (uintptr_t)A.K, (uintptr_t)nb,
+                                      (uintptr_t)mrhs_.p, (uintptr_t)mstate_.p};
+        for (int q = 0; q < nb; ++q) {
+            key.push_back((uintptr_t)x[q]);
+            key.push_back((uintptr_t)b[q]);
+        }
+        key.push_back(1);
+        gexec = graph_for(key, [&]() { iteration(); });
+        key.back() = kGraphIters;
+        gexec_long = graph_for(key, [&]() {
+            for (int k = 0; k < kGraphIters; ++k) iteration();
+        });
+    }
     for (;;) {
-        for (int k = 0; k < batch; ++k) {
-            hipLaunchKernelGGL(k_m_p, dim3(g), dim3(kBlock), 0, st_, n, V, nb,
-                               (const double *)A.diag, (const KState *)mstate_.p);
-            CFD_REDUCE_LAUNCH_K(k_m_spmv<0>, Gs, st_, A, V, nb, mstate_.p,
-                               part_.p, fin);
-            CFD_REDUCE_LAUNCH_K(k_m_s, G, st_, n, V, nb,
-                               (const double *)A.diag, mstate_.p, part_.p, fin);
-            CFD_REDUCE_LAUNCH_K(k_m_spmv<1>, Gs, st_, A, V, nb, mstate_.p,
-                               part_.p, fin);
-            CFD_REDUCE_LAUNCH_K(k_m_x, G, st_, n, V, nb, mstate_.p,
-                               part_.p, fin);
+        if (gexec) {
+            int k = 0;
+            for (; k + kGraphIters <= batch; k += kGraphIters)
+                CFD_HIP(hipGraphLaunch(gexec_long, st_));
+            for (; k < batch; ++k) CFD_HIP(hipGraphLaunch(gexec, st_));
+            graph_launches += batch;
+        } else {
+            for (int k = 0; k < batch; ++k) iteration();
         }
         CFD_LAUNCH_CHECK();
         issued += batch;

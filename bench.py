@@ -315,8 +315,7 @@ def main():
         return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              "(multi-GPU sharding is a design, see the sharding section of DESIGN.md)",
-              file=sys.stderr)
+              "(sharding across GPUs is designed, not built)", file=sys.stderr)
         sys.exit(2)
     if a.workload == "cavity":
         return run_cavity(a)

@@ -238,8 +238,10 @@ int cfd_ns_stats(const cfd_ns *h, double out[8]);
 int cfd_ns_reset_stats(cfd_ns *h);
 
 /* Live HIP-event timing of the pressure CG's SpMV kernel (the dominant
- * kernel of the step), recorded on the solver's own stream around each
- * launch. enable resets the counters. out = {FP64 launches that did work,
+ * kernel of the step): events on the solver's own stream attached to one
+ * launch in eight (the kernel's start and end), the launches of timed
+ * steps that did work after the first of each solve. enable resets the
+ * counters. out = {FP64 launches timed,
  * their total ms, FP64 algorithmic bytes per launch, stored off-diagonals,
  * FP32 launches (mixed-precision inner solves), their total ms, FP32 bytes
  * per launch, 0}. */

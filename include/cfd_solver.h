@@ -249,8 +249,9 @@ int cfd_ns_profile(cfd_ns *h, int enable);
 int cfd_ns_profile_read(const cfd_ns *h, double out[8]);
 
 /* Run-time options (no counterpart in the reference):
- *   "graphs" 0/1: replay CG iterations from captured HIP graphs (default 1;
- *   results are bit-identical either way)
+ *   "graphs" 0/1: replay Krylov iterations (CG, BiCGSTAB, GMRES restart
+ *   cycles) from captured HIP graphs (default 1; results are bit-identical
+ *   either way)
  *   "overlap_solid" 0/1: run the solid conduction solve of an outer
  *   iteration on its own stream beside the next iteration's momentum,
  *   pressure and k-epsilon steps (default 1; results are identical, a solid

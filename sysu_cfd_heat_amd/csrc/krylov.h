@@ -113,10 +113,11 @@ class Krylov {
     int check_first = 4;  // iterations before the first host check
     int check_max = 64;   // batch cap
 
-    // Krylov iterations (CG, BiCGSTAB, multi-RHS BiCGSTAB) replayed from HIP
-    // graphs of 1 and 8 iterations, captured once per matrix / vector /
-    // preconditioner configuration (results are identical to the eager
-    // launches). CG does not use them while live timing is on.
+    // Krylov iterations replayed from captured HIP graphs (CG, BiCGSTAB and
+    // multi-RHS BiCGSTAB: graphs of 1 and 8 iterations; GMRES: one restart
+    // cycle), captured once per matrix / vector / preconditioner
+    // configuration (results are identical to the eager launches). CG does
+    // not use them while live timing is on.
     bool use_graphs = true;
     long long graph_launches = 0;
 

@@ -2219,13 +2219,15 @@ SolveStats Krylov::gmres(const Ell &A, const double *b, double *x, const cfd_sol
         CFD_HIP(hipMemcpyAsync(state_.p, &init, sizeof(KState), hipMemcpyHostToDevice, st_));
         CFD_HIP(hipStreamSynchronize(st_));
     }
-    double *W = w0_.p, *Wn = w1_.p;
-    CFD_REDUCE_LAUNCH(k_residual<double>, G, st_, A, b, (const double *)x, W,
+    CFD_REDUCE_LAUNCH(k_residual<double>, G, st_, A, b, (const double *)x, w0_.p,
                        (double *)nullptr, (double *)nullptr, (double *)nullptr, state_.p, part_.p, fin, 2,
                        cfg.tol, cfg.maxit);
     CFD_LAUNCH_CHECK();
     read_state();
-    while (!hstate_->done) {
+    // one restart cycle's launches (the same every cycle; kernels past the
+    // cycle's end or the solve's do nothing), replayed from a captured graph
+    auto cycle = [&]() {
+        double *W = w0_.p, *Wn = w1_.p;
         for (int j = 0; j < m; ++j) {
             double *Vj = V_.p + (size_t)j * n;
             if (jac) {
@@ -2256,11 +2258,35 @@ SolveStats Krylov::gmres(const Ell &A, const double *b, double *x, const cfd_sol
             apply_pc(A, pc, cfg.omega, (const double *)wd_.q.p, wd_.z.p, col, wd_);
             hipLaunchKernelGGL(k_add, dim3(g), dim3(kBlock), 0, st_, n, x, wd_.z.p, state_.p);
         }
-        W = w0_.p;
-        Wn = w1_.p;
-        CFD_REDUCE_LAUNCH(k_residual<double>, G, st_, A, b, (const double *)x, W,
+        CFD_REDUCE_LAUNCH(k_residual<double>, G, st_, A, b, (const double *)x, w0_.p,
                            (double *)nullptr, (double *)nullptr, (double *)nullptr, state_.p, part_.p, fin, 2,
                            cfg.tol, -1);
+    };
+    hipGraphExec_t gexec = nullptr;
+    if (use_graphs) {
+        std::vector<uintptr_t> key = {
+            4, (uintptr_t)A.diag, (uintptr_t)A.val, (uintptr_t)A.col, (uintptr_t)x, (uintptr_t)b,
+            (uintptr_t)V_.p, (uintptr_t)w0_.p, (uintptr_t)w1_.p, (uintptr_t)wd_.z.p,
+            (uintptr_t)wd_.q.p, (uintptr_t)wd_.tmp.p, (uintptr_t)wd_.ud.p, (uintptr_t)wd_.lu.p,
+            (uintptr_t)n, (uintptr_t)A.K, (uintptr_t)pc, (uintptr_t)col,
+            (uintptr_t)(col ? col->ncolour : 0), (uintptr_t)(col ? col->triangle_free : 0),
+            (uintptr_t)(col ? col->block : 0)};
+        uint64_t bits;
+        std::memcpy(&bits, &cfg.omega, sizeof bits);
+        key.push_back((uintptr_t)bits);
+        std::memcpy(&bits, &cfg.tol, sizeof bits);
+        key.push_back((uintptr_t)bits);
+        if (col)
+            for (int k = 0; k <= col->ncolour; ++k) key.push_back((uintptr_t)col->off[k]);
+        gexec = graph_for(key, cycle);
+    }
+    while (!hstate_->done) {
+        if (gexec) {
+            CFD_HIP(hipGraphLaunch(gexec, st_));
+            ++graph_launches;
+        } else {
+            cycle();
+        }
         CFD_LAUNCH_CHECK();
         read_state();
     }

@@ -57,19 +57,22 @@ def test_mixed_precision_pressure():
         assert e <= 1e-6, (nm, e)
 
 
-@pytest.mark.parametrize("case", ["cavity", "channel"])
+@pytest.mark.parametrize("case", ["cavity", "channel", "channel_reference"])
 def test_graph_replay_is_bit_identical(case):
-    """CG iterations replayed from a captured HIP graph give exactly the
-    eager launches' results, solve after solve (several SIMPLE iterations,
-    each with its own pressure solve, eager host work and synchronisations
-    between the replays)."""
+    """Krylov iterations replayed from captured HIP graphs (CG, BiCGSTAB,
+    multi-RHS BiCGSTAB; GMRES restart cycles with the reference's solver
+    settings) give exactly the eager launches' results, solve after solve
+    (several SIMPLE iterations, eager host work and synchronisations between
+    the replays)."""
     import sysu_cfd_heat_amd as P
     from sysu_cfd_heat_amd import cases
     print(f"[forms] graphs {case}", flush=True)
     if case == "cavity":
         raw, p = cases.cavity(n=8, tets=True, jitter=0.1, seed=2, max_step=8)
-    else:
+    elif case == "channel":
         raw, p = cases.channel(nx=10, ny=4, nz=4, max_step=8)
+    else:  # GMRES+SSOR transport, CG+SSOR pressure
+        raw, p = cases.channel(nx=10, ny=4, nz=4, max_step=8, solvers="reference")
     out = []
     for graphs in (0, 1):
         s = P.NSSolver(raw, p)

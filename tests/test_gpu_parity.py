@@ -166,6 +166,27 @@ def test_ilu_order_matches_oracle(case, precond):
             assert abs(ig - io) <= 1, (ig, io)
 
 
+@pytest.mark.parametrize("n", [1, 2])
+def test_block_order_tiny_meshes(n):
+    """Block-colour ILU(0) at its edges: one hexahedron (one block, one
+    colour: only the last colour's launch runs) and 2^3 hexahedra (blocks of
+    two colours, fewer than a wave): device and oracle agree."""
+    P, O = _solvers()
+    from sysu_cfd_heat_amd import cases
+    raw, p = cases.cavity(n=n, tets=True, jitter=0.0, seed=1, max_step=6)
+    bc = P.Mesh(raw, p, 0).get("cell_bcolour")
+    assert bc.max() == (0 if n == 1 else 1) and bc.min() == 0
+    gpu, ref = P.NSSolver(raw, p), O.NSSolver(raw, p, mc=1)
+    for _ in range(3):
+        gpu.call("Iterate")
+        ref.call("Iterate")
+        (ig, rg), (io, ro) = gpu.last_solve("p"), ref.last_solve("p")
+        assert abs(ig - io) <= 1 and rg < 1e-8, (ig, io, rg, ro)
+    for nm in ("Un", "Vn", "Wn", "Pn"):
+        e = rel_l2(gpu.get(nm), ref.get(nm))
+        assert e <= 1e-6, (nm, e)
+
+
 # ---- transient runs (src/navier_velocity.cpp:103-118, navier_scalar.cpp:
 # 72-91,352-367, HeatConductionSolver.cpp:68-84, navier.cpp:13-97,190-241) --
 def _transient_pair(kind, scheme, dt, max_outer=3, total=None):

@@ -40,6 +40,9 @@ def parse():
                          "default nx/2, i.e. 1/8 of the cells)")
     ap.add_argument("--learned", type=int, default=1,
                     help="p-solve iterations: learned guess (+ mixed precision) vs plain FP64")
+    ap.add_argument("--coarse", type=int, default=128,
+                    help="aggregates of the two-level pressure CG in the p-solve comparison "
+                         "(0: leave it out)")
     ap.add_argument("--learn-nx", type=int, default=32,
                     help="training mesh nx (ny, nz = nx/2) for the learned guess")
     ap.add_argument("--workload", default="channel",
@@ -133,12 +136,18 @@ def learned_vs_plain(a):
     print(f"[bench] learned guess: {net.num_samples()} samples, loss {first:.3f} -> {last:.3f} "
           f"in {train_s:.1f}s", file=sys.stderr, flush=True)
     iters, res = 3, {}
-    for name, mode, prec in (("plain_fp64", 0, 0), ("learned_fp64", 2, 0),
-                             ("learned_mixed", 2, 1), ("prev_p_fp64", 1, 0)):
+    runs = [("plain_fp64", 0, 0, 0), ("learned_fp64", 2, 0, 0), ("learned_mixed", 2, 1, 0),
+            ("prev_p_fp64", 1, 0, 0)]
+    if a.coarse:  # the two-level CG (aggregate coarse correction) with and without the guess
+        runs += [("two_level_fp64", 0, 0, a.coarse), ("learned_two_level_fp64", 2, 0, a.coarse),
+                 ("learned_two_level_mixed", 2, 1, a.coarse)]
+    for name, mode, prec, coarse in runs:
         raw, p = cases.channel(nx=a.nx, ny=a.ny, nz=a.nz, max_step=iters + 2, u_in=1.25)
         p.pres_precision = prec
         s = P.NSSolver(raw, p)
         s.set_guess(mode, net if mode == 2 else None)
+        if coarse:
+            s.set_option("coarse", coarse)
         worst, secs = 0.0, 0.0
         for _ in range(iters):
             s.sync()
@@ -161,7 +170,15 @@ def learned_vs_plain(a):
             "plain_over_learned_mixed": plain / max(res["learned_mixed"]["p_iters"], 1),
             "plain_over_learned_fp64": plain / max(res["learned_fp64"]["p_iters"], 1),
             "time_plain_over_learned_mixed": res["plain_fp64"]["ms_per_step"]
-            / max(res["learned_mixed"]["ms_per_step"], 1e-9)}
+            / max(res["learned_mixed"]["ms_per_step"], 1e-9),
+            **({"coarse_aggregates_target": a.coarse,
+                "plain_over_learned_two_level_mixed":
+                    plain / max(res["learned_two_level_mixed"]["p_iters"], 1),
+                "plain_over_two_level_fp64": plain / max(res["two_level_fp64"]["p_iters"], 1),
+                "time_plain_over_learned_two_level_mixed": res["plain_fp64"]["ms_per_step"]
+                / max(res["learned_two_level_mixed"]["ms_per_step"], 1e-9),
+                "time_plain_over_two_level_fp64": res["plain_fp64"]["ms_per_step"]
+                / max(res["two_level_fp64"]["ms_per_step"], 1e-9)} if a.coarse else {})}
 
 
 def _line(metric, value, unit, a, ms, workload, extra):

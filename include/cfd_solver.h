@@ -268,7 +268,14 @@ int cfd_ns_profile_read(const cfd_ns *h, double out[8]);
  *   has one; 0: the point multicolour order (default 1; the order changes
  *   the preconditioner, so iteration counts differ, every solve still meets
  *   its tolerance on the true FP64 residual)
- * cfd_ns_get_option also reads "graph_launches" (CG iterations replayed). */
+ *   "coarse" 0..256: two-level pressure CG, the preconditioner (ILU(0) or
+ *   SSOR) plus an additive coarse correction Z E^-1 Z^T r over about this many
+ *   aggregates of cells (boxes over the cell centroids, E = Z^T A Z assembled
+ *   and inverted on the device per solve); 0: off, the reference's one-level
+ *   CG (default 0; FP64 and mixed precision; not with batched cases). Every
+ *   solve still meets its tolerance on the true FP64 residual.
+ * cfd_ns_get_option also reads "graph_launches" (CG iterations replayed) and
+ * "coarse_aggregates" (the aggregates in use). */
 int cfd_ns_set_option(cfd_ns *h, const char *name, double value);
 double cfd_ns_get_option(const cfd_ns *h, const char *name);
 /* the autotuner's choices so far, "kernel@size=threads,..." into buf (len

@@ -73,6 +73,42 @@ struct KState {
 };
 
 // Work vectors of one precision.
+// Two-level additive preconditioner for CG: M^-1 r = B^-1 r + Z E^-1 Z^T r,
+// B^-1 the ILU(0) / SSOR / Jacobi sweep, Z the 0/1 indicator of m aggregates
+// of rows and E = Z^T A Z (dense, inverted on the device once per matrix).
+// The coarse term removes the smooth error the one-level sweeps leave, which
+// is what the pressure-correction CG spends most of its iterations on. An
+// extension with no counterpart in the reference (LASPack's CGIter applies
+// one preconditioner); the solution meets the same stopping test.
+constexpr int kCoarseMax = 256;  // aggregates (E and its inverse: m^2 doubles)
+struct Coarse {
+    int n = 0, m = 0;
+    std::vector<int> agg_host;         // aggregate of each natural row
+    DBuf<int> agg_nat;                 // the same on the device
+    // E = Z^T A Z from the natural-order matrix: entry ekey[e] (a * m + b)
+    // sums the values at epos[eptr[e] .. eptr[e+1]): k * n + i for slot k of
+    // row i, -(i + 1) for the diagonal of row i, in a fixed order
+    DBuf<int> eptr, epos, ekey;
+    int ne = 0;
+    // per solve order (natural or a colouring's permutation): each row's
+    // aggregate and each aggregate's rows
+    const void *order_for = nullptr;
+    bool order_ready = false;
+    DBuf<int> agg, aptr, arows;
+    DBuf<double> E, s, c;  // E holds E^-1 after set-up; s = Z^T r, c = E^-1 s
+    // agg: aggregate of each natural row, in [0, m); ell_col_host: K x n
+    // slot-major columns (padding slots name the row itself)
+    void build(const std::vector<int> &agg, int m, const int *ell_col_host, int n, int K,
+               hipStream_t st);
+    // rows of each aggregate in the order of col (nullptr: natural order)
+    void set_order(const struct Colouring *col, hipStream_t st);
+};
+
+// aggregates of cells from their centroids: a box grid over the bounding box
+// with about `target` boxes (at most kCoarseMax), empty boxes dropped;
+// returns m and fills agg
+int box_aggregates(const double *x, int n, int target, std::vector<int> &agg);
+
 template <class T>
 struct KWork {
     DBuf<T> r, z, p0, az, q, tmp, lu, ud, pdiag, pval, pb, px;
@@ -130,7 +166,17 @@ class Krylov {
     long long prof_launches = 0;
     double prof_bytes_scale = 1.0;  // 0.5 while the FP32 inner solve runs
 
+    // Two-level CG (set by the caller for the systems it suits; used by CG
+    // solves of coarse->n rows with a preconditioner, not in batched mode)
+    Coarse *coarse = nullptr;
+
   private:
+    const Coarse *active_coarse_ = nullptr;
+    // assemble and invert E for this matrix; returns the coarse space the CG
+    // iterations use (nullptr: none)
+    const Coarse *coarse_setup(const Ell &A0, Colouring *col, const cfd_solve_cfg &cfg);
+    template <class T>
+    void coarse_apply(const Coarse *cz, const T *r, T *z, double *sc, const KState *st);
     hipStream_t st_;
     KWork<double> wd_;
     KWork<float> wf_;

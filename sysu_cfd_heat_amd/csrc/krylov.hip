@@ -348,6 +348,126 @@ __global__ void k_cg_rho(int n, const T *__restrict__ r, const T *__restrict__ z
     }
 }
 
+// ------------------------------------------------------- two-level CG --
+// E = Z^T A Z: one wave per nonzero entry, summing its value list (lane l
+// takes entries l, l + 64, ...; the wave sum combines the lanes in a fixed
+// order). E must be zero beforehand.
+__global__ void k_coarse_assemble(int ne, const int *__restrict__ eptr,
+                                  const int *__restrict__ epos, const int *__restrict__ ekey,
+                                  const double *__restrict__ diag,
+                                  const double *__restrict__ val, double *__restrict__ E) {
+    const int e = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (e >= ne) return;
+    double v = 0.;
+    for (int k = eptr[e] + lane; k < eptr[e + 1]; k += 64) {
+        const int q = epos[k];
+        v += q >= 0 ? val[q] : diag[-q - 1];
+    }
+    v = wave_sum(v);
+    if (lane == 0) E[ekey[e]] = v;
+}
+
+__global__ void k_coarse_zero(int n, double *__restrict__ E) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) E[i] = 0.;
+}
+
+// E^-1 in place by Gauss-Jordan elimination without pivoting (E is definite
+// up to sign: diagonal pivots). One workgroup; E sits in global memory, each
+// step a rank-one update between barriers. When every row of E sums to zero
+// (A has the constant null vector, as the pressure correction does without a
+// fixed-pressure boundary), E + tau 1 1^T with tau = mean diagonal / m is
+// inverted instead: the same on the zero-sum vectors Z^T r, and non-singular.
+__global__ void __launch_bounds__(1024) k_coarse_invert(int m, double *__restrict__ E) {
+    __shared__ double rowk[kCoarseMax], colk[kCoarseMax];
+    __shared__ int nonzero_sum;
+    const int t = threadIdx.x, B = blockDim.x;
+    if (t == 0) nonzero_sum = 0;
+    __syncthreads();
+    for (int a = t; a < m; a += B) {
+        double rs = 0.;
+        for (int b = 0; b < m; ++b) rs += E[a * m + b];
+        if (!(fabs(rs) <= 1e-9 * fabs(E[a * m + a]))) nonzero_sum = 1;
+    }
+    __syncthreads();
+    if (!nonzero_sum) {
+        double tr = 0.;
+        for (int a = 0; a < m; ++a) tr += E[a * m + a];
+        const double tau = tr / ((double)m * m);
+        __syncthreads();  // every thread has read the diagonal
+        for (int q = t; q < m * m; q += B) E[q] += tau;
+        __syncthreads();
+    }
+    for (int k = 0; k < m; ++k) {
+        const double piv = E[k * m + k];
+        for (int j = t; j < m; j += B) {
+            rowk[j] = (j == k ? 1. : E[k * m + j]) / piv;
+            colk[j] = E[j * m + k];
+        }
+        __syncthreads();
+        for (int q = t; q < m * m; q += B) {
+            const int i = q / m, j = q - i * m;
+            E[q] = i == k ? rowk[j] : (j == k ? 0. : E[q]) - colk[i] * rowk[j];
+        }
+        __syncthreads();
+    }
+}
+
+// s_a = sum of r over aggregate a's rows (one workgroup per aggregate, a
+// fixed order)
+template <class T>
+__global__ void __launch_bounds__(1024) k_coarse_restrict(const int *__restrict__ aptr,
+                                                          const int *__restrict__ arows,
+                                                          const T *__restrict__ r,
+                                                          double *__restrict__ s,
+                                                          const KState *st) {
+    if (st && st->done) return;
+    const int a = blockIdx.x, b0 = aptr[a], cnt = aptr[a + 1] - b0;
+    double v[1] = {0.};
+    for (int q = threadIdx.x; q < cnt; q += blockDim.x) v[0] += (double)r[arows[b0 + q]];
+    block_sum<1>(v);
+    if (threadIdx.x == 0) s[a] = v[0];
+}
+
+// c = E^-1 s and *sc = s.c (r.(Z c), the coarse term of rho = r.z). One
+// workgroup; TPR threads per row of E^-1 combine their column shares in a
+// fixed order.
+__global__ void __launch_bounds__(1024) k_coarse_solve(int m, const double *__restrict__ Ei,
+                                                       const double *__restrict__ s,
+                                                       double *__restrict__ c, double *sc,
+                                                       const KState *st) {
+    if (st && st->done) return;
+    __shared__ double ss[kCoarseMax];
+    const int t = threadIdx.x, B = blockDim.x;
+    for (int a = t; a < m; a += B) ss[a] = s[a];
+    __syncthreads();
+    int tpr = 1;
+    while (tpr < 64 && tpr * 2 * m <= B) tpr *= 2;
+    const int part = t % tpr;
+    double dot[1] = {0.};
+    for (int a0 = 0; a0 < m; a0 += B / tpr) {
+        const int a = a0 + t / tpr;
+        double acc = 0.;
+        if (a < m)
+            for (int b = part; b < m; b += tpr) acc += Ei[(size_t)a * m + b] * ss[b];
+        for (int o = tpr >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (a < m && part == 0) {
+            c[a] = acc;
+            dot[0] += ss[a] * acc;
+        }
+    }
+    block_sum<1>(dot);
+    if (t == 0) *sc = dot[0];
+}
+
+// z += Z c
+template <class T>
+__global__ void k_coarse_prolong(int n, T *__restrict__ z, const int *__restrict__ agg,
+                                 const double *__restrict__ c, const KState *st) {
+    if (st && st->done) return;
+    GRID_LOOP(i, n) z[i] += (T)c[agg[i]];
+}
+
 // ------------------------------------------------------------- BiCGSTAB --
 // p = r (first) or r + beta (p - omega v); Jacobi: ph = p / d
 template <bool JAC>
@@ -1852,9 +1972,11 @@ SolveStats Krylov::solve(const Ell &A0, const double *b0, double *x0, const cfd_
         build_segments(perm ? col : nullptr, n);
     }
     SolveStats S;
+    active_coarse_ = coarse_setup(A0, col, cfg);
     if (cfg.method == CFD_CG) S = cg(A, b, x, cfg, col, pc, wd_, per_case);
     else if (cfg.method == CFD_BICGSTAB) S = bicgstab(A, b, x, cfg, col, pc, per_case);
     else S = gmres(A, b, x, cfg, col, pc);
+    active_coarse_ = nullptr;
     S.relres = per_case ? case_relres(A, x, b) : relres(A, x, b);
     if (perm) {
         hipLaunchKernelGGL(k_perm_scatter, dim3(grid_for(n)), dim3(kBlock), 0, st_, n,
@@ -1895,6 +2017,7 @@ SolveStats Krylov::solve_mixed(const Ell &A0, const double *b0, double *x0,
     CFD_LAUNCH_CHECK();
     EllF Af{n, A.K, wf_.pdiag.p, wf_.pval.p, A.col};
     setup_pc(Af, pc, col, wf_, cfg.omega);
+    active_coarse_ = coarse_setup(A0, col, cfg);
     SolveStats S;
     int total = 0;
     double rel = 0.;
@@ -1931,6 +2054,7 @@ SolveStats Krylov::solve_mixed(const Ell &A0, const double *b0, double *x0,
         S.refinements++;
         if (si.iters == 0) break;
     }
+    active_coarse_ = nullptr;
     S.iters = total;
     S.relres = rel;
     S.rtcres = rel;
@@ -1943,6 +2067,148 @@ SolveStats Krylov::solve_mixed(const Ell &A0, const double *b0, double *x0,
     return S;
 }
 
+// ------------------------------------------------- two-level CG (host) --
+int box_aggregates(const double *x, int n, int target, std::vector<int> &agg) {
+    agg.assign(n, 0);
+    if (n == 0) return 0;
+    target = std::max(1, std::min(target, kCoarseMax));
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) lo[d] = hi[d] = x[d];
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], x[3 * i + d]);
+            hi[d] = std::max(hi[d], x[3 * i + d]);
+        }
+    double len[3], vol = 1.;
+    for (int d = 0; d < 3; ++d) {
+        len[d] = std::max(hi[d] - lo[d], 1e-300);
+        vol *= len[d];
+    }
+    // boxes about as long as wide: g_d proportional to the extent
+    for (double scale = std::cbrt((double)target / vol);; scale *= 0.95) {
+        int g[3];
+        long long tot = 1;
+        for (int d = 0; d < 3; ++d) {
+            g[d] = std::max(1, (int)std::lround(len[d] * scale));
+            tot *= g[d];
+        }
+        if (tot > kCoarseMax && scale > 0) continue;
+        std::vector<int> id(tot, -1);
+        int m = 0;
+        for (int i = 0; i < n; ++i) {
+            long long key = 0;
+            for (int d = 0; d < 3; ++d) {
+                int k = (int)((x[3 * i + d] - lo[d]) / len[d] * g[d]);
+                key = key * g[d] + std::min(std::max(k, 0), g[d] - 1);
+            }
+            if (id[key] < 0) id[key] = m++;
+            agg[i] = id[key];
+        }
+        return m;
+    }
+}
+
+void Coarse::build(const std::vector<int> &a, int m_, const int *ell_col_host, int n_, int K,
+                   hipStream_t st) {
+    if (m_ < 1 || m_ > kCoarseMax || (int)a.size() != n_)
+        throw HipError("coarse space: 1 to 256 aggregates, one per row");
+    n = n_;
+    m = m_;
+    agg_host = a;
+    // E's value lists by counting sort on the key a * m + b (row order, then
+    // diagonal before slots, within a key)
+    std::vector<int> cnt((size_t)m * m + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        ++cnt[(size_t)a[i] * m + a[i] + 1];
+        for (int k = 0; k < K; ++k) {
+            const int j = ell_col_host[(size_t)k * n + i];
+            if (j != i) ++cnt[(size_t)a[i] * m + a[j] + 1];
+        }
+    }
+    std::vector<int> key, ptr{0};
+    for (size_t q = 0; q < (size_t)m * m; ++q)
+        if (cnt[q + 1]) {
+            key.push_back((int)q);
+            ptr.push_back(ptr.back() + cnt[q + 1]);
+        }
+    for (size_t q = 1; q <= (size_t)m * m; ++q) cnt[q] += cnt[q - 1];
+    std::vector<int> pos(cnt[(size_t)m * m]);
+    for (int i = 0; i < n; ++i) {
+        pos[cnt[(size_t)a[i] * m + a[i]]++] = -(i + 1);
+        for (int k = 0; k < K; ++k) {
+            const int j = ell_col_host[(size_t)k * n + i];
+            if (j != i) pos[cnt[(size_t)a[i] * m + a[j]]++] = k * n + i;
+        }
+    }
+    ne = (int)key.size();
+    auto up = [&](DBuf<int> &d, const std::vector<int> &h) {
+        d.alloc(h.size());
+        d.upload(h.data(), h.size(), st);
+    };
+    up(eptr, ptr);
+    up(epos, pos);
+    up(ekey, key);
+    up(agg_nat, a);
+    E.alloc((size_t)m * m);
+    s.alloc(m);
+    c.alloc(m);
+    order_ready = false;
+}
+
+void Coarse::set_order(const Colouring *col, hipStream_t st) {
+    if (order_ready && order_for == (const void *)col) return;
+    std::vector<int> perm(n);
+    if (col) col->perm.download(perm.data(), n, st);
+    else
+        for (int i = 0; i < n; ++i) perm[i] = i;
+    std::vector<int> ag(n), ptr(m + 1, 0), rows(n);
+    for (int p = 0; p < n; ++p) {
+        ag[p] = agg_host[perm[p]];
+        ++ptr[ag[p] + 1];
+    }
+    for (int a = 0; a < m; ++a) ptr[a + 1] += ptr[a];
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int p = 0; p < n; ++p) rows[fill[ag[p]]++] = p;
+    auto up = [&](DBuf<int> &d, const std::vector<int> &h) {
+        d.alloc(h.size());
+        d.upload(h.data(), h.size(), st);
+    };
+    up(agg, ag);
+    up(aptr, ptr);
+    up(arows, rows);
+    order_for = col;
+    order_ready = true;
+}
+
+const Coarse *Krylov::coarse_setup(const Ell &A0, Colouring *col, const cfd_solve_cfg &cfg) {
+    Coarse *cz = coarse;
+    if (!cz || cz->n != A0.n || cfg.method != CFD_CG || batched() ||
+        (cfg.precond != CFD_PC_ILU && cfg.precond != CFD_PC_SSOR))
+        return nullptr;
+    cz->set_order(col, st_);
+    const int m = cz->m;
+    hipLaunchKernelGGL(k_coarse_zero, dim3(grid_for((long long)m * m)), dim3(kBlock), 0, st_,
+                       m * m, cz->E.p);
+    hipLaunchKernelGGL(k_coarse_assemble, dim3(grid_for((long long)cz->ne * 64)), dim3(kBlock),
+                       0, st_, cz->ne, (const int *)cz->eptr.p, (const int *)cz->epos.p,
+                       (const int *)cz->ekey.p, A0.diag, A0.val, cz->E.p);
+    hipLaunchKernelGGL(k_coarse_invert, dim3(1), dim3(1024), 0, st_, m, cz->E.p);
+    CFD_LAUNCH_CHECK();
+    return cz;
+}
+
+// z += Z E^-1 Z^T r and sc = r.(Z E^-1 Z^T r), after the one-level sweep
+// has written z
+template <class T>
+void Krylov::coarse_apply(const Coarse *cz, const T *r, T *z, double *sc, const KState *st) {
+    hipLaunchKernelGGL(k_coarse_restrict<T>, dim3(cz->m), dim3(1024), 0, st_,
+                       (const int *)cz->aptr.p, (const int *)cz->arows.p, r, cz->s.p, st);
+    hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(1024), 0, st_, cz->m,
+                       (const double *)cz->E.p, (const double *)cz->s.p, cz->c.p, sc, st);
+    hipLaunchKernelGGL(k_coarse_prolong<T>, dim3(grid_for(cz->n)), dim3(kBlock), 0, st_, cz->n, z,
+                       (const int *)cz->agg.p, (const double *)cz->c.p, st);
+}
+
 template <class T>
 SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &cfg,
                       Colouring *col, int pc, KWork<T> &w, bool per_case) {
@@ -1950,7 +2216,11 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
     const double dev_tol = per_case ? 0. : cfg.tol;  // batched: the host decides
     const int n = A.n, G = red_grid(n), Gs = spmv_grid(n);
     const bool jac = pc == CFD_PC_JACOBI;
-    const int c0 = az_rows(A, pc, col);  // rows whose (A z)_i the preconditioner leaves in az
+    // two-level: z = B^-1 r + Z E^-1 Z^T r (the coarse term is added after the
+    // sweep, so no row's (A z)_i comes from the sweep)
+    const Coarse *cz = (!per_case && !jac && active_coarse_ && active_coarse_->n == n)
+                           ? active_coarse_ : nullptr;
+    const int c0 = cz ? 0 : az_rows(A, pc, col);  // rows whose (A z)_i the preconditioner leaves in az
     RowMap rm;  // block-colour order: each XCD takes an eighth of every colour's blocks
     if (pc == CFD_PC_ILU && col && col->block > 1 && col->ncolour <= 4 && Gs % 8 == 0) {
         bool ok = true;
@@ -1970,11 +2240,16 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
     auto precond_rho_partials = [&](const KState *stp) -> int {
         int nbk = -1;
         apply_pc(A, pc, cfg.omega, (const T *)w.r.p, w.z.p, col, w,
-                 (pc == CFD_PC_ILU || pc == CFD_PC_SSOR) ? &nbk : nullptr, stp, w.az.p);
+                 (pc == CFD_PC_ILU || pc == CFD_PC_SSOR) ? &nbk : nullptr, stp,
+                 c0 > 0 ? w.az.p : nullptr);
         if (nbk < 0) {
             nbk = G;
             hipLaunchKernelGGL(k_cg_rho<T>, dim3(G), dim3(kBlock), 0, st_, n, (const T *)w.r.p,
                                (const T *)w.z.p, state_.p, rpart, 0);
+        }
+        if (cz) {  // r.(Z c) is one more partial of rho
+            coarse_apply<T>(cz, (const T *)w.r.p, w.z.p, rpart + nbk, stp);
+            ++nbk;
         }
         return nbk;
     };
@@ -2021,7 +2296,8 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
             (uintptr_t)w.az.p,
             (uintptr_t)n, (uintptr_t)A.K, (uintptr_t)pc, (uintptr_t)col,
             (uintptr_t)(col ? col->ncolour : 0), (uintptr_t)(col ? col->triangle_free : 0),
-            (uintptr_t)(col ? col->block : 0)};
+            (uintptr_t)(col ? col->block : 0), (uintptr_t)cz,
+            (uintptr_t)(cz ? cz->agg.p : nullptr), (uintptr_t)(cz ? cz->m : 0)};
         uint64_t om;
         std::memcpy(&om, &cfg.omega, sizeof om);
         key.push_back((uintptr_t)om);

@@ -112,6 +112,11 @@ class NSSolver {
     // ILU(0) and SSOR of the fluid and the solid in block-colour order where
     // the mesh has one (else the point colouring)
     bool block_order = true;
+    // two-level pressure CG: about coarse_target aggregates of cells (boxes
+    // over the centroids), 0: off
+    int coarse_target = 0;
+    Coarse coarse;
+    int set_coarse(int target);
     int run_solid(const std::function<int()> &f, bool async);
     int solid_join();
     // perfect gas (DensityModel 1): rho = p / (Rcpcv T), Rcpcv = cp - cp / gama

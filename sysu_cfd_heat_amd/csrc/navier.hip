@@ -447,6 +447,20 @@ Colouring *colouring_of(const HostMesh &hm, const DevMeshOwner &dmo, int pc, boo
 }
 }  // namespace
 
+int NSSolver::set_coarse(int target) {
+    if (target > 0 && cases.ncase() > 1) {
+        last_error() = "coarse: the batched cases solve per case";
+        return -1;
+    }
+    coarse_target = target;
+    if (target > 0) {
+        std::vector<int> agg;
+        const int m = box_aggregates(hm.cell_x.data(), hm.ncel, target, agg);
+        coarse.build(agg, m, dmo.ell_col_host.data(), hm.ncel, dmo.dm.K, s);
+    }
+    return 0;
+}
+
 Colouring *NSSolver::colouring_for(const cfd_solve_cfg &c) {
     return colouring_of(hm, dmo, c.precond, block_order, colour, colour_ready, bcolour,
                         bcolour_ready, s);
@@ -569,7 +583,9 @@ int NSSolver::CalculatePressure() {
     kry->prof = prof_pressure;
     cfd_solve_cfg pc = P.pres;
     if (gas() && pc.method == CFD_CG) pc.method = CFD_GMRES;
+    kry->coarse = coarse_target > 0 ? &coarse : nullptr;
     last_p = solve(Ap(), bp, xsol, pc, colouring_for(pc), P.pres_precision);
+    kry->coarse = nullptr;
     kry->prof = false;
     stats[3] += last_p.iters;
     if (!(last_p.relres <= 1.)) {
@@ -1198,6 +1214,13 @@ int cfd_ns_set_option(cfd_ns *h, const char *name, double value) {
         h->ns.block_order = value != 0.;
         return 0;
     }
+    if (k == "coarse") {
+        if (!(value >= 0. && value <= kCoarseMax) || value != (double)(int)value) {
+            last_error() = "coarse must be an aggregate count in 0..256";
+            return -1;
+        }
+        return guarded([&] { return h->ns.set_coarse((int)value); });
+    }
     if (k == "mixed_inner_tol") {
         if (!(value > 0. && value < 1.)) {
             last_error() = "mixed_inner_tol must lie in (0, 1)";
@@ -1218,6 +1241,8 @@ double cfd_ns_get_option(const cfd_ns *h, const char *name) {
     if (k == "mixed_inner_tol") return h->ns.mixed_inner_tol;
     if (k == "autotune") return get_autotune() ? 1. : 0.;
     if (k == "block_order") return h->ns.block_order ? 1. : 0.;
+    if (k == "coarse") return (double)h->ns.coarse_target;
+    if (k == "coarse_aggregates") return h->ns.coarse_target ? (double)h->ns.coarse.m : 0.;
     if (k == "graph_launches") return (double)h->ns.kry->graph_launches;
     last_error() = "unknown option " + k;
     return std::nan("");

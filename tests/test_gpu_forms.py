@@ -228,3 +228,56 @@ def test_block_order_against_point_order(precond):
     for nm in a:
         e = rel_l2(b[nm], a[nm])
         assert e <= 1e-6, (nm, e)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_two_level_pressure(precision):
+    """Two-level pressure CG (ILU(0) plus an aggregate coarse correction):
+    every solve meets the same true FP64 tolerance, the fields agree with the
+    one-level FP64 solve within it, it takes fewer iterations, and graph
+    replay gives the eager launches' results bit for bit."""
+    import sysu_cfd_heat_amd as P
+    from sysu_cfd_heat_amd import cases
+    print(f"[forms] two-level pressure precision={precision}", flush=True)
+    raw, p = cases.channel(nx=24, ny=12, nz=12, max_step=6)
+    p.pres_precision = 0
+    a, ra, sa = _run(raw, p, 3)
+    p.pres_precision = precision
+    out = []
+    for graphs in (0, 1):
+        s = P.NSSolver(raw, p)
+        s.set_option("graphs", graphs)
+        s.set_option("coarse", 64)
+        assert 1 <= s.get_option("coarse_aggregates") <= 64
+        res = []
+        for _ in range(3):
+            s.call("Iterate")
+            res.append(s.last_solve("p"))
+        out.append(({nm: s.get(nm).copy() for nm in ("Un", "Vn", "Wn", "Pn")}, res,
+                    s.stats().copy()))
+        s.close()
+    (b, rb, sb), (c, rc, sc) = out
+    for (it, rr) in rb:
+        assert rr <= p.pres.tol, rr
+    for nm in a:
+        assert rel_l2(b[nm], a[nm]) <= 1e-6, (nm, rel_l2(b[nm], a[nm]))
+        assert np.array_equal(b[nm], c[nm]), nm
+    assert rb == rc
+    if precision == 0:
+        assert sb[3] < sa[3], (sb[3], sa[3])  # pressure iterations
+
+
+def test_two_level_option_errors():
+    import sysu_cfd_heat_amd as P
+    from sysu_cfd_heat_amd import cases
+    raw, p = cases.cavity(n=4, max_step=2)
+    s = P.NSSolver(raw, p)
+    with pytest.raises(RuntimeError):
+        s.set_option("coarse", 257)
+    with pytest.raises(RuntimeError):
+        s.set_option("coarse", 1.5)
+    s.set_option("coarse", 8)
+    assert s.get_option("coarse") == 8
+    s.set_option("coarse", 0)
+    s.call("Iterate")
+    s.close()

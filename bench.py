@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--coarse", type=int, default=128,
                     help="aggregates of the two-level pressure CG in the p-solve comparison "
                          "(0: leave it out)")
+    ap.add_argument("--pres-coarse", type=int, default=0,
+                    help="timed run: two-level pressure CG over this many aggregates "
+                         "(0: the reference's one-level CG)")
     ap.add_argument("--learn-nx", type=int, default=32,
                     help="training mesh nx (ny, nz = nx/2) for the learned guess")
     ap.add_argument("--workload", default="channel",
@@ -356,6 +359,7 @@ def main():
     gpu.set_option("autotune", a.autotune)
     gpu.set_option("graphs", a.graphs)
     gpu.set_option("block_order", a.block_order)
+    gpu.set_option("coarse", a.pres_coarse)
     setup_s = time.time() - t0
     print(f"[bench] setup {setup_s:.1f}s cells={raw.counts[1]} fluid={gpu.counts(0)[1]} "
           f"solid={gpu.counts(1)[1]}", file=sys.stderr, flush=True)
@@ -391,6 +395,31 @@ def main():
               "solid_iters_per_step": st[6] / a.steps}
     gpu.close()
     roof["traffic"] = measured_traffic(a)
+
+    two_level = None
+    if a.coarse and not a.pres_coarse:
+        # the same timed run with the two-level pressure CG
+        g2 = NSSolver(raw, p)
+        for nm, v in (("mixed_inner_tol", a.mixed_inner_tol), ("autotune", a.autotune),
+                      ("graphs", a.graphs), ("block_order", a.block_order), ("coarse", a.coarse)):
+            g2.set_option(nm, v)
+        for k in range(a.warmup):
+            g2.call("Iterate")
+        g2.sync()
+        g2._fn("reset_stats")(g2.h)
+        t3 = time.perf_counter()
+        for k in range(a.steps):
+            g2.call("Iterate")
+        g2.sync()
+        el2 = time.perf_counter() - t3
+        st2 = g2.stats()
+        two_level = {"value": a.steps / el2, "unit": "SIMPLE iters/s",
+                     "ms_per_step": 1000.0 * el2 / a.steps,
+                     "aggregates": int(g2.get_option("coarse_aggregates")),
+                     "p_iters_per_step": st2[3] / a.steps,
+                     "speedup_vs_one_level": (a.steps / el2) / value}
+        g2.close()
+        print(f"[bench] two-level pressure CG: {two_level}", file=sys.stderr, flush=True)
 
     learned = learned_vs_plain(a) if a.learned else None
 
@@ -432,8 +461,11 @@ def main():
                    "solvers": "BiCGSTAB+Jacobi / CG+ILU(0), 1e-8" if a.solvers == "config" else
                    "reference: GMRES+SSOR / CG+SSOR(1.3) 1e-6",
                    "sweep_order": "block-colour (6-tet blocks)" if a.block_order else "point colour",
+                   "pressure_cg": (f"two-level, {a.pres_coarse} aggregates" if a.pres_coarse
+                                   else "one-level (the reference's)"),
                    "parallelism": "single GPU"},
-        "roofline": roof, "cpu_baseline": cpu, "krylov": krylov, "p_iters_learned": learned,
+        "roofline": roof, "cpu_baseline": cpu, "krylov": krylov,
+        "two_level_pressure": two_level, "p_iters_learned": learned,
         "device_s_per_step": st[1] / max(st[0], 1),
         "autotune": __import__("sysu_cfd_heat_amd").autotune_choices(),
     }

@@ -95,7 +95,8 @@ struct Coarse {
     const void *order_for = nullptr;
     bool order_ready = false;
     DBuf<int> agg, aptr, arows;
-    DBuf<double> E, s, c;  // E holds E^-1 after set-up; s = Z^T r, c = E^-1 s
+    // E holds E^-1 after set-up; spart: partial sums of s = Z^T r; c = E^-1 s
+    DBuf<double> E, spart, c;
     // agg: aggregate of each natural row, in [0, m); ell_col_host: K x n
     // slot-major columns (padding slots name the row itself)
     void build(const std::vector<int> &agg, int m, const int *ell_col_host, int n, int K,
@@ -176,7 +177,7 @@ class Krylov {
     // iterations use (nullptr: none)
     const Coarse *coarse_setup(const Ell &A0, Colouring *col, const cfd_solve_cfg &cfg);
     template <class T>
-    void coarse_apply(const Coarse *cz, const T *r, T *z, double *sc, const KState *st);
+    int coarse_apply(const Coarse *cz, const T *r, double *sc, const KState *st);
     hipStream_t st_;
     KWork<double> wd_;
     KWork<float> wf_;

@@ -49,6 +49,38 @@ __device__ __forceinline__ T row_sum_k(const EllT<T> &A, int i, const T *__restr
     for (int k = 0; k < KK; ++k) s += v[k];
     return s;
 }
+// The two-level CG's coarse term Z c, added to every z the sparse product
+// reads (z holds the one-level sweep's result): z_k + c[agg[k]]
+struct CoarseTerm {
+    const int *agg = nullptr;
+    const double *c = nullptr;
+};
+template <class T, int KK>
+__device__ __forceinline__ T row_sum_ck(const EllT<T> &A, int i, const T *__restrict__ x,
+                                        CoarseTerm ct, T s) {
+    const size_t n = A.n;
+    int c[KK];
+    T v[KK];
+#pragma unroll
+    for (int k = 0; k < KK; ++k) c[k] = A.col[k * n + i];
+#pragma unroll
+    for (int k = 0; k < KK; ++k) v[k] = A.val[k * n + i] * (x[c[k]] + (T)ct.c[ct.agg[c[k]]]);
+#pragma unroll
+    for (int k = 0; k < KK; ++k) s += v[k];
+    return s;
+}
+template <class T>
+__device__ __forceinline__ T row_sum_c(const EllT<T> &A, int i, const T *__restrict__ x,
+                                       CoarseTerm ct, T s) {
+    if (A.K == 4) return row_sum_ck<T, 4>(A, i, x, ct, s);
+    if (A.K == 6) return row_sum_ck<T, 6>(A, i, x, ct, s);
+    const size_t n = A.n;
+    for (int k = 0; k < A.K; ++k) {
+        const int j = A.col[k * n + i];
+        s += A.val[k * n + i] * (x[j] + (T)ct.c[ct.agg[j]]);
+    }
+    return s;
+}
 template <class T>
 __device__ __forceinline__ T row_sum(const EllT<T> &A, int i, const T *__restrict__ x, T s) {
     if (A.K == 4) return row_sum_k<T, 4>(A, i, x, s);
@@ -211,10 +243,11 @@ __device__ __forceinline__ int map_row(const RowMap &m, int x, int lx) {
     return m.off[c] + j * m.nb[c] + x * nbx + (rem - j * nbx);
 }
 
-template <class T>
+// CZ (two-level CG): z + Z c in place of z throughout (c0 is then 0).
+template <class T, bool CZ>
 __global__ void k_cg_pq(EllT<T> A, const T *__restrict__ z, T *__restrict__ p, T *__restrict__ q,
                         T *__restrict__ x, const T *__restrict__ az, int c0, RowMap rm,
-                        const KState *st, double *part) {
+                        CoarseTerm ct, const KState *st, double *part) {
     if (st->done) return;
     const T beta = (T)st->beta, alpha = (T)st->alpha;
     const bool upd = st->iter > 0;
@@ -232,9 +265,19 @@ __global__ void k_cg_pq(EllT<T> A, const T *__restrict__ z, T *__restrict__ p, T
         const int i = mapped ? map_row(rm, xc, li) : li, lj = li + stride;
         const bool hj = lj < nl;
         const int j = !hj ? 0 : mapped ? map_row(rm, xc, lj) : lj;
-        const T zi = z[i], zj = hj ? z[j] : T(0);
-        T si = i < c0 ? az[i] : row_sum(A, i, z, A.diag[i] * zi);
-        T sj = hj ? (j < c0 ? az[j] : row_sum(A, j, z, A.diag[j] * zj)) : T(0);
+        T zi = z[i], zj = hj ? z[j] : T(0);
+        T si, sj = T(0);
+        if (CZ) {
+            zi += (T)ct.c[ct.agg[i]];
+            si = row_sum_c(A, i, z, ct, A.diag[i] * zi);
+            if (hj) {
+                zj += (T)ct.c[ct.agg[j]];
+                sj = row_sum_c(A, j, z, ct, A.diag[j] * zj);
+            }
+        } else {
+            si = i < c0 ? az[i] : row_sum(A, i, z, A.diag[i] * zi);
+            sj = hj ? (j < c0 ? az[j] : row_sum(A, j, z, A.diag[j] * zj)) : T(0);
+        }
         const T poi = p[i];
         if (upd) x[i] += alpha * poi;
         const T pi = zi + beta * poi;
@@ -413,59 +456,54 @@ __global__ void __launch_bounds__(1024) k_coarse_invert(int m, double *__restric
     }
 }
 
-// s_a = sum of r over aggregate a's rows (one workgroup per aggregate, a
+// Partial sums of s = Z^T r: workgroup (a, j) sums the j-th of kCoarseChunks
+// equal chunks of aggregate a's rows into spart[a * kCoarseChunks + j] (a
 // fixed order)
+constexpr int kCoarseChunks = 8;
 template <class T>
-__global__ void __launch_bounds__(1024) k_coarse_restrict(const int *__restrict__ aptr,
-                                                          const int *__restrict__ arows,
-                                                          const T *__restrict__ r,
-                                                          double *__restrict__ s,
-                                                          const KState *st) {
+__global__ void k_coarse_restrict(const int *__restrict__ aptr, const int *__restrict__ arows,
+                                  const T *__restrict__ r, double *__restrict__ spart,
+                                  const KState *st) {
     if (st && st->done) return;
-    const int a = blockIdx.x, b0 = aptr[a], cnt = aptr[a + 1] - b0;
+    const int a = blockIdx.x / kCoarseChunks, j = blockIdx.x % kCoarseChunks;
+    const int b0 = aptr[a], cnt = aptr[a + 1] - b0;
+    const int len = (cnt + kCoarseChunks - 1) / kCoarseChunks;
+    const int q0 = b0 + j * len, q1 = b0 + min(cnt, (j + 1) * len);
     double v[1] = {0.};
-    for (int q = threadIdx.x; q < cnt; q += blockDim.x) v[0] += (double)r[arows[b0 + q]];
+    for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x) v[0] += (double)r[arows[q]];
     block_sum<1>(v);
-    if (threadIdx.x == 0) s[a] = v[0];
+    if (threadIdx.x == 0) spart[blockIdx.x] = v[0];
 }
 
-// c = E^-1 s and *sc = s.c (r.(Z c), the coarse term of rho = r.z). One
-// workgroup; TPR threads per row of E^-1 combine their column shares in a
-// fixed order.
-__global__ void __launch_bounds__(1024) k_coarse_solve(int m, const double *__restrict__ Ei,
-                                                       const double *__restrict__ s,
-                                                       double *__restrict__ c, double *sc,
-                                                       const KState *st) {
+// c = E^-1 s and the partials of s.c = r.(Z c), the coarse term of rho =
+// r.z: workgroup g takes rows [g * kCoarseRows, (g + 1) * kCoarseRows) of
+// E^-1, each row by blockDim / kCoarseRows threads combined in a fixed order,
+// after summing the chunk partials of s itself.
+constexpr int kCoarseRows = 8;
+__global__ void k_coarse_solve(int m, const double *__restrict__ Ei,
+                               const double *__restrict__ spart, double *__restrict__ c,
+                               double *__restrict__ scpart, const KState *st) {
     if (st && st->done) return;
     __shared__ double ss[kCoarseMax];
     const int t = threadIdx.x, B = blockDim.x;
-    for (int a = t; a < m; a += B) ss[a] = s[a];
+    for (int b = t; b < m; b += B) {
+        double v = 0.;
+        for (int j = 0; j < kCoarseChunks; ++j) v += spart[b * kCoarseChunks + j];
+        ss[b] = v;
+    }
     __syncthreads();
-    int tpr = 1;
-    while (tpr < 64 && tpr * 2 * m <= B) tpr *= 2;
-    const int part = t % tpr;
+    const int tpr = B / kCoarseRows, part = t % tpr, a = blockIdx.x * kCoarseRows + t / tpr;
+    double acc = 0.;
+    if (a < m)
+        for (int b = part; b < m; b += tpr) acc += Ei[(size_t)a * m + b] * ss[b];
+    for (int o = tpr >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
     double dot[1] = {0.};
-    for (int a0 = 0; a0 < m; a0 += B / tpr) {
-        const int a = a0 + t / tpr;
-        double acc = 0.;
-        if (a < m)
-            for (int b = part; b < m; b += tpr) acc += Ei[(size_t)a * m + b] * ss[b];
-        for (int o = tpr >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-        if (a < m && part == 0) {
-            c[a] = acc;
-            dot[0] += ss[a] * acc;
-        }
+    if (a < m && part == 0) {
+        c[a] = acc;
+        dot[0] = ss[a] * acc;
     }
     block_sum<1>(dot);
-    if (t == 0) *sc = dot[0];
-}
-
-// z += Z c
-template <class T>
-__global__ void k_coarse_prolong(int n, T *__restrict__ z, const int *__restrict__ agg,
-                                 const double *__restrict__ c, const KState *st) {
-    if (st && st->done) return;
-    GRID_LOOP(i, n) z[i] += (T)c[agg[i]];
+    if (t == 0) scpart[blockIdx.x] = dot[0];
 }
 
 // ------------------------------------------------------------- BiCGSTAB --
@@ -2150,7 +2188,7 @@ void Coarse::build(const std::vector<int> &a, int m_, const int *ell_col_host, i
     up(ekey, key);
     up(agg_nat, a);
     E.alloc((size_t)m * m);
-    s.alloc(m);
+    spart.alloc((size_t)m * kCoarseChunks);
     c.alloc(m);
     order_ready = false;
 }
@@ -2197,16 +2235,16 @@ const Coarse *Krylov::coarse_setup(const Ell &A0, Colouring *col, const cfd_solv
     return cz;
 }
 
-// z += Z E^-1 Z^T r and sc = r.(Z E^-1 Z^T r), after the one-level sweep
-// has written z
+// c = E^-1 Z^T r (k_cg_pq adds Z c to the z it reads) and the partials of
+// r.(Z c) at sc; returns their count
 template <class T>
-void Krylov::coarse_apply(const Coarse *cz, const T *r, T *z, double *sc, const KState *st) {
-    hipLaunchKernelGGL(k_coarse_restrict<T>, dim3(cz->m), dim3(1024), 0, st_,
-                       (const int *)cz->aptr.p, (const int *)cz->arows.p, r, cz->s.p, st);
-    hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(1024), 0, st_, cz->m,
-                       (const double *)cz->E.p, (const double *)cz->s.p, cz->c.p, sc, st);
-    hipLaunchKernelGGL(k_coarse_prolong<T>, dim3(grid_for(cz->n)), dim3(kBlock), 0, st_, cz->n, z,
-                       (const int *)cz->agg.p, (const double *)cz->c.p, st);
+int Krylov::coarse_apply(const Coarse *cz, const T *r, double *sc, const KState *st) {
+    const int g = (cz->m + kCoarseRows - 1) / kCoarseRows;
+    hipLaunchKernelGGL(k_coarse_restrict<T>, dim3(cz->m * kCoarseChunks), dim3(kBlock), 0, st_,
+                       (const int *)cz->aptr.p, (const int *)cz->arows.p, r, cz->spart.p, st);
+    hipLaunchKernelGGL(k_coarse_solve, dim3(g), dim3(kBlock), 0, st_, cz->m,
+                       (const double *)cz->E.p, (const double *)cz->spart.p, cz->c.p, sc, st);
+    return g;
 }
 
 template <class T>
@@ -2247,10 +2285,8 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
             hipLaunchKernelGGL(k_cg_rho<T>, dim3(G), dim3(kBlock), 0, st_, n, (const T *)w.r.p,
                                (const T *)w.z.p, state_.p, rpart, 0);
         }
-        if (cz) {  // r.(Z c) is one more partial of rho
-            coarse_apply<T>(cz, (const T *)w.r.p, w.z.p, rpart + nbk, stp);
-            ++nbk;
-        }
+        if (cz)  // the partials of r.(Z c) follow those of r.z
+            nbk += coarse_apply<T>(cz, (const T *)w.r.p, rpart + nbk, stp);
         return nbk;
     };
     auto precond_rho = [&]() {
@@ -2270,6 +2306,18 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
     // one CG iteration after the first pq launch (the same launches every
     // iteration: p and q are updated in place)
     double *pqpart = part_.p + kPartPQ;
+    CoarseTerm ct;
+    if (cz) ct = CoarseTerm{(const int *)cz->agg.p, (const double *)cz->c.p};
+    auto launch_pq = [&]() {
+        if (cz)
+            hipLaunchKernelGGL((k_cg_pq<T, true>), dim3(Gs), dim3(kBlock), 0, st_, A,
+                               (const T *)w.z.p, w.p0.p, w.q.p, x, (const T *)w.az.p, c0, rm, ct,
+                               (const KState *)state_.p, pqpart);
+        else
+            hipLaunchKernelGGL((k_cg_pq<T, false>), dim3(Gs), dim3(kBlock), 0, st_, A,
+                               (const T *)w.z.p, w.p0.p, w.q.p, x, (const T *)w.az.p, c0, rm, ct,
+                               (const KState *)state_.p, pqpart);
+    };
     auto rest_of_iteration = [&]() {
         if (jac) {
             CFD_REDUCE_LAUNCH_K((k_cg_update<true, T>), G, st_, n, w.r.p, (const T *)w.q.p,
@@ -2305,9 +2353,7 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
             for (int k = 0; k <= col->ncolour; ++k) key.push_back((uintptr_t)col->off[k]);
         auto record = [&](int iters) {
             for (int k = 0; k < iters; ++k) {
-                hipLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, A,
-                                   (const T *)w.z.p, w.p0.p, w.q.p, x, (const T *)w.az.p, c0, rm,
-                                   state_.p, pqpart);
+                launch_pq();
                 rest_of_iteration();
             }
         };
@@ -2329,16 +2375,20 @@ SolveStats Krylov::cg(const EllT<T> &A, const T *b, T *x, const cfd_solve_cfg &c
             // 0's launch has no x update to apply)
             const bool timed = prof && (issued + k) % kProfEvery == 1;
             if (timed) {  // the events are part of the launch: kernel start to end
-                hipExtLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, ev(2 * nev),
-                                      ev(2 * nev + 1), 0u, A, (const T *)w.z.p, w.p0.p, w.q.p,
-                                      x, (const T *)w.az.p, c0, rm, (const KState *)state_.p,
-                                      pqpart);
+                if (cz)
+                    hipExtLaunchKernelGGL(k_cg_pq<T, true>, dim3(Gs), dim3(kBlock), 0, st_,
+                                          ev(2 * nev), ev(2 * nev + 1), 0u, A, (const T *)w.z.p,
+                                          w.p0.p, w.q.p, x, (const T *)w.az.p, c0, rm, ct,
+                                          (const KState *)state_.p, pqpart);
+                else
+                    hipExtLaunchKernelGGL(k_cg_pq<T, false>, dim3(Gs), dim3(kBlock), 0, st_,
+                                          ev(2 * nev), ev(2 * nev + 1), 0u, A, (const T *)w.z.p,
+                                          w.p0.p, w.q.p, x, (const T *)w.az.p, c0, rm, ct,
+                                          (const KState *)state_.p, pqpart);
                 ev_iter.push_back(issued + k);
                 ++nev;
             } else {
-                hipLaunchKernelGGL(k_cg_pq<T>, dim3(Gs), dim3(kBlock), 0, st_, A,
-                                   (const T *)w.z.p, w.p0.p, w.q.p, x, (const T *)w.az.p, c0, rm,
-                                   state_.p, pqpart);
+                launch_pq();
             }
             rest_of_iteration();
         }

@@ -455,7 +455,21 @@ int NSSolver::set_coarse(int target) {
     coarse_target = target;
     if (target > 0) {
         std::vector<int> agg;
-        const int m = box_aggregates(hm.cell_x.data(), hm.ncel, target, agg);
+        int m;
+        if (!hm.bcolour.empty() && hm.bcolour[0] >= 0 && hm.ncel % kIluBlock == 0) {
+            // whole ILU blocks (the tetrahedra of one hexahedron) per aggregate,
+            // binned by the block's mean centroid
+            const int nb = hm.ncel / kIluBlock;
+            std::vector<double> bx(3 * (size_t)nb, 0.);
+            for (int i = 0; i < hm.ncel; ++i)
+                for (int d = 0; d < 3; ++d) bx[3 * (i / kIluBlock) + d] += hm.cell_x[3 * i + d] / kIluBlock;
+            std::vector<int> bagg;
+            m = box_aggregates(bx.data(), nb, target, bagg);
+            agg.resize(hm.ncel);
+            for (int i = 0; i < hm.ncel; ++i) agg[i] = bagg[i / kIluBlock];
+        } else {
+            m = box_aggregates(hm.cell_x.data(), hm.ncel, target, agg);
+        }
         coarse.build(agg, m, dmo.ell_col_host.data(), hm.ncel, dmo.dm.K, s);
     }
     return 0;
@@ -1168,7 +1182,9 @@ int cfd_ns_profile_read(const cfd_ns *h, double out[8]) {
     const bool blk = n.block_order && n.bcolour_ready;
     const Colouring &pc = blk ? n.bcolour : n.colour;  // ILU's order
     const int ppc = n.P.pres.precond;
-    const bool az = (blk ? (ppc == CFD_PC_ILU || ppc == CFD_PC_SSOR) && K <= 4
+    // (not with the two-level CG, whose coarse term is added after the sweep)
+    const bool az = n.coarse_target == 0 &&
+                    (blk ? (ppc == CFD_PC_ILU || ppc == CFD_PC_SSOR) && K <= 4
                          : ppc == CFD_PC_ILU && n.colour_ready && (K == 4 || K == 6)) &&
                     pc.ncolour > 1;
     long long nnz = 0, nnz_read = 0;

@@ -392,22 +392,23 @@ __global__ void k_cg_rho(int n, const T *__restrict__ r, const T *__restrict__ z
 }
 
 // ------------------------------------------------------- two-level CG --
-// E = Z^T A Z: one wave per nonzero entry, summing its value list (lane l
-// takes entries l, l + 64, ...; the wave sum combines the lanes in a fixed
-// order). E must be zero beforehand.
-__global__ void k_coarse_assemble(int ne, const int *__restrict__ eptr,
-                                  const int *__restrict__ epos, const int *__restrict__ ekey,
-                                  const double *__restrict__ diag,
-                                  const double *__restrict__ val, double *__restrict__ E) {
-    const int e = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
-    if (e >= ne) return;
-    double v = 0.;
-    for (int k = eptr[e] + lane; k < eptr[e + 1]; k += 64) {
+// E = Z^T A Z: one workgroup per nonzero entry, summing its value list
+// (thread t takes entries t, t + blockDim, ...; block_sum combines the
+// threads in a fixed order). E must be zero beforehand.
+__global__ void __launch_bounds__(1024) k_coarse_assemble(const int *__restrict__ eptr,
+                                                          const int *__restrict__ epos,
+                                                          const int *__restrict__ ekey,
+                                                          const double *__restrict__ diag,
+                                                          const double *__restrict__ val,
+                                                          double *__restrict__ E) {
+    const int e = blockIdx.x;
+    double v[1] = {0.};
+    for (int k = eptr[e] + (int)threadIdx.x; k < eptr[e + 1]; k += blockDim.x) {
         const int q = epos[k];
-        v += q >= 0 ? val[q] : diag[-q - 1];
+        v[0] += q >= 0 ? val[q] : diag[-q - 1];
     }
-    v = wave_sum(v);
-    if (lane == 0) E[ekey[e]] = v;
+    block_sum<1>(v);
+    if (threadIdx.x == 0) E[ekey[e]] = v[0];
 }
 
 __global__ void k_coarse_zero(int n, double *__restrict__ E) {
@@ -2227,9 +2228,10 @@ const Coarse *Krylov::coarse_setup(const Ell &A0, Colouring *col, const cfd_solv
     const int m = cz->m;
     hipLaunchKernelGGL(k_coarse_zero, dim3(grid_for((long long)m * m)), dim3(kBlock), 0, st_,
                        m * m, cz->E.p);
-    hipLaunchKernelGGL(k_coarse_assemble, dim3(grid_for((long long)cz->ne * 64)), dim3(kBlock),
-                       0, st_, cz->ne, (const int *)cz->eptr.p, (const int *)cz->epos.p,
-                       (const int *)cz->ekey.p, A0.diag, A0.val, cz->E.p);
+    if (cz->ne > 0)
+        hipLaunchKernelGGL(k_coarse_assemble, dim3(cz->ne), dim3(1024), 0, st_,
+                           (const int *)cz->eptr.p, (const int *)cz->epos.p,
+                           (const int *)cz->ekey.p, A0.diag, A0.val, cz->E.p);
     hipLaunchKernelGGL(k_coarse_invert, dim3(1), dim3(1024), 0, st_, m, cz->E.p);
     CFD_LAUNCH_CHECK();
     return cz;

@@ -460,7 +460,7 @@ __global__ void __launch_bounds__(1024) k_coarse_invert(int m, double *__restric
 // Partial sums of s = Z^T r: workgroup (a, j) sums the j-th of kCoarseChunks
 // equal chunks of aggregate a's rows into spart[a * kCoarseChunks + j] (a
 // fixed order)
-constexpr int kCoarseChunks = 8;
+constexpr int kCoarseChunks = 32;
 template <class T>
 __global__ void k_coarse_restrict(const int *__restrict__ aptr, const int *__restrict__ arows,
                                   const T *__restrict__ r, double *__restrict__ spart,

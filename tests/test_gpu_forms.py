@@ -4,7 +4,8 @@ Multi-RHS momentum: the three systems share every matrix read but keep
 their own recurrences and reductions, so u, v, w are bit-identical to three
 single solves. Mixed-precision pressure: FP32 inner CG, FP64 refinement; every
 solve ends at the same true FP64 relative residual as the FP64 solve, and the
-fields agree within that tolerance.
+fields agree within that tolerance. Two-level pressure CG: the same bar, fewer
+iterations, and graph replay bit for bit.
 """
 import numpy as np
 import pytest
